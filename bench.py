@@ -1,0 +1,176 @@
+"""Benchmark: stochastic LLG-Heun spin-steps/s at 4M atoms (FCC 100^3, 12 neighbours, 600 K),
+FP32 fused kernels (the headline value) beside FP64 fused and the plain FP64 kernels, with the
+HBM roofline of the Heun step and the CPU restatement of the reference timed on the host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--L 100] [--cpu-steps 3] [--no-cpu]
+
+One "step" is one Heun step (predictor + corrector) over all 4,000,000 spins, inputs resident
+in HBM.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "spin-steps/sec LLG-Heun at 4M atoms, FP32 vs FP64, % of HBM roofline"
+HBM_PEAK_GBS = 8000.0
+
+
+def b_alg(p: int, z: int, c_j: int = 0) -> int:
+    """Algorithmic bytes per spin per Heun step: read S, write S', read S', write S (3p each),
+    z int32 neighbour indices, c_J exchange reals per edge, one int32 material id."""
+    return 12 * p + z * (4 + c_j * p) + 4
+
+
+def run_gpu(system, precision, kernel, steps, warmup, seed=7, T=600.0, use_graph=False, block=0):
+    from vampire_amd.context import Context
+    spins = np.zeros((3, system.n))
+    spins[2] = 1.0
+    c = Context(system, precision=precision, kernel=kernel, seed=seed, block_size=block, device=0)
+    c.set_conditions(T)
+    c.set_spins(spins)
+    if warmup:
+        c.llg_heun(warmup)
+    c.synchronize()
+    t0 = time.perf_counter()
+    if use_graph:
+        chunk = 50
+        c.llg_heun_graph(chunk, steps // chunk)
+        rest = steps - chunk * (steps // chunk)
+        if rest:
+            c.llg_heun(rest)
+    else:
+        c.llg_heun(steps)
+    c.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = c.time_kernels(20)
+    return c, wall, kernel_ms
+
+
+def cpu_baseline(system, nsteps, threads):
+    from oracle import oracle as O
+    o = O.Oracle(system)
+    o.set_conditions(600.0)
+    spins = np.zeros((3, system.n))
+    spins[2] = 1.0
+    o.llg_heun(spins, 1, seed=7, threads=threads)       # first touch
+    t0 = time.perf_counter()
+    o.llg_heun(spins, nsteps, seed=7, threads=threads)
+    return system.n * nsteps / (time.perf_counter() - t0)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--L", type=int, default=100, help="FCC unit cells per edge (100 -> 4M atoms)")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="replay the fused step as a HIP graph")
+    ap.add_argument("--drift-steps", type=int, default=1000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (replicas and spatial shards "
+              "of this path would run one process per GPU, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from vampire_amd import system as S
+    from vampire_amd.context import FP32, FP64, FUSED, PLAIN
+
+    t_build = time.perf_counter()
+    system = S.config_c1(args.L)
+    t_build = time.perf_counter() - t_build
+    n = system.n
+    z = int(system.nbr_end[0] - system.nbr_start[0] + 1)
+    print(f"# system: FCC {args.L}^3, {n} atoms, z={z}, built in {t_build:.1f} s", file=sys.stderr, flush=True)
+
+    res = {}
+    for name, prec, kern in (("fp32_fused", FP32, FUSED), ("fp64_fused", FP64, FUSED), ("fp64_plain", FP64, PLAIN)):
+        c, wall, kms = run_gpu(system, prec, kern, args.steps, args.warmup, use_graph=args.graph and kern == FUSED)
+        res[name] = dict(wall=wall, kernel_ms=kms.tolist(), sps=n * args.steps / wall,
+                         ms_per_step=1e3 * wall / args.steps)
+        c.close()
+        print(f"# {name}: {res[name]['ms_per_step']:.4f} ms/step, {res[name]['sps']:.3e} spin-steps/s, "
+              f"kernels {['%.4f' % v for v in kms]} ms", file=sys.stderr, flush=True)
+
+    # FP32 vs FP64 drift after drift-steps from the same state and Philox stream
+    drift = {}
+    if args.drift_steps:
+        outs = {}
+        for prec in (FP64, FP32):
+            from vampire_amd.context import Context
+            c = Context(system, precision=prec, kernel=FUSED, seed=7, device=0)
+            c.set_conditions(600.0)
+            sp = np.zeros((3, n))
+            sp[2] = 1.0
+            c.set_spins(sp)
+            c.llg_heun(args.drift_steps)
+            outs[prec] = c.get_spins()
+            c.close()
+        d = np.abs(outs[FP32] - outs[FP64])
+        m64 = outs[FP64].mean(axis=1)
+        m32 = outs[FP32].mean(axis=1)
+        drift = dict(steps=args.drift_steps, max_abs_dm=float(d.max()),
+                     dM_over_Ms=float(np.linalg.norm(m32 - m64)), mz_fp64=float(m64[2]), mz_fp32=float(m32[2]))
+        print(f"# drift: {drift}", file=sys.stderr, flush=True)
+
+    head = res["fp32_fused"]
+    p32 = b_alg(4, z)
+    step_kernel_ms = float(sum(head["kernel_ms"]))
+    achieved = p32 * n / (step_kernel_ms * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="fused predictor + corrector (one Heun step)",
+                    bytes_per_spin_step=p32, kernel_ms=head["kernel_ms"])
+    cpu = None
+    if not args.no_cpu:
+        nthreads = min(16, os.cpu_count() or 1)
+        v1 = cpu_baseline(system, args.cpu_steps, 1)
+        vN = cpu_baseline(system, args.cpu_steps, nthreads)
+        cpu = dict(value=round(vN, 1), unit="spin-steps/s", cores=nthreads, kind="port",
+                   sample=f"{args.cpu_steps} Heun steps of the same 4M-atom FCC system, FP64, OpenMP over "
+                          f"{nthreads} threads (1 thread: {v1:.4g} spin-steps/s)",
+                   value_1thread=round(v1, 1))
+        print(f"# cpu: 1 thread {v1:.3e}, {nthreads} threads {vN:.3e} spin-steps/s", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC,
+        "value": round(head["sps"], 1),
+        "unit": "spin-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"FCC {args.L}^3 ferromagnet, {n} atoms, 12 NN scalar J, Ku along z, alpha=0.05, "
+                               "T=600 K, spins +z, Philox seed 7, stochastic LLG-Heun dt=1e-16 s",
+                   "atoms": n, "neighbours": z, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "fp64_fused": {"value": round(res["fp64_fused"]["sps"], 1), "ms_per_step": round(res["fp64_fused"]["ms_per_step"], 5),
+                       "kernel_ms": res["fp64_fused"]["kernel_ms"],
+                       "hbm_frac_b_alg": round(b_alg(8, z) * n / (sum(res["fp64_fused"]["kernel_ms"]) * 1e-3) / 1e9
+                                               / HBM_PEAK_GBS, 4)},
+        "fp64_plain": {"value": round(res["fp64_plain"]["sps"], 1), "ms_per_step": round(res["fp64_plain"]["ms_per_step"], 5),
+                       "kernel_ms": res["fp64_plain"]["kernel_ms"]},
+        "fp32_over_fp64_plain": round(head["sps"] / res["fp64_plain"]["sps"], 3),
+        "fp32_vs_fp64_drift": drift,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,249 @@
+/*
+ * vmx.h — C ABI of the MI355X spin-dynamics hot path (libvmx.so).
+ *
+ * The ABI is what VAMPIRE's accelerator boundary (namespace gpu, hdr/gpu.hpp:13-58)
+ * needs from a device backend.  gpu::initialize() (src/gpu/initialize.cpp:25) reads the
+ * atoms::, mp::, exchange:: and anisotropy:: globals and hands them to vmx_create();
+ * gpu::llg_heun() (src/gpu/llg_heun.cpp:23) becomes vmx_set_conditions() + vmx_llg_heun();
+ * gpu::stats::update/get/reset (src/gpu/statistics.cpp:25,45,59) become vmx_stats_*;
+ * gpu::config::synchronise / transfer_spin_positions_from_gpu_to_cpu (src/gpu/config.cpp:24,
+ * src/gpu/transfer.cpp:21) become vmx_get_spins().  The C++ shim that does this marshalling
+ * is vampire_amd/shim/gpu_shim.cpp.
+ *
+ * Conventions
+ *  - Every function returns VMX_OK (0) or a non-zero status; vmx_last_error() gives the text
+ *    (thread-local).  The shim maps a non-zero status to err::vexit(), as the reference does
+ *    for backend errors (src/gpu/initialize.cpp:33-40, src/cuda/cuda_utils.hpp:32-43).
+ *  - All pointers are host pointers unless a name ends in _dev.  Nothing passed in is retained
+ *    after the call returns: the library copies what it needs into device memory it owns
+ *    (allocated in vmx_create, released in vmx_destroy).
+ *  - One context is used from one host thread at a time.  Work is queued on the context's
+ *    stream; functions that return data to the host synchronise that stream.
+ *  - Units follow the reference: fields in tesla, exchange J_ij/mu_s (T), anisotropy K/mu_s
+ *    (T), reduced time step dt = dt_SI * gamma (src/main/initialise_variables.cpp:355-356).
+ */
+#ifndef VMX_H
+#define VMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------- */
+enum {
+    VMX_OK = 0,
+    VMX_ERR_ARG = 1,     /* invalid argument (size, index out of range, null pointer)          */
+    VMX_ERR_HIP = 2,     /* the HIP runtime reported an error                                    */
+    VMX_ERR_STATE = 3,   /* call not valid in the context's current state                        */
+    VMX_ERR_NOMEM = 4    /* device or host allocation failed                                     */
+};
+
+/* ---- exchange representations: values of exchange::exchange_t (hdr/exchange_types.hpp:24-27) */
+enum {
+    VMX_EXCHANGE_ISOTROPIC = 0,   /* one J per interaction id (zval_t, hdr/exchange.hpp:24-34)  */
+    VMX_EXCHANGE_VECTORIAL = 1,   /* diagonal Jxx,Jyy,Jzz (zvec_t, hdr/exchange.hpp:39-49)       */
+    VMX_EXCHANGE_TENSORIAL = 2    /* full 3x3 row-major (zten_t, hdr/exchange.hpp:54-67); DMI   */
+};
+
+/* ---- arithmetic of the spin and field arrays ------------------------------------------ */
+enum {
+    VMX_FP64 = 0,   /* spins, fields and integration in double                              */
+    VMX_FP32 = 1    /* spins, fields and integration in float; statistics and energies are
+                       always accumulated in double                                          */
+};
+
+/* ---- integrator kernel organisation ---------------------------------------------------- */
+enum {
+    VMX_KERNEL_PLAIN = 0,  /* five kernels per Heun step with field, noise and dS arrays in
+                              HBM, the dataflow of the reference step (src/simulate/LLGHeun.cpp
+                              :160-282, src/cuda/llg_heun.cu:121-208)                        */
+    VMX_KERNEL_FUSED = 1   /* two kernels per step: predictor, corrector; fields, noise and dS
+                              are recomputed in registers instead of stored                  */
+};
+
+/* ---- anisotropy terms (flags of anisotropy::internal, src/anisotropy/fields.cpp:31-68) ----- */
+#define VMX_ANISOTROPY_UNIAXIAL_SECOND_ORDER 0x1u  /* uniaxial_second_order.cpp:51-98          */
+#define VMX_ANISOTROPY_CUBIC_FOURTH_ORDER    0x2u  /* cubic_fourth_order.cpp:34-74             */
+#define VMX_ANISOTROPY_CUBIC_FOURTH_ROTATION 0x4u  /* cubic_fourth_order_rotation.cpp:34-100   */
+
+/* ---- Hamiltonian terms of the external field (src/simulate/fields.cpp:120-169) ---------- */
+#define VMX_FIELD_THERMAL       0x1u  /* sim::hamiltonian_simulation_flags[3] (fields.cpp:153)  */
+#define VMX_FIELD_APPLIED       0x2u  /* sim::hamiltonian_simulation_flags[2] (fields.cpp:156)  */
+#define VMX_FIELD_LOCAL_APPLIED 0x4u  /* sim::local_applied_field (fields.cpp:191)              */
+
+/* ---- per-material constants after mp::set_derived_parameters ---------------------------- */
+typedef struct vmx_material {
+    double alpha;                        /* Gilbert damping                          material.hpp  */
+    double gamma_rel;                    /* gyromagnetic ratio relative to gamma_SI               */
+    double mu_s_SI;                      /* moment (J/T)                                         */
+    double one_oneplusalpha_sq;          /* -gamma_rel/(1+alpha^2)   initialise_variables.cpp:423 */
+    double alpha_oneplusalpha_sq;        /* alpha*one_oneplusalpha_sq initialise_variables.cpp:424 */
+    double H_th_sigma;                   /* sqrt(2 alpha kB/(mu_s gamma_rel dt))             :431 */
+    double temperature_rescaling_alpha;  /* T<Tc: Tc (T/Tc)^alpha   (src/simulate/fields.cpp:269) */
+    double temperature_rescaling_Tc;
+    double ku2;                          /* Ku/mu_s (T)        anisotropy/initialize.cpp:59-62    */
+    double ku_vector[3];                 /* uniaxial easy axis                                   */
+    double kc4;                          /* Kc/mu_s (T)        anisotropy/initialize.cpp:74-77    */
+    double kc_vector1[3];                /* orthonormal cubic basis (initialize.cpp:102-143)      */
+    double kc_vector2[3];
+    double kc_vector3[3];
+    double applied_field_strength;       /* local applied field (fields.cpp:191-199)              */
+    double applied_field_unit_vector[3];
+} vmx_material;
+
+/* ---- the system handed over at initialisation ---------------------------------------- */
+typedef struct vmx_system {
+    int32_t num_atoms;                 /* atoms::num_atoms                                       */
+    int32_t num_materials;             /* mp::num_materials                                      */
+    const int32_t* type;               /* [num_atoms] atoms::type_array                           */
+    const double* moment;              /* [num_atoms] atoms::m_spin_array (mu_B), statistics      */
+    const int32_t* nbr_start;          /* [num_atoms] atoms::neighbour_list_start_index, inclusive */
+    const int32_t* nbr_end;            /* [num_atoms] atoms::neighbour_list_end_index, inclusive   */
+    int64_t num_nbr;                   /* length of the two arrays below                          */
+    const int32_t* nbr;                /* atoms::neighbour_list_array                             */
+    const int32_t* nbr_iid;            /* atoms::neighbour_interaction_type_array                 */
+    int32_t exchange_type;             /* VMX_EXCHANGE_*  (exchange::get_exchange_type())         */
+    int64_t num_interactions;          /* entries in the exchange table                           */
+    const double* exchange;            /* [num_interactions][1|3|9]: i/v/t_exchange_list          */
+    const vmx_material* materials;     /* [num_materials]                                         */
+    uint32_t anisotropy_flags;         /* VMX_ANISOTROPY_*                                       */
+    int32_t num_replicas;              /* >= 1; independent systems integrated together           */
+    const int32_t* replica_offset;     /* [num_replicas+1] atom ranges, or NULL when num_replicas=1 */
+} vmx_system;
+
+typedef struct vmx_options {
+    int32_t precision;    /* VMX_FP64 or VMX_FP32                                              */
+    int32_t kernel;       /* VMX_KERNEL_PLAIN or VMX_KERNEL_FUSED                              */
+    int32_t block_size;   /* threads per workgroup, 64..1024 in multiples of 64; 0 = 256       */
+    int32_t device;       /* HIP device ordinal; -1 = current device                            */
+    uint64_t seed;        /* Philox key for thermal noise and Monte Carlo moves                 */
+} vmx_options;
+
+typedef struct vmx_ctx vmx_ctx;
+
+/* Text of the last error on this thread ("" if none). */
+const char* vmx_last_error(void);
+
+/* Library version string. */
+const char* vmx_version(void);
+
+/* Number of visible HIP devices (0 when there is no GPU or no driver). */
+int vmx_device_count(int32_t* count);
+
+/* Builds a context: validates the system, compresses the neighbour list and exchange table,
+ * uploads them and allocates all device state.  Replaces vcuda::initialize
+ * (src/cuda/initialize.cu:43-151) behind gpu::initialize (src/gpu/initialize.cpp:25-43). */
+int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out);
+
+/* Releases all device memory of the context (gpu::finalize, src/gpu/finalize.cpp:23-32). */
+int vmx_destroy(vmx_ctx* ctx);
+
+/* Copies host spins (atoms::{x,y,z}_spin_array) to the device. */
+int vmx_set_spins(vmx_ctx* ctx, const double* sx, const double* sy, const double* sz);
+
+/* Copies device spins to the host (gpu::config::synchronise, src/gpu/config.cpp:24;
+ * gpu::transfer_spin_positions_from_gpu_to_cpu, src/gpu/transfer.cpp:21). */
+int vmx_get_spins(vmx_ctx* ctx, double* sx, double* sy, double* sz);
+
+/* Reduced integration step mp::dt and mp::half_dt (initialise_variables.cpp:355-356). */
+int vmx_set_time_step(vmx_ctx* ctx, double dt, double half_dt);
+
+/* Conditions read by the reference at every step (src/cuda/external_fields.cu:52-56):
+ * temperature sim::temperature, applied field H = sim::H_vec * sim::H_applied (T), and the
+ * VMX_FIELD_* flags.  replica = -1 sets every replica. */
+int vmx_set_conditions(vmx_ctx* ctx, int32_t replica, double temperature, const double H[3],
+                       uint32_t field_flags);
+
+/* Advances every replica by nsteps stochastic LLG-Heun steps (sim::LLG_Heun,
+ * src/simulate/LLGHeun.cpp:160-282; gpu::llg_heun, src/gpu/llg_heun.cpp:23).
+ * noise == NULL: thermal noise is generated on the device by Philox-4x32-10 keyed by
+ *   (seed, replica-local atom index, step, stream).
+ * noise != NULL: host array [nsteps][3][num_atoms] of standard normal deviates, consumed in
+ *   the reference's order (x for all atoms, then y, then z: src/simulate/fields.cpp:274-276);
+ *   used to feed the device exactly the stream a CPU run consumed. */
+int vmx_llg_heun(vmx_ctx* ctx, int32_t nsteps, const double* noise);
+
+/* Records nsteps Heun steps of the fused kernels into a HIP graph once and replays it
+ * nrepeat times (launch-bound small systems and replica batches).  Noise is Philox. */
+int vmx_llg_heun_graph(vmx_ctx* ctx, int32_t nsteps, int32_t nrepeat);
+
+/* Number of Heun steps taken so far (the Philox step counter). */
+int vmx_get_step(vmx_ctx* ctx, int64_t* step);
+int vmx_set_step(vmx_ctx* ctx, int64_t step);
+
+/* Checkerboard Metropolis: nsweeps sweeps; each sweep visits every atom once, one graph colour
+ * at a time (colours computed at the first call), with the adaptive Gaussian cone move and
+ * acceptance of montecarlo::mc_step (src/montecarlo/mc.cpp:29-127, mc_moves.cpp:71-87,158-162).
+ * Temperatures and fields come from vmx_set_conditions. */
+int vmx_mc_sweep(vmx_ctx* ctx, int32_t nsweeps);
+
+/* Adaptive cone width per replica (montecarlo::internal::adaptive_sigma) and the move and
+ * reject totals (sim::mc_statistics_moves / _reject). */
+int vmx_mc_get_state(vmx_ctx* ctx, double* sigma, double* moves, double* rejects);
+int vmx_mc_set_sigma(vmx_ctx* ctx, int32_t replica, double sigma);
+int vmx_mc_num_colours(vmx_ctx* ctx, int32_t* ncolours);
+
+/* ---- statistics (src/statistics, gpu::stats) ------------------------------------------ */
+/* Registers a magnetisation statistic with mask[num_atoms] in [0, mask_size)
+ * (magnetization_statistic_t::set_mask, src/statistics/magnetization.cpp:53-113); the last
+ * mask id collects non-magnetic atoms and is not normalised.  Returns a handle. */
+int vmx_stats_add_magnetization(vmx_ctx* ctx, int32_t mask_size, const int32_t* mask,
+                                int32_t* handle);
+
+/* Registers an energy statistic (energy_statistic_t, src/statistics/energy.cpp:52-258). */
+int vmx_stats_add_energy(vmx_ctx* ctx, int32_t mask_size, const int32_t* mask,
+                         int32_t* handle);
+
+/* Computes every registered statistic from the current device spins and adds it to the
+ * running means (stats::update, src/statistics/statistics.cpp:26-63). Accumulation is FP64. */
+int vmx_stats_update(vmx_ctx* ctx);
+
+/* Latest magnetisation [4*mask_size] (m_x,m_y,m_z unit vector, |M|/Ms per mask id: CPU
+ * semantics of magnetization.cpp:157-168), running sum [4*mask_size] and update counter
+ * (what gpu::stats::get hands to set_magnetization, magnetization.cpp:194-208). */
+int vmx_stats_get_magnetization(vmx_ctx* ctx, int32_t handle, double* magnetization,
+                                double* mean_sum, int64_t* counter);
+
+/* Latest energies [5][mask_size] in order total, exchange, anisotropy, applied-field,
+ * magnetostatic (tesla x mu_B, energy.cpp:141-258), running sums [5][mask_size], counter. */
+int vmx_stats_get_energy(vmx_ctx* ctx, int32_t handle, double* energy, double* mean_sum,
+                         int64_t* counter);
+
+/* Zeroes running sums and counters of every statistic (gpu::stats::reset). */
+int vmx_stats_reset(vmx_ctx* ctx);
+
+/* ---- diagnostics ---------------------------------------------------------------------- */
+/* Spin-dependent field H_spin(S) (exchange + anisotropy; fields.cpp:67-118) and external field
+ * (thermal from `noise` [3][N] or Philox at the current step, plus applied; fields.cpp
+ * :120-289) of the current spins, copied to host [3][num_atoms] arrays (NULL to skip). */
+int vmx_compute_fields(vmx_ctx* ctx, const double* noise, double* h_spin, double* h_ext);
+
+/* The Philox standard normals the device uses for thermal noise at `step`, [3][num_atoms]. */
+int vmx_thermal_noise(vmx_ctx* ctx, int64_t step, double* noise);
+
+/* Blocks until all queued work of the context has finished. */
+int vmx_synchronize(vmx_ctx* ctx);
+
+/* Average device time (ms) of the last `vmx_llg_heun` call's kernels per Heun step, measured
+ * with HIP events on the context's stream (0 if not measured). */
+int vmx_last_step_time_ms(vmx_ctx* ctx, double* ms);
+
+/* Runs nsteps Heun steps (Philox noise) with HIP events around every kernel launch and returns
+ * the average device time (ms) of each kernel of the step in `avg_ms` (fused: predictor,
+ * corrector; plain: spin fields of S, external fields, predictor, spin fields of S', corrector)
+ * and their number in `nkernels`.  `avg_ms` must hold 5 values. */
+int vmx_time_kernels(vmx_ctx* ctx, int32_t nsteps, double* avg_ms, int32_t* nkernels);
+
+/* Chooses the workgroup size (and for FP32 the dS-storage variant) by timing a few steps of
+ * each candidate on this system; the result applies to later calls.  The spins and step
+ * counter are restored afterwards.  Writes the chosen block size. */
+int vmx_autotune(vmx_ctx* ctx, int32_t steps_per_trial, int32_t* chosen_block_size);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VMX_H */

@@ -1,0 +1,245 @@
+// ref_harness.cpp — drives the reference VAMPIRE CPU code (compiled from its own sources by
+// oracle/ref/Makefile) to produce golden vectors for the oracle.  Development container only.
+//
+//   ref_harness <rundir> dump                       system after cs::create + sim::run set-up
+//   ref_harness <rundir> llg <nsteps> <every>       sim::LLG_Heun with injected noise
+//   ref_harness <rundir> mc <nsweeps> <T> <every>   montecarlo::mc_step, reference RNG
+//
+// <rundir> holds `input` (and the material file it names).  Inputs read from <rundir>:
+//   spins_in.bin  optional: 3*N doubles (x block, y block, z block) replacing the initial spins
+//   noise.bin     llg: nsteps*3*N doubles served by mtrandom::gaussian in call order
+// Outputs written to <rundir>: system.bin, fields.bin, traj.bin, stats.bin, mc.bin, meta.txt.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+#include <unistd.h>
+
+#include "anisotropy.hpp"
+#include "atoms.hpp"
+#include "create.hpp"
+#include "exchange.hpp"
+#include "material.hpp"
+#include "montecarlo.hpp"
+#include "random.hpp"
+#include "sim.hpp"
+#include "stats.hpp"
+#include "vio.hpp"
+#include "vmpi.hpp"
+
+// fields.cpp entry points (declared there, not in a header)
+int calculate_spin_fields(const int, const int);
+int calculate_external_fields(const int, const int);
+
+namespace mtrandom {
+double mt_ziggurat_gaussian();  // the reference Ziggurat, renamed at compile time
+}
+
+namespace {
+const double* g_noise = nullptr;
+size_t g_noise_len = 0, g_noise_pos = 0;
+}  // namespace
+
+namespace mtrandom {
+// Serves the injected stream while one is installed, the reference generator otherwise.
+double gaussian() {
+    if (g_noise) {
+        if (g_noise_pos >= g_noise_len) {
+            std::cerr << "ref_harness: injected noise exhausted" << std::endl;
+            std::exit(3);
+        }
+        return g_noise[g_noise_pos++];
+    }
+    return mt_ziggurat_gaussian();
+}
+}  // namespace mtrandom
+
+namespace {
+
+template <typename T>
+void put(std::ofstream& f, const std::vector<T>& v) {
+    f.write(reinterpret_cast<const char*>(v.data()), v.size() * sizeof(T));
+}
+
+std::vector<double> read_doubles(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    std::vector<double> v;
+    if (!f) return v;
+    f.seekg(0, std::ios::end);
+    const size_t bytes = f.tellg();
+    f.seekg(0);
+    v.resize(bytes / sizeof(double));
+    f.read(reinterpret_cast<char*>(v.data()), bytes);
+    return v;
+}
+
+void setup() {
+    mp::initialise("input");
+    cs::create();
+    // the set-up part of sim::run (src/simulate/sim.cpp:236-267)
+    sim::initialize(mp::num_materials);
+    montecarlo::initialize();
+    anisotropy::initialize(atoms::num_atoms, atoms::type_array, mp::mu_s_array);
+    mtrandom::grnd.seed(vmpi::parallel_rng_seed(mtrandom::integration_seed));
+    std::vector<bool> nm(mp::num_materials, false);
+    stats::initialize(atoms::num_atoms, mp::num_materials, atoms::m_spin_array, atoms::type_array,
+                      atoms::category_array, nm);
+    std::vector<double> sp = read_doubles("spins_in.bin");
+    const int n = atoms::num_atoms;
+    if (sp.size() == 3 * (size_t)n) {
+        for (int i = 0; i < n; ++i) {
+            atoms::x_spin_array[i] = sp[i];
+            atoms::y_spin_array[i] = sp[n + i];
+            atoms::z_spin_array[i] = sp[2 * n + i];
+        }
+    }
+}
+
+void dump_system() {
+    const int n = atoms::num_atoms;
+    std::ofstream meta("meta.txt");
+    meta.precision(17);
+    meta << "num_atoms " << n << "\n";
+    meta << "num_nbr " << atoms::neighbour_list_array.size() << "\n";
+    meta << "exchange_type " << exchange::get_exchange_type() << "\n";
+    meta << "num_materials " << mp::num_materials << "\n";
+    meta << "dt " << mp::dt << "\nhalf_dt " << mp::half_dt << "\n";
+    for (int m = 0; m < mp::num_materials; ++m) {
+        const mp::materials_t& a = mp::material[m];
+        meta << "material " << m << " alpha " << a.alpha << " gamma_rel " << a.gamma_rel << " mu_s_SI " << a.mu_s_SI
+             << " one_oneplusalpha_sq " << a.one_oneplusalpha_sq << " alpha_oneplusalpha_sq "
+             << a.alpha_oneplusalpha_sq << " H_th_sigma " << a.H_th_sigma << " ku2 "
+             << anisotropy::get_ku2(m) << " mu_s_muB " << mp::mu_s_array[m] << "\n";
+    }
+    std::ofstream f("system.bin", std::ios::binary);
+    put(f, atoms::type_array);
+    put(f, atoms::neighbour_list_start_index);
+    put(f, atoms::neighbour_list_end_index);
+    put(f, atoms::neighbour_list_array);
+    put(f, atoms::neighbour_interaction_type_array);
+    put(f, atoms::m_spin_array);
+    put(f, atoms::x_coord_array);
+    put(f, atoms::y_coord_array);
+    put(f, atoms::z_coord_array);
+    put(f, atoms::x_spin_array);
+    put(f, atoms::y_spin_array);
+    put(f, atoms::z_spin_array);
+    const int et = exchange::get_exchange_type();
+    std::vector<double> J;
+    if (et == 0) for (auto& z : atoms::i_exchange_list) J.push_back(z.Jij);
+    if (et == 1) for (auto& z : atoms::v_exchange_list) for (int k = 0; k < 3; ++k) J.push_back(z.Jij[k]);
+    if (et == 2) for (auto& z : atoms::t_exchange_list) for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) J.push_back(z.Jij[a][b]);
+    meta << "num_interactions " << J.size() / (et == 0 ? 1 : (et == 1 ? 3 : 9)) << "\n";
+    put(f, J);
+}
+
+void dump_stats(std::ofstream& f) {
+    stats::update(atoms::x_spin_array, atoms::y_spin_array, atoms::z_spin_array, atoms::m_spin_array,
+                  atoms::type_array, sim::temperature);
+    std::vector<double> v;
+    const std::vector<double>& sm = stats::system_magnetization.get_magnetization();
+    v.insert(v.end(), sm.begin(), sm.end());
+    const std::vector<double>& mm = stats::material_magnetization.get_magnetization();
+    v.insert(v.end(), mm.begin(), mm.end());
+    for (auto* e : {&stats::system_energy, &stats::material_energy}) {
+        if (!e->is_initialized()) continue;
+        typedef const std::vector<double>& (stats::energy_statistic_t::*getter_t)();
+        const getter_t getters[5] = {&stats::energy_statistic_t::get_total_energy,
+                                     &stats::energy_statistic_t::get_exchange_energy,
+                                     &stats::energy_statistic_t::get_anisotropy_energy,
+                                     &stats::energy_statistic_t::get_applied_field_energy,
+                                     &stats::energy_statistic_t::get_magnetostatic_energy};
+        for (int g = 0; g < 5; ++g) {
+            const std::vector<double>& ev = (e->*getters[g])();
+            v.insert(v.end(), ev.begin(), ev.end());
+        }
+    }
+    put(f, v);
+}
+
+void dump_fields(std::ofstream& f) {
+    calculate_spin_fields(0, atoms::num_atoms);
+    calculate_external_fields(0, atoms::num_atoms);
+    put(f, atoms::x_total_spin_field_array);
+    put(f, atoms::y_total_spin_field_array);
+    put(f, atoms::z_total_spin_field_array);
+    put(f, atoms::x_total_external_field_array);
+    put(f, atoms::y_total_external_field_array);
+    put(f, atoms::z_total_external_field_array);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 3) {
+        std::cerr << "usage: ref_harness <rundir> dump|llg|mc ..." << std::endl;
+        return 2;
+    }
+    if (chdir(argv[1]) != 0) { std::perror("chdir"); return 2; }
+    vmpi::initialise(argc, argv);
+    vout::zLogTsInit(std::string("ref_harness"));
+    const std::string cmd = argv[2];
+    setup();
+    const int n = atoms::num_atoms;
+    if (cmd == "dump") {
+        dump_system();
+        std::ofstream st("stats.bin", std::ios::binary);
+        dump_stats(st);
+        return 0;
+    }
+    if (cmd == "llg") {
+        const int nsteps = std::atoi(argv[3]);
+        const int every = std::atoi(argv[4]);
+        std::vector<double> noise = read_doubles("noise.bin");
+        // one extra 3N block at the front feeds the field dump before the first step
+        if (noise.size() != (size_t)(nsteps + 1) * 3 * n) {
+            std::cerr << "noise.bin must hold (nsteps+1)*3*N doubles" << std::endl;
+            return 2;
+        }
+        g_noise = noise.data();
+        g_noise_len = noise.size();
+        {
+            std::ofstream ff("fields.bin", std::ios::binary);
+            dump_fields(ff);
+        }
+        std::ofstream tf("traj.bin", std::ios::binary);
+        for (int s = 1; s <= nsteps; ++s) {
+            sim::LLG_Heun();
+            if (s % every == 0) {
+                put(tf, atoms::x_spin_array);
+                put(tf, atoms::y_spin_array);
+                put(tf, atoms::z_spin_array);
+            }
+        }
+        if (g_noise_pos != g_noise_len) {
+            std::cerr << "noise consumed " << g_noise_pos << " of " << g_noise_len << std::endl;
+            return 4;
+        }
+        g_noise = nullptr;
+        std::ofstream st("stats.bin", std::ios::binary);
+        dump_stats(st);
+        return 0;
+    }
+    if (cmd == "mc") {
+        const int nsweeps = std::atoi(argv[3]);
+        sim::temperature = std::atof(argv[4]);
+        const int every = std::atoi(argv[5]);
+        std::ofstream mf("mc.bin", std::ios::binary);
+        for (int s = 1; s <= nsweeps; ++s) {
+            montecarlo::mc_step(atoms::x_spin_array, atoms::y_spin_array, atoms::z_spin_array, atoms::num_atoms,
+                                atoms::type_array);
+            if (s % every == 0) {
+                stats::system_magnetization.calculate_magnetization(atoms::x_spin_array, atoms::y_spin_array,
+                                                                    atoms::z_spin_array, atoms::m_spin_array);
+                const std::vector<double>& m = stats::system_magnetization.get_magnetization();
+                put(mf, m);
+            }
+        }
+        return 0;
+    }
+    std::cerr << "unknown command " << cmd << std::endl;
+    return 2;
+}

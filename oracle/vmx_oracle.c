@@ -1,0 +1,601 @@
+/*
+ * vmx_oracle.c — CPU oracle for the LLG-Heun / Metropolis hot path.
+ *
+ * TEST INFRASTRUCTURE.  This file restates, in plain C and in the reference's own loop and
+ * operation order, the VAMPIRE CPU functions the device path replaces.  It is used only by
+ * tests/ (parity checks), __graft_entry__.smoke() (as the checker) and bench.py (the timed CPU
+ * baseline).  The product path (vampire_amd/, libvmx.so) never calls it.
+ *
+ * Built by __graft_entry__.build() as oracle/liboracle.so:
+ *   gcc -O3 -std=c99 -ffp-contract=off -fopenmp -fPIC -shared vmx_oracle.c -lm
+ * (no FMA contraction: every product and sum rounds where the reference's C++ rounds).
+ *
+ * Reference functions restated (file:line in the VAMPIRE tree):
+ *   or_spin_fields        sim::calculate_spin_fields        src/simulate/fields.cpp:67-118
+ *                         exchange::internal::exchange_fields src/exchange/exchange_fields.cpp:29-169
+ *                         anisotropy::fields               src/anisotropy/fields.cpp:31-68,
+ *                           uniaxial_second_order.cpp:51-98, cubic_fourth_order.cpp:34-74,
+ *                           cubic_fourth_order_rotation.cpp:34-100
+ *   or_external_fields    sim::calculate_external_fields   src/simulate/fields.cpp:120-169
+ *                         calculate_thermal_fields         fields.cpp:246-289
+ *                         calculate_applied_fields         fields.cpp:171-244
+ *   or_llg_heun           sim::LLG_Heun                    src/simulate/LLGHeun.cpp:160-282
+ *   or_magnetization      magnetization_statistic_t::calculate_magnetization
+ *                                                          src/statistics/magnetization.cpp:134-180
+ *   or_energy             energy_statistic_t::calculate    src/statistics/energy.cpp:141-258
+ *   or_spin_energy        sim::calculate_spin_energy       src/simulate/energy.cpp:147-174
+ *                         exchange::single_spin_energy     src/exchange/energy.cpp:27-140
+ *                         anisotropy::single_spin_energy   src/anisotropy/energy.cpp:31-50
+ *   or_mc_random_site     montecarlo::mc_step              src/montecarlo/mc.cpp:29-127,
+ *                                                          mc_moves.cpp:71-87,158-162
+ *   or_mc_checkerboard    the colour-ordered variant of mc_step run by the device
+ *
+ * Noise: the reference draws MT19937 + Ziggurat normals (src/random/random.cpp:219-244), x for all
+ * atoms, then y, then z (fields.cpp:274-276).  The oracle takes either an explicit noise array in
+ * that order or generates the device's documented stream: Philox-4x32-10 (Salmon et al., SC'11)
+ * keyed by the seed, counter (atom, step_lo, step_hi, stream), Box-Muller on word pairs.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct {
+    double alpha, gamma_rel, mu_s_SI;
+    double one_oneplusalpha_sq, alpha_oneplusalpha_sq, H_th_sigma;
+    double temperature_rescaling_alpha, temperature_rescaling_Tc;
+    double ku2, ku_vector[3];
+    double kc4, kc_vector1[3], kc_vector2[3], kc_vector3[3];
+    double applied_field_strength, applied_field_unit_vector[3];
+} or_material;
+
+typedef struct {
+    int n, nmat;
+    const int* type;
+    const double* moment;
+    const int* nbr_start;        /* inclusive */
+    const int* nbr_end;          /* inclusive */
+    const int* nbr;
+    const int* iid;
+    int exchange_type;           /* 0 isotropic, 1 vectorial, 2 tensorial */
+    const double* exchange;      /* [num_interactions][1|3|9] */
+    const or_material* mat;
+    unsigned ani;                /* 1 uniaxial2, 2 cubic4, 4 cubic4 rotated */
+} or_sys;
+
+typedef struct {
+    double temperature;
+    double H[3];                 /* sim::H_vec * sim::H_applied */
+    unsigned fld;                /* 1 thermal, 2 applied, 4 local applied */
+    double dt, half_dt;
+} or_cond;
+
+/* ---- Philox-4x32-10 and Box-Muller ------------------------------------------------------ */
+static uint32_t mulhi32(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
+
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = mulhi32(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        const uint32_t hi1 = mulhi32(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static void philox_at(uint64_t seed, uint32_t local, uint64_t step, uint32_t stream, uint32_t out[4]) {
+    const uint32_t ctr[4] = {local, (uint32_t)step, (uint32_t)(step >> 32), stream};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    or_philox4x32_10(ctr, key, out);
+}
+
+/* three normals from one block: (x0,x1) and (x2,x3) Box-Muller, uniforms (k+0.5)/2^32 */
+void or_normals3(uint64_t seed, int local, uint64_t step, uint32_t stream, double g[3]) {
+    uint32_t u[4];
+    philox_at(seed, (uint32_t)local, step, stream, u);
+    const double s = 2.3283064365386963e-10, twopi = 6.283185307179586;
+    const double u1 = ((double)u[0] + 0.5) * s, u2 = ((double)u[1] + 0.5) * s;
+    const double u3 = ((double)u[2] + 0.5) * s, u4 = ((double)u[3] + 0.5) * s;
+    const double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+    g[0] = r1 * cos(twopi * u2);
+    g[1] = r1 * sin(twopi * u2);
+    g[2] = r2 * cos(twopi * u4);
+}
+
+/* [3][n] thermal normals of one step in the reference's consumption layout */
+void or_thermal_noise(uint64_t seed, int n, uint64_t step, double* out) {
+    for (int i = 0; i < n; ++i) {
+        double g[3];
+        or_normals3(seed, i, step, 0u, g);
+        out[i] = g[0];
+        out[(size_t)n + i] = g[1];
+        out[2 * (size_t)n + i] = g[2];
+    }
+}
+
+/* ---- fields ------------------------------------------------------------------------------- */
+static void set_threads(int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+}
+
+void or_spin_fields(const or_sys* s, const double* sx, const double* sy, const double* sz,
+                    double* hx_out, double* hy_out, double* hz_out) {
+    const int n = s->n;
+    /* fields.cpp:79-81 — zero */
+    for (int atom = 0; atom < n; ++atom) { hx_out[atom] = 0.0; hy_out[atom] = 0.0; hz_out[atom] = 0.0; }
+    /* exchange_fields.cpp:52-162 */
+#pragma omp parallel for schedule(static)
+    for (int atom = 0; atom < n; ++atom) {
+        double hx = 0.0, hy = 0.0, hz = 0.0;
+        const int start = s->nbr_start[atom];
+        const int end = s->nbr_end[atom] + 1;
+        for (int nn = start; nn < end; ++nn) {
+            const int natom = s->nbr[nn];
+            const int iid = s->iid[nn];
+            if (s->exchange_type == 0) {
+                const double Jij = s->exchange[iid];
+                hx += Jij * sx[natom];
+                hy += Jij * sy[natom];
+                hz += Jij * sz[natom];
+            } else if (s->exchange_type == 1) {
+                const double* J = s->exchange + 3 * (size_t)iid;
+                hx += J[0] * sx[natom];
+                hy += J[1] * sy[natom];
+                hz += J[2] * sz[natom];
+            } else {
+                const double* J = s->exchange + 9 * (size_t)iid;
+                const double S[3] = {sx[natom], sy[natom], sz[natom]};
+                hx += (J[0] * S[0] + J[1] * S[1] + J[2] * S[2]);
+                hy += (J[3] * S[0] + J[4] * S[1] + J[5] * S[2]);
+                hz += (J[6] * S[0] + J[7] * S[1] + J[8] * S[2]);
+            }
+        }
+        hx_out[atom] += hx;
+        hy_out[atom] += hy;
+        hz_out[atom] += hz;
+    }
+    /* anisotropy terms, one pass each, in the order of anisotropy/fields.cpp:44-59 */
+    if (s->ani & 1u) {
+        const double scale = 2.0;
+        for (int atom = 0; atom < n; ++atom) {
+            const or_material* m = &s->mat[s->type[atom]];
+            const double ex = m->ku_vector[0], ey = m->ku_vector[1], ez = m->ku_vector[2];
+            const double sdote = (sx[atom] * ex + sy[atom] * ey + sz[atom] * ez);
+            const double k2 = scale * m->ku2 * sdote;
+            hx_out[atom] += ex * k2;
+            hy_out[atom] += ey * k2;
+            hz_out[atom] += ez * k2;
+        }
+    }
+    if (s->ani & 2u) {
+        const double scale = 0.5 * 4.0;
+        for (int atom = 0; atom < n; ++atom) {
+            const or_material* m = &s->mat[s->type[atom]];
+            const double k4 = scale * m->kc4;
+            const double x = sx[atom], y = sy[atom], z = sz[atom];
+            hx_out[atom] += x * x * x * k4;
+            hy_out[atom] += y * y * y * k4;
+            hz_out[atom] += z * z * z * k4;
+        }
+    }
+    if (s->ani & 4u) {
+        const double scale = 0.5 * 4.0;
+        for (int atom = 0; atom < n; ++atom) {
+            const or_material* m = &s->mat[s->type[atom]];
+            const double* e1 = m->kc_vector1;
+            const double* e2 = m->kc_vector2;
+            const double* e3 = m->kc_vector3;
+            const double k4 = scale * m->kc4;
+            const double x = sx[atom], y = sy[atom], z = sz[atom];
+            const double d1 = x * e1[0] + y * e1[1] + z * e1[2];
+            const double d2 = x * e2[0] + y * e2[1] + z * e2[2];
+            const double d3 = x * e3[0] + y * e3[1] + z * e3[2];
+            const double d1c = d1 * d1 * d1, d2c = d2 * d2 * d2, d3c = d3 * d3 * d3;
+            hx_out[atom] += k4 * (d1c * e1[0] + d2c * e2[0] + d3c * e3[0]);
+            hy_out[atom] += k4 * (d1c * e1[1] + d2c * e2[1] + d3c * e3[1]);
+            hz_out[atom] += k4 * (d1c * e1[2] + d2c * e2[2] + d3c * e3[2]);
+        }
+    }
+}
+
+/* per-material thermal width sigma_prefactor (fields.cpp:258-272) and applied field (:181-210) */
+static void material_conditions(const or_sys* s, const or_cond* c, double* sigma, double* happ) {
+    for (int m = 0; m < s->nmat; ++m) {
+        const or_material* mt = &s->mat[m];
+        const double temperature = c->temperature;
+        const double alpha = mt->temperature_rescaling_alpha;
+        const double Tc = mt->temperature_rescaling_Tc;
+        const double rescaled_temperature = temperature < Tc ? Tc * pow(temperature / Tc, alpha) : temperature;
+        const double sqrt_T = sqrt(rescaled_temperature);
+        sigma[m] = sqrt_T * mt->H_th_sigma;
+        for (int k = 0; k < 3; ++k) {
+            if (c->fld & 4u)
+                happ[3 * m + k] = c->H[k] + mt->applied_field_strength * mt->applied_field_unit_vector[k];
+            else
+                happ[3 * m + k] = c->H[k];
+        }
+    }
+}
+
+void or_external_fields(const or_sys* s, const or_cond* c, const double* noise, uint64_t seed, uint64_t step,
+                        double* ex, double* ey, double* ez) {
+    const int n = s->n;
+    double* sigma = (double*)malloc(sizeof(double) * (size_t)s->nmat);
+    double* happ = (double*)malloc(sizeof(double) * 3 * (size_t)s->nmat);
+    material_conditions(s, c, sigma, happ);
+    for (int atom = 0; atom < n; ++atom) { ex[atom] = 0.0; ey[atom] = 0.0; ez[atom] = 0.0; }
+    if (c->fld & 1u) {
+        if (noise) {
+            for (int atom = 0; atom < n; ++atom) {
+                ex[atom] = noise[atom];
+                ey[atom] = noise[(size_t)n + atom];
+                ez[atom] = noise[2 * (size_t)n + atom];
+            }
+        } else {
+#pragma omp parallel for schedule(static)
+            for (int atom = 0; atom < n; ++atom) {
+                double g[3];
+                or_normals3(seed, atom, step, 0u, g);
+                ex[atom] = g[0]; ey[atom] = g[1]; ez[atom] = g[2];
+            }
+        }
+        for (int atom = 0; atom < n; ++atom) {
+            const double H_th_sigma = sigma[s->type[atom]];
+            ex[atom] *= H_th_sigma;
+            ey[atom] *= H_th_sigma;
+            ez[atom] *= H_th_sigma;
+        }
+    }
+    if (c->fld & 2u) {
+        for (int atom = 0; atom < n; ++atom) {
+            const int m = s->type[atom];
+            ex[atom] += happ[3 * m + 0];
+            ey[atom] += happ[3 * m + 1];
+            ez[atom] += happ[3 * m + 2];
+        }
+    }
+    free(sigma);
+    free(happ);
+}
+
+/* ---- integrator: sim::LLG_Heun ------------------------------------------------------------ */
+typedef struct {
+    double *x0, *y0, *z0, *xs, *ys, *zs, *xe, *ye, *ze, *xh, *yh, *zh;
+    double *hsx, *hsy, *hsz, *hex, *hey, *hez;
+} heun_arrays;
+
+static int alloc_arrays(heun_arrays* a, int n) {
+    double** p[] = {&a->x0, &a->y0, &a->z0, &a->xs, &a->ys, &a->zs, &a->xe, &a->ye, &a->ze, &a->xh, &a->yh,
+                    &a->zh, &a->hsx, &a->hsy, &a->hsz, &a->hex, &a->hey, &a->hez};
+    for (size_t k = 0; k < sizeof p / sizeof p[0]; ++k) {
+        *p[k] = (double*)calloc((size_t)n, sizeof(double));
+        if (!*p[k]) return -1;
+    }
+    return 0;
+}
+
+static void free_arrays(heun_arrays* a) {
+    double* p[] = {a->x0, a->y0, a->z0, a->xs, a->ys, a->zs, a->xe, a->ye, a->ze, a->xh, a->yh,
+                   a->zh, a->hsx, a->hsy, a->hsz, a->hex, a->hey, a->hez};
+    for (size_t k = 0; k < sizeof p / sizeof p[0]; ++k) free(p[k]);
+}
+
+static void heun_step(const or_sys* s, const or_cond* c, double* sx, double* sy, double* sz, const double* noise,
+                      uint64_t seed, uint64_t step, heun_arrays* a) {
+    const int num_atoms = s->n;
+    for (int atom = 0; atom < num_atoms; atom++) {             /* LLGHeun.cpp:177-181 */
+        a->x0[atom] = sx[atom]; a->y0[atom] = sy[atom]; a->z0[atom] = sz[atom];
+    }
+    or_spin_fields(s, sx, sy, sz, a->hsx, a->hsy, a->hsz);      /* :184 */
+    or_external_fields(s, c, noise, seed, step, a->hex, a->hey, a->hez);   /* :185 */
+#pragma omp parallel for schedule(static)
+    for (int atom = 0; atom < num_atoms; atom++) {             /* :188-226 */
+        const or_material* m = &s->mat[s->type[atom]];
+        const double one_oneplusalpha_sq = m->one_oneplusalpha_sq;
+        const double alpha_oneplusalpha_sq = m->alpha_oneplusalpha_sq;
+        const double S[3] = {sx[atom], sy[atom], sz[atom]};
+        const double H[3] = {a->hsx[atom] + a->hex[atom], a->hsy[atom] + a->hey[atom], a->hsz[atom] + a->hez[atom]};
+        double xyz[3], S_new[3];
+        xyz[0] = (one_oneplusalpha_sq) * (S[1] * H[2] - S[2] * H[1]) +
+                 (alpha_oneplusalpha_sq) * (S[1] * (S[0] * H[1] - S[1] * H[0]) - S[2] * (S[2] * H[0] - S[0] * H[2]));
+        xyz[1] = (one_oneplusalpha_sq) * (S[2] * H[0] - S[0] * H[2]) +
+                 (alpha_oneplusalpha_sq) * (S[2] * (S[1] * H[2] - S[2] * H[1]) - S[0] * (S[0] * H[1] - S[1] * H[0]));
+        xyz[2] = (one_oneplusalpha_sq) * (S[0] * H[1] - S[1] * H[0]) +
+                 (alpha_oneplusalpha_sq) * (S[0] * (S[2] * H[0] - S[0] * H[2]) - S[1] * (S[1] * H[2] - S[2] * H[1]));
+        a->xe[atom] = xyz[0]; a->ye[atom] = xyz[1]; a->ze[atom] = xyz[2];
+        S_new[0] = S[0] + xyz[0] * c->dt;
+        S_new[1] = S[1] + xyz[1] * c->dt;
+        S_new[2] = S[2] + xyz[2] * c->dt;
+        const double mod_S = 1.0 / sqrt(S_new[0] * S_new[0] + S_new[1] * S_new[1] + S_new[2] * S_new[2]);
+        a->xs[atom] = S_new[0] * mod_S;
+        a->ys[atom] = S_new[1] * mod_S;
+        a->zs[atom] = S_new[2] * mod_S;
+    }
+    for (int atom = 0; atom < num_atoms; atom++) {             /* :229-233 */
+        sx[atom] = a->xs[atom]; sy[atom] = a->ys[atom]; sz[atom] = a->zs[atom];
+    }
+    or_spin_fields(s, sx, sy, sz, a->hsx, a->hsy, a->hsz);      /* :236 */
+#pragma omp parallel for schedule(static)
+    for (int atom = 0; atom < num_atoms; atom++) {             /* :239-260 */
+        const or_material* m = &s->mat[s->type[atom]];
+        const double one_oneplusalpha_sq = m->one_oneplusalpha_sq;
+        const double alpha_oneplusalpha_sq = m->alpha_oneplusalpha_sq;
+        const double S[3] = {sx[atom], sy[atom], sz[atom]};
+        const double H[3] = {a->hsx[atom] + a->hex[atom], a->hsy[atom] + a->hey[atom], a->hsz[atom] + a->hez[atom]};
+        a->xh[atom] = (one_oneplusalpha_sq) * (S[1] * H[2] - S[2] * H[1]) +
+                      (alpha_oneplusalpha_sq) * (S[1] * (S[0] * H[1] - S[1] * H[0]) - S[2] * (S[2] * H[0] - S[0] * H[2]));
+        a->yh[atom] = (one_oneplusalpha_sq) * (S[2] * H[0] - S[0] * H[2]) +
+                      (alpha_oneplusalpha_sq) * (S[2] * (S[1] * H[2] - S[2] * H[1]) - S[0] * (S[0] * H[1] - S[1] * H[0]));
+        a->zh[atom] = (one_oneplusalpha_sq) * (S[0] * H[1] - S[1] * H[0]) +
+                      (alpha_oneplusalpha_sq) * (S[0] * (S[2] * H[0] - S[0] * H[2]) - S[1] * (S[1] * H[2] - S[2] * H[1]));
+    }
+#pragma omp parallel for schedule(static)
+    for (int atom = 0; atom < num_atoms; atom++) {             /* :263-279 */
+        double S_new[3];
+        S_new[0] = a->x0[atom] + c->half_dt * (a->xe[atom] + a->xh[atom]);
+        S_new[1] = a->y0[atom] + c->half_dt * (a->ye[atom] + a->yh[atom]);
+        S_new[2] = a->z0[atom] + c->half_dt * (a->ze[atom] + a->zh[atom]);
+        const double mod_S = 1.0 / sqrt(S_new[0] * S_new[0] + S_new[1] * S_new[1] + S_new[2] * S_new[2]);
+        sx[atom] = S_new[0] * mod_S;
+        sy[atom] = S_new[1] * mod_S;
+        sz[atom] = S_new[2] * mod_S;
+    }
+}
+
+/* nsteps Heun steps; noise [nsteps][3][n] or NULL for the Philox stream at steps step0.. */
+int or_llg_heun(const or_sys* s, const or_cond* c, double* sx, double* sy, double* sz, int nsteps,
+                const double* noise, uint64_t seed, uint64_t step0, int nthreads) {
+    heun_arrays a;
+    memset(&a, 0, sizeof a);
+    if (alloc_arrays(&a, s->n)) { free_arrays(&a); return -1; }
+    set_threads(nthreads);
+    for (int k = 0; k < nsteps; ++k)
+        heun_step(s, c, sx, sy, sz, noise ? noise + (size_t)k * 3 * s->n : NULL, seed, step0 + (uint64_t)k, &a);
+    free_arrays(&a);
+    return 0;
+}
+
+/* ---- statistics --------------------------------------------------------------------------- */
+void or_magnetization(int n, const double* sx, const double* sy, const double* sz, const double* mm,
+                      const int* mask, int in_mask_size, double* magnetization) {
+    const int mask_size = in_mask_size - 1;
+    int* count = (int*)calloc((size_t)in_mask_size, sizeof(int));
+    for (int k = 0; k < 4 * in_mask_size; ++k) magnetization[k] = 0.0;
+    for (int atom = 0; atom < n; ++atom) {
+        const int mask_id = mask[atom];
+        magnetization[4 * mask_id + 0] += sx[atom] * mm[atom];
+        magnetization[4 * mask_id + 1] += sy[atom] * mm[atom];
+        magnetization[4 * mask_id + 2] += sz[atom] * mm[atom];
+        magnetization[4 * mask_id + 3] += mm[atom];
+        count[mask_id]++;
+    }
+    for (int mask_id = 0; mask_id < mask_size; ++mask_id) {
+        const double msat = magnetization[4 * mask_id + 3];
+        const double magm = sqrt(magnetization[4 * mask_id + 0] * magnetization[4 * mask_id + 0] +
+                                 magnetization[4 * mask_id + 1] * magnetization[4 * mask_id + 1] +
+                                 magnetization[4 * mask_id + 2] * magnetization[4 * mask_id + 2]);
+        magnetization[4 * mask_id + 0] = magnetization[4 * mask_id + 0] / magm;
+        magnetization[4 * mask_id + 1] = magnetization[4 * mask_id + 1] / magm;
+        magnetization[4 * mask_id + 2] = magnetization[4 * mask_id + 2] / magm;
+        magnetization[4 * mask_id + 3] = magm / msat;
+    }
+    for (int mask_id = 0; mask_id < mask_size; ++mask_id)
+        if (count[mask_id] == 0)
+            for (int k = 0; k < 4; ++k) magnetization[4 * mask_id + k] = 0.0;
+    free(count);
+}
+
+static double exchange_energy(const or_sys* s, const double* sx_a, const double* sy_a, const double* sz_a,
+                              int atom, double sx, double sy, double sz) {
+    double energy = 0.0;
+    for (int nn = s->nbr_start[atom]; nn <= s->nbr_end[atom]; ++nn) {
+        const int natom = s->nbr[nn];
+        const int iid = s->iid[nn];
+        if (s->exchange_type == 0) {
+            const double Jij = s->exchange[iid];
+            energy -= Jij * (sx_a[natom] * sx + sy_a[natom] * sy + sz_a[natom] * sz);
+        } else if (s->exchange_type == 1) {
+            const double* J = s->exchange + 3 * (size_t)iid;
+            energy -= (J[0] * sx_a[natom] * sx + J[1] * sy_a[natom] * sy + J[2] * sz_a[natom] * sz);
+        } else {
+            const double* J = s->exchange + 9 * (size_t)iid;
+            const double S[3] = {sx_a[natom], sy_a[natom], sz_a[natom]};
+            energy -= (J[0] * S[0] * sx + J[1] * S[1] * sx + J[2] * S[2] * sx +
+                       J[3] * S[0] * sy + J[4] * S[1] * sy + J[5] * S[2] * sy +
+                       J[6] * S[0] * sz + J[7] * S[1] * sz + J[8] * S[2] * sz);
+        }
+    }
+    return energy;
+}
+
+static double anisotropy_energy(const or_sys* s, int mat, double sx, double sy, double sz) {
+    double energy = 0.0;
+    const or_material* m = &s->mat[mat];
+    if (s->ani & 1u) {
+        const double sdote = (sx * m->ku_vector[0] + sy * m->ku_vector[1] + sz * m->ku_vector[2]);
+        energy += -m->ku2 * (sdote * sdote);
+    }
+    if (s->ani & 2u) {
+        const double sx4 = sx * sx * sx * sx, sy4 = sy * sy * sy * sy, sz4 = sz * sz * sz * sz;
+        energy += -0.5 * m->kc4 * (sx4 + sy4 + sz4);
+    }
+    if (s->ani & 4u) {
+        const double* e1 = m->kc_vector1;
+        const double* e2 = m->kc_vector2;
+        const double* e3 = m->kc_vector3;
+        const double d1 = sx * e1[0] + sy * e1[1] + sz * e1[2];
+        const double d2 = sx * e2[0] + sy * e2[1] + sz * e2[2];
+        const double d3 = sx * e3[0] + sy * e3[1] + sz * e3[2];
+        const double a = d1 * d1 * d1 * d1, b = d2 * d2 * d2 * d2, cc = d3 * d3 * d3 * d3;
+        energy += -0.5 * m->kc4 * (a + b + cc);
+    }
+    return energy;
+}
+
+static double applied_energy(const or_cond* c, double sx, double sy, double sz) {
+    /* -H_applied (H_vec . S) with H = H_vec*H_applied supplied directly */
+    return -(c->H[0] * sx + c->H[1] * sy + c->H[2] * sz);
+}
+
+double or_spin_energy(const or_sys* s, const or_cond* c, const double* sx_a, const double* sy_a,
+                      const double* sz_a, int atom, double sx, double sy, double sz) {
+    double energy = 0.0;
+    energy += exchange_energy(s, sx_a, sy_a, sz_a, atom, sx, sy, sz);
+    energy += anisotropy_energy(s, s->type[atom], sx, sy, sz);
+    energy += applied_energy(c, sx, sy, sz);
+    return energy;
+}
+
+/* energies [5][in_mask_size]: total, exchange, anisotropy, applied, magnetostatic */
+void or_energy(const or_sys* s, const or_cond* c, const double* sx, const double* sy, const double* sz,
+               const int* mask, int in_mask_size, double* out) {
+    const int n = s->n, mask_size = in_mask_size - 1;
+    double* tot = out;
+    double* exc = out + in_mask_size;
+    double* ani = out + 2 * in_mask_size;
+    double* app = out + 3 * in_mask_size;
+    double* mag = out + 4 * in_mask_size;
+    int* count = (int*)calloc((size_t)in_mask_size, sizeof(int));
+    for (int k = 0; k < 5 * in_mask_size; ++k) out[k] = 0.0;
+    for (int atom = 0; atom < n; ++atom) {
+        exc[mask[atom]] += exchange_energy(s, sx, sy, sz, atom, sx[atom], sy[atom], sz[atom]) * s->moment[atom];
+        count[mask[atom]]++;
+    }
+    for (int id = 0; id < mask_size; ++id) exc[id] = 0.5 * exc[id];
+    for (int atom = 0; atom < n; ++atom)
+        ani[mask[atom]] += anisotropy_energy(s, s->type[atom], sx[atom], sy[atom], sz[atom]) * s->moment[atom];
+    for (int atom = 0; atom < n; ++atom)
+        app[mask[atom]] += applied_energy(c, sx[atom], sy[atom], sz[atom]) * s->moment[atom];
+    for (int id = 0; id < mask_size; ++id) mag[id] = 0.5 * mag[id];
+    for (int id = 0; id < mask_size; ++id) tot[id] = exc[id] + ani[id] + app[id] + mag[id];
+    for (int id = 0; id < mask_size; ++id)
+        if (count[id] == 0) { tot[id] = 0.0; exc[id] = 0.0; ani[id] = 0.0; app[id] = 0.0; mag[id] = 0.0; }
+    free(count);
+}
+
+/* ---- Metropolis ------------------------------------------------------------------------------- */
+static double kBTBohr_of(const or_material* m, double T) {
+    const double alpha = m->temperature_rescaling_alpha;
+    const double Tc = m->temperature_rescaling_Tc;
+    const double rescaled_temperature = T < Tc ? Tc * pow(T / Tc, alpha) : T;
+    return 9.27400915e-24 / (rescaled_temperature * 1.3806503e-23);
+}
+
+static int metropolis_move(const or_sys* s, const or_cond* c, double* sx, double* sy, double* sz, int atom,
+                           const double g[3], double angle, double u) {
+    const int imaterial = s->type[atom];
+    const double Sold[3] = {sx[atom], sy[atom], sz[atom]};
+    double Snew[3] = {Sold[0] + g[0] * angle, Sold[1] + g[1] * angle, Sold[2] + g[2] * angle};
+    const double r = 1.0 / sqrt(Snew[0] * Snew[0] + Snew[1] * Snew[1] + Snew[2] * Snew[2]);
+    Snew[0] *= r; Snew[1] *= r; Snew[2] *= r;
+    const double Eold = or_spin_energy(s, c, sx, sy, sz, atom, Sold[0], Sold[1], Sold[2]);
+    const double Enew = or_spin_energy(s, c, sx, sy, sz, atom, Snew[0], Snew[1], Snew[2]);
+    const double DE = (Enew - Eold) * s->mat[imaterial].mu_s_SI * 1.07828231e23;
+    int accept = DE < 0;
+    if (!accept) accept = exp(-DE * kBTBohr_of(&s->mat[imaterial], c->temperature)) >= u;
+    if (accept) { sx[atom] = Snew[0]; sy[atom] = Snew[1]; sz[atom] = Snew[2]; }
+    return accept;
+}
+
+static void adapt(double* sigma, double moves, double rej) {
+    const double last_rejection_rate = rej / moves;
+    const double factor = 0.5 / last_rejection_rate;
+    *sigma *= factor;
+    if (*sigma > 60.0 || *sigma < 1e-5) *sigma = 60.0;
+}
+
+/* Colour-ordered sweeps: every atom once per sweep, colours in order; move normals from
+ * Philox stream 1 and acceptance uniform (word 0 / 2^32) from stream 2 at (atom, sweep). */
+void or_mc_checkerboard(const or_sys* s, const or_cond* c, double* sx, double* sy, double* sz,
+                        const int* colour_list, const int* colour_start, int ncolours, int nsweeps,
+                        uint64_t seed, uint64_t sweep0, double* sigma, double* tot_moves, double* tot_rej) {
+    for (int w = 0; w < nsweeps; ++w) {
+        const uint64_t sweep = sweep0 + (uint64_t)w;
+        double rej = 0.0;
+        for (int k = 0; k < ncolours; ++k) {
+            for (int t = colour_start[k]; t < colour_start[k + 1]; ++t) {
+                const int atom = colour_list[t];
+                double g[3];
+                or_normals3(seed, atom, sweep, 1u, g);
+                uint32_t ua[4];
+                philox_at(seed, (uint32_t)atom, sweep, 2u, ua);
+                const double u = (double)ua[0] * 2.3283064365386963e-10;
+                if (!metropolis_move(s, c, sx, sy, sz, atom, g, *sigma, u)) rej += 1.0;
+            }
+        }
+        *tot_moves += (double)s->n;
+        *tot_rej += rej;
+        adapt(sigma, (double)s->n, rej);
+    }
+}
+
+/* montecarlo::mc_step: N trials per sweep at random sites (with replacement).  The uniform
+ * stream is Philox at (trial, sweep, stream 3..4): site = int(N u) from word 0, acceptance from
+ * word 1; move normals from stream 5. */
+void or_mc_random_site(const or_sys* s, const or_cond* c, double* sx, double* sy, double* sz, int nsweeps,
+                       uint64_t seed, uint64_t sweep0, double* sigma, double* tot_moves, double* tot_rej) {
+    const int nmoves = s->n;
+    for (int w = 0; w < nsweeps; ++w) {
+        const uint64_t sweep = sweep0 + (uint64_t)w;
+        double rej = 0.0;
+        for (int i = 0; i < nmoves; ++i) {
+            uint32_t uu[4];
+            philox_at(seed, (uint32_t)i, sweep, 3u, uu);
+            const int atom = (int)(nmoves * ((double)uu[0] * 2.3283064365386963e-10));
+            double g[3];
+            or_normals3(seed, i, sweep, 5u, g);
+            const double u = (double)uu[1] * 2.3283064365386963e-10;
+            if (!metropolis_move(s, c, sx, sy, sz, atom, g, *sigma, u)) rej += 1.0;
+        }
+        *tot_moves += (double)nmoves;
+        *tot_rej += rej;
+        adapt(sigma, (double)nmoves, rej);
+    }
+}
+
+/* Greedy colouring in atom order over the symmetrised neighbour graph (the device's colouring).
+ * colour[n] out; returns the number of colours. */
+int or_greedy_colour(const or_sys* s, int* colour) {
+    const int n = s->n;
+    int nnz = 0;
+    for (int i = 0; i < n; ++i) nnz += s->nbr_end[i] - s->nbr_start[i] + 1;
+    int* rrow = (int*)calloc((size_t)n + 1, sizeof(int));
+    int* rcol = (int*)malloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+    int* fill = (int*)malloc(sizeof(int) * (size_t)n);
+    int* seen = (int*)malloc(sizeof(int) * 1024);
+    for (int i = 0; i < n; ++i)
+        for (int e = s->nbr_start[i]; e <= s->nbr_end[i]; ++e) rrow[s->nbr[e] + 1]++;
+    for (int i = 0; i < n; ++i) rrow[i + 1] += rrow[i];
+    for (int i = 0; i < n; ++i) fill[i] = rrow[i];
+    for (int i = 0; i < n; ++i)
+        for (int e = s->nbr_start[i]; e <= s->nbr_end[i]; ++e) rcol[fill[s->nbr[e]]++] = i;
+    for (int k = 0; k < 1024; ++k) seen[k] = -1;
+    int nc = 0;
+    for (int i = 0; i < n; ++i) colour[i] = -1;
+    for (int i = 0; i < n; ++i) {
+        for (int e = s->nbr_start[i]; e <= s->nbr_end[i]; ++e) {
+            const int j = s->nbr[e];
+            if (j != i && colour[j] >= 0) seen[colour[j]] = i;
+        }
+        for (int e = rrow[i]; e < rrow[i + 1]; ++e) {
+            const int j = rcol[e];
+            if (j != i && colour[j] >= 0) seen[colour[j]] = i;
+        }
+        int k = 0;
+        while (k < 1024 && seen[k] == i) ++k;
+        colour[i] = k;
+        if (k + 1 > nc) nc = k + 1;
+    }
+    free(rrow); free(rcol); free(fill); free(seen);
+    return nc;
+}

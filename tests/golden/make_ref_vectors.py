@@ -1,0 +1,219 @@
+"""Writes the golden vectors under tests/golden/ from the reference VAMPIRE CPU code.
+
+Run in the development container (needs /root/reference and oracle/_ref/ref_harness, built by
+`make -C oracle ref`):
+
+    python tests/golden/make_ref_vectors.py
+
+For each case an input deck and material file are written to a scratch directory, the reference
+builds the system itself (cs::create, exchange unrolling, anisotropy set-up) and the harness
+dumps what it built, the spin and external fields, the spins after some sim::LLG_Heun steps fed
+an injected noise stream, and the masked magnetisation / energy statistics.  The noise stream
+is Philox-4x32-10 normals (oracle.thermal_noise) so it can be regenerated from (seed, step) and
+is not stored.  A Monte Carlo case records |m| per sweep of the reference montecarlo::mc_step
+with its own MT19937 generator at several temperatures.
+
+Fixtures are .npz files of inputs and expected outputs only.
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle as O  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def deck(crystal, L, a, materials, pbc=(True, True, True), T=0.0, H=0.0, Hdir=(0, 0, 1), dmi_cutoff=None,
+         Lz=None):
+    lines = [
+        f"create:crystal-structure={crystal}",
+        f"dimensions:unit-cell-size = {a} !A",
+        f"dimensions:system-size-x = {L * a - 0.01} !A",
+        f"dimensions:system-size-y = {L * a - 0.01} !A",
+        f"dimensions:system-size-z = {(Lz or L) * a - 0.01} !A",
+        "material:file=test.mat",
+        f"sim:temperature={T}",
+        "sim:time-step=1.0E-16",
+        f"sim:applied-field-strength={H} !T",
+        "sim:applied-field-unit-vector=" + ",".join(str(v) for v in Hdir),
+        "sim:program=time-series",
+        "sim:integrator=llg-heun",
+        "sim:total-time-steps=0",
+        "output:total-energy",
+        "output:material-magnetisation",
+    ]
+    for axis, p in zip("xyz", pbc):
+        if p:
+            lines.append(f"create:periodic-boundaries-{axis}")
+    if dmi_cutoff is not None:
+        lines.append(f"exchange:dmi-cutoff-range = {dmi_cutoff} !A")
+    mat = [f"material:num-materials={len(materials)}"]
+    for k, m in enumerate(materials, start=1):
+        for key, val in m.items():
+            mat.append(f"material[{k}]:{key}={val}")
+    return "\n".join(lines) + "\n", "\n".join(mat) + "\n"
+
+
+def run(rundir, *args):
+    r = subprocess.run([HARNESS, rundir, *map(str, args)], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"harness failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+
+
+def read_meta(rundir):
+    meta = {"materials": []}
+    for ln in open(os.path.join(rundir, "meta.txt")):
+        p = ln.split()
+        if p[0] == "material":
+            d = {p[k]: float(p[k + 1]) for k in range(2, len(p), 2)}
+            meta["materials"].append(d)
+        else:
+            meta[p[0]] = float(p[1]) if "." in p[1] or "e" in p[1] else int(p[1])
+    return meta
+
+
+def read_system(rundir, meta):
+    n, nnb, nint = meta["num_atoms"], meta["num_nbr"], meta["num_interactions"]
+    K = {0: 1, 1: 3, 2: 9}[meta["exchange_type"]]
+    raw = open(os.path.join(rundir, "system.bin"), "rb").read()
+    off = 0
+
+    def take(dtype, count):
+        nonlocal off
+        a = np.frombuffer(raw, dtype=dtype, count=count, offset=off).copy()
+        off += a.nbytes
+        return a
+
+    sysd = dict(
+        type=take(np.int32, n), nbr_start=take(np.int32, n), nbr_end=take(np.int32, n),
+        nbr=take(np.int32, nnb), nbr_iid=take(np.int32, nnb), moment=take(np.float64, n),
+        coords=np.stack([take(np.float64, n), take(np.float64, n), take(np.float64, n)], axis=1),
+        spins0=np.stack([take(np.float64, n), take(np.float64, n), take(np.float64, n)]),
+        exchange=take(np.float64, nint * K).reshape(nint, K),
+    )
+    assert off == len(raw), (off, len(raw))
+    return sysd
+
+
+def digest(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def case(name, crystal, L, a, materials, steps, snaps, T, H=0.0, Hdir=(0, 0, 1), pbc=(True, True, True),
+         seed=11, spin_seed=5, dmi_cutoff=None, Lz=None, slim=False):
+    """slim: store the neighbour list and exchange table as a SHA-256 digest (plus the distinct J
+    values) instead of the arrays; the builder must reproduce them."""
+    with tempfile.TemporaryDirectory(prefix="vref_") as d:
+        inp, mat = deck(crystal, L, a, materials, pbc, T, H, Hdir, dmi_cutoff, Lz)
+        open(os.path.join(d, "input"), "w").write(inp)
+        open(os.path.join(d, "test.mat"), "w").write(mat)
+        run(d, "dump")
+        meta = read_meta(d)
+        sysd = read_system(d, meta)
+        n = meta["num_atoms"]
+        rng = np.random.default_rng(spin_seed)
+        g = rng.standard_normal((3, n))
+        spins = g / np.sqrt((g * g).sum(axis=0))
+        spins.tofile(os.path.join(d, "spins_in.bin"))
+        # noise: block for the field dump at step "-1" is Philox step 0, then steps 1..nsteps
+        blocks = [O.thermal_noise(seed, n, s) for s in range(steps + 1)]
+        np.concatenate([b.reshape(-1) for b in blocks]).tofile(os.path.join(d, "noise.bin"))
+        run(d, "llg", steps, 1)
+        fields = np.fromfile(os.path.join(d, "fields.bin"), dtype=np.float64).reshape(6, n)
+        traj = np.fromfile(os.path.join(d, "traj.bin"), dtype=np.float64).reshape(steps, 3, n)
+        stats_after = np.fromfile(os.path.join(d, "stats.bin"), dtype=np.float64)
+    big = {k: v for k, v in sysd.items() if k != "spins0"}
+    if slim:
+        big = dict(type=big["type"], moment=big["moment"],
+                   csr_digest=digest(big["nbr_start"], big["nbr_end"], big["nbr"]),
+                   exchange_values=np.unique(big["exchange"]))
+        fields = fields[:, :0]
+    np.savez_compressed(
+        os.path.join(OUT, f"ref_{name}.npz"),
+        crystal=crystal, L=L, a=a, T=T, H=H, Hdir=np.array(Hdir, dtype=float), pbc=np.array(pbc),
+        noise_seed=seed, spins_in=spins, snap_steps=np.array(snaps),
+        snaps=traj[np.array(snaps) - 1], fields=fields, stats_after=stats_after,
+        dt=meta["dt"], half_dt=meta["half_dt"], exchange_type=meta["exchange_type"],
+        materials=np.array([[m[k] for k in ("alpha", "gamma_rel", "mu_s_SI", "one_oneplusalpha_sq",
+                                            "alpha_oneplusalpha_sq", "H_th_sigma", "ku2", "mu_s_muB")]
+                            for m in meta["materials"]]),
+        **big,
+    )
+    print(f"{name}: n={n} nnz={meta['num_nbr']} exchange_type={meta['exchange_type']} "
+          f"interactions={meta['num_interactions']}")
+
+
+def mc_case(name, L, a, materials, temps, sweeps, every, equil):
+    rows = []
+    for T in temps:
+        with tempfile.TemporaryDirectory(prefix="vref_mc_") as d:
+            inp, mat = deck("sc", L, a, materials, (True, True, True), T)
+            open(os.path.join(d, "input"), "w").write(inp)
+            open(os.path.join(d, "test.mat"), "w").write(mat)
+            run(d, "mc", sweeps, T, every)
+            m = np.fromfile(os.path.join(d, "mc.bin"), dtype=np.float64).reshape(-1, 8)
+        mlen = m[:, 3]
+        keep = mlen[equil // every:]
+        # error bar from 10 blocks (autocorrelation-aware)
+        blocks = np.array_split(keep, 10)
+        bm = np.array([b.mean() for b in blocks])
+        rows.append((T, keep.mean(), bm.std(ddof=1) / np.sqrt(len(bm))))
+        print(f"  T={T:7.1f}  <|m|>={keep.mean():.4f} +- {rows[-1][2]:.4f}")
+    np.savez_compressed(os.path.join(OUT, f"ref_{name}.npz"), L=L, a=a, sweeps=sweeps, equil=equil,
+                        M_T=np.array(rows))
+
+
+CO = {"material-name": "Co", "damping-constant": 0.1, "exchange-matrix[1]": 6.064e-21,
+      "atomic-spin-moment": "1.72 !muB", "second-order-uniaxial-anisotropy-constant": 4.644e-24,
+      "uniaxial-anisotropy-direction": "0,0,1", "initial-spin-direction": "random"}
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        sys.exit("build the reference harness first: make -C oracle ref")
+    # SC, scalar exchange, uniaxial anisotropy, applied field, thermal noise
+    case("sc6_uniaxial", "sc", 6, 3.54, [CO], steps=200, snaps=[1, 10, 200], T=300.0, H=0.5)
+    # FCC, cubic anisotropy, alpha 0.05, T = 600 K
+    fcc = dict(CO, **{"damping-constant": 0.05, "second-order-uniaxial-anisotropy-constant": 0.0,
+                      "fourth-order-cubic-anisotropy-constant": 3.0e-24})
+    case("fcc3_cubic", "fcc", 3, 3.54, [fcc], steps=100, snaps=[1, 100], T=600.0, H=0.0)
+    # BCC, rotated cubic basis, field along (1,0,1)/sqrt2
+    bcc = dict(CO, **{"second-order-uniaxial-anisotropy-constant": 0.0,
+                      "fourth-order-cubic-anisotropy-constant": -2.0e-24,
+                      "cubic-anisotropy-direction-1": "1,1,0", "cubic-anisotropy-direction-2": "-1,1,0"})
+    case("bcc4_rotated", "bcc", 4, 2.87, [bcc], steps=100, snaps=[1, 100], T=100.0, H=2.0, Hdir=(1, 0, 1))
+    # two-material SC film, open in z, ferrimagnetic inter-material exchange and interfacial DMI:
+    # tensorial exchange with the reference's DMI tensors
+    m1 = {"material-name": "A", "damping-constant": 0.1, "exchange-matrix[1]": 5e-21,
+          "exchange-matrix[2]": -1.2e-21, "atomic-spin-moment": "1.6 !muB", "dmi-constant[2]": 0.3e-21,
+          "minimum-height": 0.0, "maximum-height": 0.5, "initial-spin-direction": "random"}
+    m2 = {"material-name": "B", "damping-constant": 0.1, "exchange-matrix[1]": -1.2e-21,
+          "exchange-matrix[2]": 5e-21, "atomic-spin-moment": "7.6 !muB", "dmi-constant[1]": 0.3e-21,
+          "minimum-height": 0.5, "maximum-height": 1.0, "initial-spin-direction": "random"}
+    case("film_dmi", "sc", 5, 2.5, [m1, m2], steps=100, snaps=[1, 100], T=300.0, H=0.0,
+         pbc=(True, True, False), dmi_cutoff=3.6, Lz=4)
+    # the CPU-runnable workload: SC 20^3, 1000 steps at 300 K
+    case("c0_sc20", "sc", 20, 3.54, [CO], steps=1000, snaps=[1000], T=300.0, H=0.0, seed=1, spin_seed=1,
+         slim=True)
+    # Metropolis M(T) of the reference random-site algorithm, SC 8^3, no anisotropy
+    mcm = dict(CO, **{"second-order-uniaxial-anisotropy-constant": 0.0, "initial-spin-direction": "0,0,1"})
+    mc_case("mc_sc8", 8, 3.54, [mcm], temps=[100.0, 300.0, 500.0, 600.0, 700.0, 900.0],
+            sweeps=6000, every=5, equil=1000)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,185 @@
+"""Device parity of the LLG-Heun path (libvmx.so through the C ABI) against the oracle and the
+reference golden vectors.  Tolerances are stated per test; FP64 kernels compute in the reference's
+operation order without FMA contraction, so most comparisons are exact or at the ulp level."""
+import numpy as np
+import pytest
+
+import refcases as RC
+from oracle import oracle as O
+from vampire_amd import system as S
+from vampire_amd.context import FP32, FP64, FUSED, PLAIN, Context
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN_CASES = ["sc6_uniaxial", "fcc3_cubic", "bcc4_rotated", "film_dmi"]
+
+
+def _ctx(system, precision=FP64, kernel=FUSED, seed=0, block=0):
+    return Context(system, precision=precision, kernel=kernel, seed=seed, block_size=block)
+
+
+def test_device_and_library(gpu):
+    from vampire_amd import context
+    assert context.device_count() >= 1
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_philox_noise_matches_oracle(gpu, precision):
+    s = S.ferromagnet("sc", 6)
+    with _ctx(s, precision=precision, seed=12345) as c:
+        for step in (0, 1, 7, 2**33 + 5):
+            dev = c.thermal_noise(step)
+            ref = O.thermal_noise(12345, s.n, step)
+            # FP64: same uniforms, device log/sincos within a few ulp of libm;
+            # FP32: 24-bit uniforms and hardware log2/sin/cos, absolute error below 2e-5
+            tol = 1e-13 if precision == FP64 else 2e-5
+            assert np.abs(dev - ref).max() <= tol * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_fields_match_reference(gpu, name):
+    f = RC.load(name)
+    s = RC.system_of(name)
+    T, H = RC.conditions(name)
+    with _ctx(s) as c:
+        c.set_conditions(T, H)
+        c.set_spins(f["spins_in"])
+        hs, he = c.fields(noise=O.thermal_noise(int(f["noise_seed"]), s.n, 0))
+    ref = f["fields"]
+    # bit-exact expected; allow 1 ulp of the field scale
+    scale = np.abs(ref).max()
+    assert np.abs(hs - ref[:3]).max() <= 2.3e-16 * scale
+    assert np.abs(he - ref[3:]).max() <= 2.3e-16 * scale
+
+
+@pytest.mark.parametrize("kernel", [PLAIN, FUSED])
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_trajectory_matches_reference(gpu, name, kernel):
+    """Device FP64 fed the reference's noise stream reproduces sim::LLG_Heun: <= 1e-10 per spin."""
+    f = RC.load(name)
+    s = RC.system_of(name)
+    T, H = RC.conditions(name)
+    seed = int(f["noise_seed"])
+    steps = [int(v) for v in f["snap_steps"]]
+    with _ctx(s, kernel=kernel) as c:
+        c.set_conditions(T, H)
+        c.set_spins(f["spins_in"])
+        last = 0
+        for k, st in enumerate(steps):
+            noise = np.stack([O.thermal_noise(seed, s.n, q) for q in range(last + 1, st + 1)])
+            c.llg_heun(st - last, noise=noise)
+            last = st
+            assert np.abs(c.get_spins() - f["snaps"][k]).max() <= 1e-10
+
+
+@pytest.mark.parametrize("kernel", [PLAIN, FUSED])
+def test_c0_1000_steps_philox(gpu, kernel):
+    """C0 (SC 20^3, 300 K): device Philox noise vs the oracle's Philox stream, 1000 steps,
+    per-spin agreement <= 1e-10 (FP64)."""
+    s = S.config_c0()
+    spins = S.random_spins(s.n, 1)
+    o = O.Oracle(s)
+    o.set_conditions(300.0)
+    ref = o.llg_heun(spins, 1000, seed=7, step0=0)
+    with _ctx(s, kernel=kernel, seed=7) as c:
+        c.set_conditions(300.0)
+        c.set_spins(spins)
+        c.llg_heun(1000)
+        dev = c.get_spins()
+    assert np.abs(dev - ref).max() <= 1e-10
+
+
+def test_c0_golden_1000_steps(gpu):
+    """Against the reference run itself: C0 after 1000 steps with the injected stream."""
+    f = RC.load("c0_sc20")
+    s = S.config_c0()
+    with _ctx(s) as c:
+        c.set_conditions(300.0)
+        c.set_spins(f["spins_in"])
+        for blk in range(10):
+            noise = np.stack([O.thermal_noise(1, s.n, q) for q in range(blk * 100 + 1, blk * 100 + 101)])
+            c.llg_heun(100, noise=noise)
+        assert np.abs(c.get_spins() - f["snaps"][0]).max() <= 1e-10
+
+
+def test_fp32_close_to_fp64(gpu):
+    """FP32 spins/fields vs FP64 from the same state and Philox stream: after 200 steps at
+    300 K the per-spin difference stays small (reported figure, bound 1e-4)."""
+    s = S.config_c0()
+    spins = S.random_spins(s.n, 3)
+    out = {}
+    for prec in (FP64, FP32):
+        with _ctx(s, precision=prec, seed=9) as c:
+            c.set_conditions(300.0)
+            c.set_spins(spins)
+            c.llg_heun(200)
+            out[prec] = c.get_spins()
+    assert np.abs(out[FP32] - out[FP64]).max() < 1e-4
+    assert np.abs(np.linalg.norm(out[FP32], axis=0) - 1.0).max() < 1e-6
+
+
+def test_graph_equals_eager(gpu):
+    s = S.ferromagnet("fcc", 6)
+    spins = S.random_spins(s.n, 4)
+    res = []
+    for use_graph in (False, True):
+        with _ctx(s, seed=3) as c:
+            c.set_conditions(400.0, (0.0, 0.0, 1.0))
+            c.set_spins(spins)
+            if use_graph:
+                c.llg_heun_graph(10, 3)
+            else:
+                c.llg_heun(30)
+            res.append(c.get_spins())
+            assert c.step == 30
+    assert np.array_equal(res[0], res[1])
+
+
+def test_replicas_equal_standalone(gpu):
+    """A batch of uneven replicas equals each replica run alone with seed + r."""
+    sizes = [3, 5, 4]
+    systems = [S.ferromagnet("sc", L) for L in sizes]
+    batch = S.concat_systems(systems)
+    spins = [S.random_spins(x.n, 10 + r) for r, x in enumerate(systems)]
+    temps = [100.0, 300.0, 600.0]
+    with _ctx(batch, seed=50) as c:
+        for r, T in enumerate(temps):
+            c.set_conditions(T, (0.0, 0.0, 0.5 * r), replica=r)
+        c.set_spins(np.concatenate(spins, axis=1))
+        c.llg_heun(50)
+        got = c.get_spins()
+    off = batch.replica_offset
+    for r, x in enumerate(systems):
+        with _ctx(x, seed=50 + r) as c:
+            c.set_conditions(temps[r], (0.0, 0.0, 0.5 * r))
+            c.set_spins(spins[r])
+            c.llg_heun(50)
+            assert np.array_equal(c.get_spins(), got[:, off[r]:off[r + 1]])
+
+
+def test_applied_field_precession_analytic(gpu):
+    """Reference test tests/physical/AppliedField: single spin, T=0, H=10 T along z, alpha=1,
+    S(0)=x; analytic Sx=sech(la)cos a, Sy=sech(la)sin a, Sz=tanh(la), a=t gamma H/(1+l^2)
+    (applied_field_errors.py:12-25); tolerance 1e-6 (run_tests.sh:53).  256 copies of the spin
+    with no neighbours run together."""
+    n = 256
+    mat = S.Material(mu_s_muB=1.72, alpha=1.0)
+    dt = 1e-16 * S.GAMMA_SI
+    sysm = S.System(n=n, type=np.zeros(n, np.int32), moment=np.full(n, 1.72), nbr_start=np.zeros(n, np.int32),
+                    nbr_end=np.full(n, -1, np.int32), nbr=np.zeros(0, np.int32), nbr_iid=np.zeros(0, np.int32),
+                    exchange_type=0, exchange=np.zeros((1, 1)), materials=[S.derived_material(mat, dt)],
+                    anisotropy_flags=0, dt=dt, half_dt=0.5 * dt)
+    spins = np.zeros((3, n))
+    spins[0] = 1.0
+    with _ctx(sysm) as c:
+        c.set_conditions(0.0, (0.0, 0.0, 10.0))
+        c.set_spins(spins)
+        worst = 0.0
+        for k in range(1, 101):
+            c.llg_heun(100)
+            t = k * 100 * 1e-16
+            lam = 1.0
+            a = t * 1.76e11 * 10.0 / (1 + lam * lam)
+            exp = np.array([np.cos(a) / np.cosh(lam * a), np.sin(a) / np.cosh(lam * a), np.tanh(lam * a)])
+            worst = max(worst, np.abs(c.get_spins()[:, 0] - exp).max())
+    assert worst < 1e-6

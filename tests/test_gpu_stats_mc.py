@@ -1,0 +1,125 @@
+"""Device statistics (FP64 reductions) and checkerboard Metropolis against the oracle and the
+reference's golden vectors."""
+import numpy as np
+import pytest
+
+import refcases as RC
+from oracle import oracle as O
+from vampire_amd import system as S
+from vampire_amd.context import FP32, FP64, FUSED, Context
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", ["sc6_uniaxial", "fcc3_cubic", "bcc4_rotated", "film_dmi"])
+def test_stats_match_reference(gpu, name):
+    """System and material magnetisation and system energies after the reference's steps
+    (stats_after = system M [8], material M [4(nmat+1)], system E [5x2])."""
+    f = RC.load(name)
+    s = RC.system_of(name)
+    T, H = RC.conditions(name)
+    nmat = len(s.materials)
+    with Context(s, precision=FP64) as c:
+        c.set_conditions(T, H)
+        c.set_spins(f["snaps"][-1])
+        hs = c.add_magnetization(np.zeros(s.n, np.int32), 2)
+        hm = c.add_magnetization(s.type, nmat + 1)
+        he = c.add_energy(np.zeros(s.n, np.int32), 2)
+        c.stats_update()
+        m_sys, _, cnt = c.magnetization(hs)
+        m_mat, _, _ = c.magnetization(hm)
+        e_sys, _, _ = c.energy(he)
+    ref = f["stats_after"]
+    assert cnt == 1
+    np.testing.assert_allclose(m_sys, ref[:8], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(m_mat, ref[8:8 + 4 * (nmat + 1)], rtol=1e-12, atol=1e-14)
+    e_ref = ref[8 + 4 * (nmat + 1):].reshape(5, 2)
+    np.testing.assert_allclose(e_sys, e_ref, rtol=1e-11, atol=1e-9)
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_stats_running_means(gpu, precision):
+    s = S.ferromagnet("bcc", 5)
+    o = O.Oracle(s)
+    o.set_conditions(200.0, (0.1, 0.0, 0.3))
+    mask = (np.arange(s.n) % 3).astype(np.int32)   # ids 0,1 magnetic, 2 collects the rest
+    with Context(s, precision=precision, seed=2) as c:
+        c.set_conditions(200.0, (0.1, 0.0, 0.3))
+        c.set_spins(S.random_spins(s.n, 8))
+        hm = c.add_magnetization(mask, 3)
+        he = c.add_energy(mask, 3)
+        acc_m, acc_e = np.zeros(12), np.zeros((5, 3))
+        for _ in range(4):
+            c.llg_heun(5)
+            c.stats_update()
+            sp = c.get_spins()
+            rm = o.magnetization(sp, mask, 3)
+            re = o.energy(sp, mask, 3)
+            cur, _, _ = c.magnetization(hm)
+            ecur, _, _ = c.energy(he)
+            tol = 1e-12 if precision == FP64 else 1e-6
+            np.testing.assert_allclose(cur, rm, rtol=tol, atol=tol)
+            np.testing.assert_allclose(ecur, re, rtol=tol, atol=tol * np.abs(re).max())
+            acc_m += rm
+            acc_e[:, :2] += re[:, :2]
+        _, mean, cnt = c.magnetization(hm)
+        _, emean, ecnt = c.energy(he)
+        assert cnt == 4 and ecnt == 4
+        np.testing.assert_allclose(mean, acc_m, rtol=1e-6, atol=1e-9)
+        np.testing.assert_allclose(emean[:, :2], acc_e[:, :2], rtol=1e-6, atol=1e-6)
+        c.stats_reset()
+        _, mean, cnt = c.magnetization(hm)
+        assert cnt == 0 and not mean.any()
+
+
+def test_mc_checkerboard_matches_oracle(gpu):
+    """FP64 checkerboard sweeps on the device reproduce the oracle's colour-ordered Metropolis
+    (same colouring, Philox streams and acceptance), including the adaptive cone width."""
+    s = S.ferromagnet("sc", 6)
+    o = O.Oracle(s)
+    o.set_conditions(400.0, (0.0, 0.0, 0.2))
+    spins = S.random_spins(s.n, 21)
+    ref, sigma, moves, rej = o.mc_checkerboard(spins, 8, seed=77, sweep0=0, sigma=60.0)
+    with Context(s, precision=FP64, seed=77) as c:
+        c.set_conditions(400.0, (0.0, 0.0, 0.2))
+        c.set_spins(spins)
+        assert c.mc_num_colours() == 2
+        c.mc_sweep(8)
+        dev = c.get_spins()
+        dsig, dmov, drej = c.mc_state()
+    assert np.abs(dev - ref).max() <= 1e-12
+    assert dmov[0] == moves and drej[0] == rej
+    assert abs(dsig[0] - sigma) <= 1e-12 * sigma
+
+
+def test_mc_curve_matches_reference(gpu):
+    """Statistical parity: <|m|>(T) of checkerboard Metropolis on SC 8^3 against the reference
+    random-site montecarlo::mc_step (its own MT19937 stream) within combined error bars
+    (4 sigma + 0.01 for finite-run bias), temperatures 100-900 K."""
+    f = RC.load("mc_sc8")
+    rows = f["M_T"]
+    temps = rows[:, 0]
+    systems = [S.ferromagnet("sc", 8, Ku=0.0) for _ in temps]
+    batch = S.concat_systems(systems)
+    n = systems[0].n
+    sweeps, equil, every = 6000, 1000, 5
+    with Context(batch, precision=FP64, seed=2024) as c:
+        for r, T in enumerate(temps):
+            c.set_conditions(float(T), replica=r)
+        sp = np.zeros((3, batch.n))
+        sp[2] = 1.0
+        c.set_spins(sp)
+        c.mc_sweep(equil)
+        samples = []
+        for _ in range((sweeps - equil) // every):
+            c.mc_sweep(every)
+            x = c.get_spins().reshape(3, len(temps), n).sum(axis=2)
+            samples.append(np.sqrt((x * x).sum(axis=0)) / n)
+    samples = np.array(samples)
+    blocks = np.array_split(samples, 10)
+    bm = np.array([b.mean(axis=0) for b in blocks])
+    mean = bm.mean(axis=0)
+    err = bm.std(axis=0, ddof=1) / np.sqrt(len(bm))
+    for r in range(len(temps)):
+        tol = 4.0 * np.hypot(err[r], rows[r, 2]) + 0.01
+        assert abs(mean[r] - rows[r, 1]) <= tol, (temps[r], mean[r], rows[r, 1], tol)

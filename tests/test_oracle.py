@@ -1,0 +1,309 @@
+"""The oracle pinned: against the reference's own outputs (golden vectors made by running the
+reference CPU code on its own input decks), against known-answer vectors, and against what the
+mathematics fixes (closed forms, invariants, brute force on tiny systems).  CPU only."""
+import hashlib
+import math
+
+import numpy as np
+import pytest
+
+import refcases as RC
+from oracle import oracle as O
+from vampire_amd import system as S
+
+GOLDEN_CASES = ["sc6_uniaxial", "fcc3_cubic", "bcc4_rotated", "film_dmi"]
+
+
+# ---------------------------------------------------------------- Philox known answers
+@pytest.mark.parametrize("ctr,key,expect", [
+    ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+])
+def test_philox_known_answers(ctr, key, expect):
+    """Random123 known-answer vectors for philox4x32-10 (Salmon et al., SC'11)."""
+    assert tuple(O.philox4x32_10(ctr, key)) == expect
+
+
+def test_thermal_noise_is_standard_normal():
+    g = O.thermal_noise(3, 200000, 5).reshape(-1)
+    assert abs(g.mean()) < 5 * (1 / math.sqrt(g.size))
+    assert abs(g.var() - 1.0) < 0.01
+    # tails out to the 2^-32 uniform limit (6.66 sigma) and no more
+    assert np.abs(g).max() < 6.7
+    # independence across steps and streams
+    h = O.thermal_noise(3, 200000, 6).reshape(-1)
+    assert abs(np.corrcoef(g, h)[0, 1]) < 0.01
+
+
+# ---------------------------------------------------------------- reference golden vectors
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_fields_equal_reference(name):
+    """calculate_spin_fields / calculate_external_fields of the reference, bit for bit."""
+    f = RC.load(name)
+    s = RC.system_of(name)
+    T, H = RC.conditions(name)
+    o = O.Oracle(s)
+    o.set_conditions(T, H)
+    hs = o.spin_fields(f["spins_in"])
+    he = o.external_fields(O.thermal_noise(int(f["noise_seed"]), s.n, 0))
+    assert np.array_equal(hs, f["fields"][:3])
+    assert np.array_equal(he, f["fields"][3:])
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_trajectory_equals_reference(name):
+    """sim::LLG_Heun of the reference fed the same noise stream: identical spins."""
+    f = RC.load(name)
+    s = RC.system_of(name)
+    T, H = RC.conditions(name)
+    o = O.Oracle(s)
+    o.set_conditions(T, H)
+    seed = int(f["noise_seed"])
+    cur, last = f["spins_in"], 0
+    for k, st in enumerate(int(v) for v in f["snap_steps"]):
+        noise = np.stack([O.thermal_noise(seed, s.n, q) for q in range(last + 1, st + 1)])
+        cur = o.llg_heun(cur, st - last, noise=noise)
+        last = st
+        assert np.array_equal(cur, f["snaps"][k])
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_statistics_equal_reference(name):
+    """magnetization_statistic_t (system, material) and energy_statistic_t (system)."""
+    f = RC.load(name)
+    s = RC.system_of(name)
+    T, H = RC.conditions(name)
+    o = O.Oracle(s)
+    o.set_conditions(T, H)
+    sp = f["snaps"][-1]
+    nmat = len(s.materials)
+    ref = f["stats_after"]
+    np.testing.assert_allclose(o.magnetization(sp, np.zeros(s.n, np.int32), 2), ref[:8], rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(o.magnetization(sp, s.type, nmat + 1), ref[8:8 + 4 * (nmat + 1)], rtol=1e-13,
+                               atol=1e-15)
+    e = o.energy(sp, np.zeros(s.n, np.int32), 2)
+    # applied-field energy is -(H.S) with H = H_vec*H_applied; the reference forms
+    # -H_applied*(H_vec.S): equal up to rounding of the product order
+    np.testing.assert_allclose(e, ref[8 + 4 * (nmat + 1):].reshape(5, 2), rtol=1e-12, atol=1e-9)
+
+
+def test_c0_equals_reference_1000_steps():
+    """The CPU workload (SC 20^3, 300 K): the builder reproduces the reference's neighbour list
+    (digest) and the oracle reproduces 1000 reference steps exactly."""
+    f = RC.load("c0_sc20")
+    s = S.config_c0()
+    h = hashlib.sha256()
+    for a in (s.nbr_start, s.nbr_end, s.nbr):
+        h.update(np.ascontiguousarray(a, dtype=np.int32).tobytes())
+    assert h.hexdigest() == str(f["csr_digest"])
+    assert np.array_equal(np.unique(s.exchange), f["exchange_values"])
+    assert np.array_equal(s.moment, f["moment"])
+    o = O.Oracle(s)
+    o.set_conditions(300.0)
+    cur = f["spins_in"]
+    for blk in range(10):
+        noise = np.stack([O.thermal_noise(1, s.n, q) for q in range(blk * 100 + 1, blk * 100 + 101)])
+        cur = o.llg_heun(cur, 100, noise=noise)
+    assert np.array_equal(cur, f["snaps"][0])
+
+
+@pytest.mark.parametrize("crystal,L,z", [("sc", 6, 6), ("fcc", 3, 12), ("bcc", 4, 8)])
+def test_builder_equals_reference(crystal, L, z):
+    name = {"sc": "sc6_uniaxial", "fcc": "fcc3_cubic", "bcc": "bcc4_rotated"}[crystal]
+    f = RC.load(name)
+    lat = S.build_lattice(crystal, L, L, L, a=float(f["a"]))
+    assert np.array_equal(lat.nbr_start, f["nbr_start"])
+    assert np.array_equal(lat.nbr_end, f["nbr_end"])
+    assert np.array_equal(lat.nbr, f["nbr"])
+    assert np.array_equal(lat.coords, f["coords"])
+    assert (lat.nbr_end - lat.nbr_start + 1 == z).all()
+
+
+def test_dmi_builder_equals_reference():
+    """DMI tensors of calculate_dmi on the two-material film (open z, cutoff 3.6 A)."""
+    f = RC.load("film_dmi")
+    keep = np.ones(5 * 5 * 4, dtype=bool)
+    lat = S.build_lattice("sc", 5, 5, 4, a=2.5, pbc=(True, True, False), keep=keep, vectors=True)
+    assert np.array_equal(lat.nbr, f["nbr"])
+    types = f["type"]
+    mats = RC.CASES["film_dmi"]["mats"]
+    J = [[5e-21, -1.2e-21], [-1.2e-21, 5e-21]]
+    D = [[0.0, 0.3e-21], [0.3e-21, 0.0]]
+    et, table, iid = S.exchange_tables(lat, types, mats, np.array(J), S.TENSORIAL, np.array(D), 3.6)
+    np.testing.assert_array_equal(table[iid], f["exchange"][f["nbr_iid"]])
+
+
+# ---------------------------------------------------------------- closed forms and invariants
+def _isolated(n, mat, dt_SI=1e-16):
+    dt = dt_SI * S.GAMMA_SI
+    return S.System(n=n, type=np.zeros(n, np.int32), moment=np.full(n, mat.mu_s_muB),
+                    nbr_start=np.zeros(n, np.int32), nbr_end=np.full(n, -1, np.int32), nbr=np.zeros(0, np.int32),
+                    nbr_iid=np.zeros(0, np.int32), exchange_type=0, exchange=np.zeros((1, 1)),
+                    materials=[S.derived_material(mat, dt)], anisotropy_flags=1 if mat.Ku else 0, dt=dt,
+                    half_dt=0.5 * dt)
+
+
+def test_single_spin_precession_closed_form():
+    """tests/physical/AppliedField of the reference: damped precession in 10 T, alpha = 1,
+    Sx = sech(la) cos a, Sy = sech(la) sin a, Sz = tanh(la), a = t gamma H/(1+l^2); tolerance 1e-6
+    (run_tests.sh:53).  Catches a wrong sign of either LLG term or a wrong gamma/dt scaling."""
+    s = _isolated(1, S.Material(alpha=1.0))
+    o = O.Oracle(s)
+    o.set_conditions(0.0, (0.0, 0.0, 10.0))
+    sp = np.array([[1.0], [0.0], [0.0]])
+    worst = 0.0
+    for k in range(1, 51):
+        sp = o.llg_heun(sp, 200)
+        a = k * 200 * 1e-16 * 1.76e11 * 10.0 / 2.0
+        exp = np.array([np.cos(a) / np.cosh(a), np.sin(a) / np.cosh(a), np.tanh(a)])
+        worst = max(worst, np.abs(sp[:, 0] - exp).max())
+    assert worst < 1e-6
+
+
+def test_heun_is_second_order():
+    """Halving dt cuts the deterministic (T=0) global error by ~4 (Heun is second order)."""
+    s = S.ferromagnet("sc", 4, alpha=0.2)
+    sp0 = S.random_spins(s.n, 2)
+    errs = []
+    ref_sys = S.ferromagnet("sc", 4, alpha=0.2, dt_SI=1e-17 / 64)
+    o = O.Oracle(ref_sys)
+    o.set_conditions(0.0, (0.0, 0.0, 1.0))
+    ref = o.llg_heun(sp0, 64 * 50)
+    for div in (1, 2, 4):
+        sd = S.ferromagnet("sc", 4, alpha=0.2, dt_SI=1e-17 / div)
+        od = O.Oracle(sd)
+        od.set_conditions(0.0, (0.0, 0.0, 1.0))
+        errs.append(np.abs(od.llg_heun(sp0, div * 50) - ref).max())
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert 3.2 < r1 < 4.8 and 3.2 < r2 < 4.8, errs
+
+
+def test_unit_length_and_energy_conservation():
+    """|S| = 1 after renormalisation; alpha = 0, T = 0 conserves the energy (to O(dt^2))."""
+    s = S.ferromagnet("sc", 5, alpha=0.0, dt_SI=1e-17)
+    o = O.Oracle(s)
+    o.set_conditions(0.0, (0.0, 0.0, 0.5))
+    sp = np.array([[0.0], [0.0], [1.0]]) + 0.3 * S.random_spins(s.n, 4)
+    sp /= np.linalg.norm(sp, axis=0)
+    e0 = o.energy(sp, np.zeros(s.n, np.int32), 2)[0, 0]
+    sp = o.llg_heun(sp, 500)
+    e1 = o.energy(sp, np.zeros(s.n, np.int32), 2)[0, 0]
+    assert np.abs(np.linalg.norm(sp, axis=0) - 1.0).max() < 1e-14
+    assert abs(e1 - e0) < 1e-4 * abs(e0)
+
+
+def test_exchange_field_brute_force():
+    """Exchange field equals a brute-force pair sum over the lattice geometry (all pairs at the
+    nearest-neighbour distance, periodic images), FCC 3^3."""
+    s = S.ferromagnet("fcc", 3, Ku=0.0)
+    sp = S.random_spins(s.n, 9)
+    o = O.Oracle(s)
+    h = o.spin_fields(sp)
+    a = 3.54
+    Lbox = 3 * a
+    J = s.exchange[0, 0]
+    d = s.coords[None, :, :] - s.coords[:, None, :]
+    d -= Lbox * np.round(d / Lbox)
+    r = np.sqrt((d * d).sum(axis=2))
+    nn = np.abs(r - a / math.sqrt(2)) < 1e-6
+    brute = J * (nn.astype(float) @ sp.T).T
+    np.testing.assert_allclose(h, brute, rtol=1e-12, atol=1e-9)
+
+
+def test_uniform_ferromagnet_field():
+    s = S.ferromagnet("bcc", 3, Ku=0.0)
+    sp = np.zeros((3, s.n))
+    sp[0] = 1.0
+    h = O.Oracle(s).spin_fields(sp)
+    np.testing.assert_allclose(h[0], 8 * s.exchange[0, 0], rtol=1e-15)
+    assert not h[1:].any()
+
+
+def test_tensor_identity_equals_isotropic():
+    """A tensorial J*I reproduces the isotropic field bit for bit."""
+    iso = S.ferromagnet("sc", 4, Ku=0.0)
+    ten = S.make_system(S.build_lattice("sc", 4, 4, 4), np.zeros(64, np.int32), [S.co_material(0.1, 0.0)],
+                        [[S.J_CO]], exchange_type=S.TENSORIAL)
+    sp = S.random_spins(64, 3)
+    assert np.array_equal(O.Oracle(iso).spin_fields(sp), O.Oracle(ten).spin_fields(sp))
+
+
+@pytest.mark.parametrize("flags,mat", [
+    ("uniaxial", S.Material(Ku=4.644e-24, ku_vector=(0.3, -0.2, 1.0))),
+    ("cubic", S.Material(Kc=3e-24)),
+    ("cubic_rotated", S.Material(Kc=-2e-24, kc_vector1=(1, 1, 0), kc_vector2=(-1, 1, 0))),
+])
+def test_anisotropy_field_is_minus_energy_gradient(flags, mat):
+    """H = -dE/dS of the single-spin anisotropy energy (central differences), the relation the
+    reference's field and energy functions must satisfy (uniaxial_second_order.cpp:104-121 etc.)."""
+    s = _isolated(4, mat)
+    s.anisotropy_flags = {"uniaxial": 1, "cubic": 2, "cubic_rotated": 4}[flags]
+    o = O.Oracle(s)
+    o.set_conditions(0.0)
+    sp = S.random_spins(4, 17)
+    h = o.spin_fields(sp)
+    eps = 1e-6
+    for i in range(4):
+        for k in range(3):
+            p, m = sp.copy(), sp.copy()
+            p[k, i] += eps
+            m[k, i] -= eps
+            g = (o.spin_energy(p, i) - o.spin_energy(m, i)) / (2 * eps)
+            assert abs(-g - h[k, i]) < 1e-6 * max(1.0, abs(h[k, i]))
+
+
+def test_dmi_tensor_antisymmetric():
+    s = S.config_c4(6, 6, 4, dilution=0.0)
+    t = s.exchange.reshape(-1, 3, 3)
+    sym = 0.5 * (t + t.transpose(0, 2, 1))
+    anti = 0.5 * (t - t.transpose(0, 2, 1))
+    # symmetric part is the isotropic exchange on the diagonal only
+    off = sym.copy()
+    off[:, [0, 1, 2], [0, 1, 2]] = 0.0
+    assert np.abs(off).max() < 1e-12
+    assert np.abs(anti).max() > 0.0
+
+
+def test_thermal_field_variance():
+    """External field at T: mean 0, variance sigma^2 T per component (fluctuation-dissipation
+    width H_th_sigma of initialise_variables.cpp:431-432)."""
+    mat = S.Material(alpha=0.1)
+    s = _isolated(100000, mat)
+    o = O.Oracle(s)
+    o.set_conditions(300.0, thermal=True, applied=False)
+    h = o.external_fields(seed=4, step=2)
+    sig = s.materials[0]["H_th_sigma"] * math.sqrt(300.0)
+    assert abs(h.mean()) < 5 * sig / math.sqrt(h.size)
+    assert abs(h.var() / sig ** 2 - 1.0) < 0.01
+
+
+def test_metropolis_single_spin_boltzmann():
+    """Isolated spins with uniaxial K under Metropolis sample p(theta) ~ sin(theta) exp(K sin^2/kT)
+    (tests/physical/Thermal/boltzmann_distribution.py:9-15, with the SI k_B)."""
+    Ku, T = 1e-23, 1.0
+    mat = S.Material(mu_s_muB=1.72, alpha=1.0, Ku=Ku)
+    s = _isolated(4000, mat)
+    o = O.Oracle(s)
+    o.set_conditions(T, thermal=False, applied=False)
+    sp = S.random_spins(s.n, 5)
+    sp, sigma, _, _ = o.mc_checkerboard(sp, 50, seed=3)
+    th = []
+    for w in range(40):
+        sp, sigma, _, _ = o.mc_checkerboard(sp, 5, seed=3, sweep0=100 + 5 * w, sigma=sigma)
+        th.append(np.arccos(np.clip(sp[2], -1, 1)))
+    th = np.concatenate(th)
+    hist, edges = np.histogram(th, bins=30, range=(0, np.pi), density=True)
+    c = 0.5 * (edges[1:] + edges[:-1])
+    p = np.sin(c) * np.exp(-Ku * np.sin(c) ** 2 / (1.3806503e-23 * T))
+    p /= np.trapezoid(p, c)
+    assert np.abs(hist - p).mean() < 0.03
+
+
+def test_checkerboard_colouring_is_proper():
+    for s in (S.ferromagnet("sc", 6), S.ferromagnet("sc", 5), S.ferromagnet("fcc", 3), S.config_c4(6, 6, 4)):
+        col, lst, start = O.Oracle(s).colouring()
+        rows = np.repeat(np.arange(s.n), s.nbr_end - s.nbr_start + 1)
+        assert (col[rows] != col[s.nbr]).all()
+        assert sorted(lst.tolist()) == list(range(s.n))

@@ -1,0 +1,226 @@
+"""Python handle on a libvmx context (include/vmx.h).
+
+``Context`` owns one device-side copy of a ``System`` and exposes the calls that VAMPIRE's
+``gpu::`` boundary makes (hdr/gpu.hpp:33-56), plus the diagnostics the parity tests use.
+Everything runs in libvmx.so; this module only marshals numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+from .system import System
+
+FP64 = "fp64"
+FP32 = "fp32"
+PLAIN = "plain"
+FUSED = "fused"
+
+
+def device_count() -> int:
+    lib = L.load()
+    c = C.c_int32(0)
+    L.check(lib.vmx_device_count(C.byref(c)))
+    return int(c.value)
+
+
+def material_struct(d: dict) -> L.vmx_material:
+    m = L.vmx_material()
+    for key in ("alpha", "gamma_rel", "mu_s_SI", "one_oneplusalpha_sq", "alpha_oneplusalpha_sq", "H_th_sigma",
+                "temperature_rescaling_alpha", "temperature_rescaling_Tc", "ku2", "kc4", "applied_field_strength"):
+        setattr(m, key, float(d[key]))
+    for key in ("ku_vector", "kc_vector1", "kc_vector2", "kc_vector3", "applied_field_unit_vector"):
+        arr = getattr(m, key)
+        for k in range(3):
+            arr[k] = float(d[key][k])
+    return m
+
+
+class Context:
+    """A device context for one system (or a batch of replicas)."""
+
+    def __init__(self, system: System, precision: str = FP64, kernel: str = FUSED, block_size: int = 0,
+                 seed: int = 0, device: int = -1):
+        self.lib = L.load()
+        self.system = system
+        self.n = system.n
+        self.precision = precision
+        self.kernel = kernel
+        # keep the arrays alive for the duration of vmx_create
+        s = system
+        self._arrs = [np.ascontiguousarray(x) for x in (s.type, s.nbr_start, s.nbr_end, s.nbr, s.nbr_iid)]
+        t, st, en, nb, iid = self._arrs
+        mom = np.ascontiguousarray(s.moment, dtype=np.float64)
+        ex = np.ascontiguousarray(s.exchange, dtype=np.float64)
+        mats = (L.vmx_material * len(s.materials))(*[material_struct(d) for d in s.materials])
+        roff = None if s.replica_offset is None else np.ascontiguousarray(s.replica_offset, dtype=np.int32)
+        sysc = L.vmx_system(
+            num_atoms=s.n, num_materials=len(s.materials), type=L.iptr(t), moment=L.dptr(mom),
+            nbr_start=L.iptr(st), nbr_end=L.iptr(en), num_nbr=nb.shape[0], nbr=L.iptr(nb),
+            nbr_iid=L.iptr(iid), exchange_type=s.exchange_type, num_interactions=ex.shape[0],
+            exchange=L.dptr(ex), materials=mats, anisotropy_flags=s.anisotropy_flags,
+            num_replicas=s.num_replicas, replica_offset=None if roff is None else L.iptr(roff))
+        opt = L.vmx_options(precision=L.VMX_FP64 if precision == FP64 else L.VMX_FP32,
+                            kernel=L.VMX_KERNEL_FUSED if kernel == FUSED else L.VMX_KERNEL_PLAIN,
+                            block_size=block_size, device=device, seed=seed)
+        h = C.c_void_p()
+        L.check(self.lib.vmx_create(C.byref(sysc), C.byref(opt), C.byref(h)))
+        self.h = h
+        self._arrs = None
+        L.check(self.lib.vmx_set_time_step(self.h, s.dt, s.half_dt))
+
+    # --- lifetime -------------------------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            L.check(self.lib.vmx_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- state ------------------------------------------------------------------------------
+    def set_spins(self, spins: np.ndarray) -> None:
+        sp = np.ascontiguousarray(spins, dtype=np.float64).reshape(3, self.n)
+        x, y, z = (np.ascontiguousarray(sp[k]) for k in range(3))
+        L.check(self.lib.vmx_set_spins(self.h, L.dptr(x), L.dptr(y), L.dptr(z)))
+
+    def get_spins(self) -> np.ndarray:
+        out = np.empty((3, self.n), dtype=np.float64)
+        x, y, z = np.empty(self.n), np.empty(self.n), np.empty(self.n)
+        L.check(self.lib.vmx_get_spins(self.h, L.dptr(x), L.dptr(y), L.dptr(z)))
+        out[0], out[1], out[2] = x, y, z
+        return out
+
+    def set_conditions(self, temperature: float, H: Sequence[float] = (0.0, 0.0, 0.0), thermal: bool = True,
+                       applied: bool = True, local_applied: bool = False, replica: int = -1) -> None:
+        flags = ((L.VMX_FIELD_THERMAL if thermal else 0) | (L.VMX_FIELD_APPLIED if applied else 0)
+                 | (L.VMX_FIELD_LOCAL_APPLIED if local_applied else 0))
+        h = (C.c_double * 3)(*[float(v) for v in H])
+        L.check(self.lib.vmx_set_conditions(self.h, replica, float(temperature), h, flags))
+
+    def set_time_step(self, dt: float, half_dt: Optional[float] = None) -> None:
+        L.check(self.lib.vmx_set_time_step(self.h, dt, 0.5 * dt if half_dt is None else half_dt))
+
+    @property
+    def step(self) -> int:
+        v = C.c_int64(0)
+        L.check(self.lib.vmx_get_step(self.h, C.byref(v)))
+        return int(v.value)
+
+    @step.setter
+    def step(self, value: int) -> None:
+        L.check(self.lib.vmx_set_step(self.h, int(value)))
+
+    # --- integration ------------------------------------------------------------------------
+    def llg_heun(self, nsteps: int = 1, noise: Optional[np.ndarray] = None) -> None:
+        if noise is None:
+            L.check(self.lib.vmx_llg_heun(self.h, int(nsteps), None))
+        else:
+            nz = np.ascontiguousarray(noise, dtype=np.float64)
+            if nz.size != nsteps * 3 * self.n:
+                raise ValueError(f"noise must hold {nsteps}x3x{self.n} values")
+            L.check(self.lib.vmx_llg_heun(self.h, int(nsteps), L.dptr(nz.reshape(-1))))
+
+    def llg_heun_graph(self, nsteps: int, nrepeat: int) -> None:
+        L.check(self.lib.vmx_llg_heun_graph(self.h, int(nsteps), int(nrepeat)))
+
+    def synchronize(self) -> None:
+        L.check(self.lib.vmx_synchronize(self.h))
+
+    def last_step_time_ms(self) -> float:
+        v = C.c_double(0.0)
+        L.check(self.lib.vmx_last_step_time_ms(self.h, C.byref(v)))
+        return float(v.value)
+
+    def time_kernels(self, nsteps: int):
+        """Average device ms of each kernel of a Heun step (HIP events around every launch)."""
+        ms = np.zeros(5)
+        nk = C.c_int32(0)
+        L.check(self.lib.vmx_time_kernels(self.h, int(nsteps), L.dptr(ms), C.byref(nk)))
+        return ms[:nk.value].copy()
+
+    def autotune(self, steps_per_trial: int = 20) -> int:
+        v = C.c_int32(0)
+        L.check(self.lib.vmx_autotune(self.h, int(steps_per_trial), C.byref(v)))
+        return int(v.value)
+
+    # --- Monte Carlo -------------------------------------------------------------------------
+    def mc_sweep(self, nsweeps: int = 1) -> None:
+        L.check(self.lib.vmx_mc_sweep(self.h, int(nsweeps)))
+
+    def mc_state(self):
+        R = self.system.num_replicas
+        s, m, r = np.empty(R), np.empty(R), np.empty(R)
+        L.check(self.lib.vmx_mc_get_state(self.h, L.dptr(s), L.dptr(m), L.dptr(r)))
+        return s, m, r
+
+    def mc_set_sigma(self, sigma: float, replica: int = -1) -> None:
+        L.check(self.lib.vmx_mc_set_sigma(self.h, replica, float(sigma)))
+
+    def mc_num_colours(self) -> int:
+        v = C.c_int32(0)
+        L.check(self.lib.vmx_mc_num_colours(self.h, C.byref(v)))
+        return int(v.value)
+
+    # --- statistics -----------------------------------------------------------------------------
+    def add_magnetization(self, mask: np.ndarray, mask_size: int) -> int:
+        m = np.ascontiguousarray(mask, dtype=np.int32)
+        h = C.c_int32(0)
+        L.check(self.lib.vmx_stats_add_magnetization(self.h, int(mask_size), L.iptr(m), C.byref(h)))
+        self._msize = getattr(self, "_msize", {})
+        self._msize[("m", h.value)] = mask_size
+        return int(h.value)
+
+    def add_energy(self, mask: np.ndarray, mask_size: int) -> int:
+        m = np.ascontiguousarray(mask, dtype=np.int32)
+        h = C.c_int32(0)
+        L.check(self.lib.vmx_stats_add_energy(self.h, int(mask_size), L.iptr(m), C.byref(h)))
+        self._msize = getattr(self, "_msize", {})
+        self._msize[("e", h.value)] = mask_size
+        return int(h.value)
+
+    def stats_update(self) -> None:
+        L.check(self.lib.vmx_stats_update(self.h))
+
+    def stats_reset(self) -> None:
+        L.check(self.lib.vmx_stats_reset(self.h))
+
+    def magnetization(self, handle: int = 0):
+        ms = self._msize[("m", handle)]
+        cur, mean = np.empty(4 * ms), np.empty(4 * ms)
+        cnt = C.c_int64(0)
+        L.check(self.lib.vmx_stats_get_magnetization(self.h, handle, L.dptr(cur), L.dptr(mean), C.byref(cnt)))
+        return cur, mean, int(cnt.value)
+
+    def energy(self, handle: int = 0):
+        ms = self._msize[("e", handle)]
+        cur, mean = np.empty(5 * ms), np.empty(5 * ms)
+        cnt = C.c_int64(0)
+        L.check(self.lib.vmx_stats_get_energy(self.h, handle, L.dptr(cur), L.dptr(mean), C.byref(cnt)))
+        return cur.reshape(5, ms), mean.reshape(5, ms), int(cnt.value)
+
+    # --- diagnostics ------------------------------------------------------------------------------
+    def fields(self, noise: Optional[np.ndarray] = None):
+        hs, he = np.empty(3 * self.n), np.empty(3 * self.n)
+        nz = None
+        if noise is not None:
+            nz = np.ascontiguousarray(noise, dtype=np.float64).reshape(-1)
+        L.check(self.lib.vmx_compute_fields(self.h, None if nz is None else L.dptr(nz), L.dptr(hs), L.dptr(he)))
+        return hs.reshape(3, self.n), he.reshape(3, self.n)
+
+    def thermal_noise(self, step: int) -> np.ndarray:
+        out = np.empty(3 * self.n)
+        L.check(self.lib.vmx_thermal_noise(self.h, int(step), L.dptr(out)))
+        return out.reshape(3, self.n)

@@ -1,0 +1,1213 @@
+// vmx_api.hip — the C ABI of include/vmx.h: context set-up, host-side validation and
+// compression of the system, stepping, statistics, Monte Carlo, graphs and autotuning.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/vmx.h"
+#include "vmx_internal.hpp"
+
+using namespace vmx;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define VMX_HIP(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(VMX_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                               \
+    } while (0)
+
+#define VMX_TRY(expr)                  \
+    do {                               \
+        int rc_ = (expr);              \
+        if (rc_ != VMX_OK) return rc_; \
+    } while (0)
+
+// Owning device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int alloc(size_t b) {
+        if (p && bytes >= b) return VMX_OK;
+        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        if (b == 0) b = 16;
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) { p = nullptr; return fail(VMX_ERR_NOMEM, "hipMalloc(%zu) failed: %s", b, hipGetErrorString(e)); }
+        bytes = b;
+        return VMX_OK;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Pinned staging ring for small host->device updates (conditions, sigma): a slot is reused only
+// after the copy that read it has completed.
+struct PinnedRing {
+    static constexpr int kSlots = 32;
+    void* slot[kSlots] = {};
+    hipEvent_t done[kSlots] = {};
+    size_t bytes = 0;
+    int next = 0;
+    ~PinnedRing() {
+        for (int k = 0; k < kSlots; ++k) {
+            if (slot[k]) (void)hipHostFree(slot[k]);
+            if (done[k]) (void)hipEventDestroy(done[k]);
+        }
+    }
+    int init(size_t b) {
+        bytes = b ? b : 16;
+        for (int k = 0; k < kSlots; ++k) {
+            VMX_HIP(hipHostMalloc(&slot[k], bytes, hipHostMallocDefault));
+            VMX_HIP(hipEventCreateWithFlags(&done[k], hipEventDisableTiming));
+            VMX_HIP(hipEventRecord(done[k], nullptr));
+        }
+        return VMX_OK;
+    }
+    int copy(void* dst, const void* src, size_t b, hipStream_t st) {
+        if (b > bytes) return fail(VMX_ERR_ARG, "staging copy of %zu bytes exceeds %zu", b, bytes);
+        const int k = next;
+        next = (next + 1) % kSlots;
+        VMX_HIP(hipEventSynchronize(done[k]));
+        std::memcpy(slot[k], src, b);
+        VMX_HIP(hipMemcpyAsync(dst, slot[k], b, hipMemcpyHostToDevice, st));
+        VMX_HIP(hipEventRecord(done[k], st));
+        return VMX_OK;
+    }
+};
+
+struct MagStat {
+    int nbins;
+    DevBuf mask, cur, mean;
+    std::vector<int> empty;  // mask ids (< nbins-1) without atoms
+    DevBuf empty_dev;
+    DevBuf counter;          // int64 on device
+};
+
+struct EnergyStat {
+    int nbins;
+    DevBuf mask, cur, mean, empty_dev, counter;
+    std::vector<int> empty;
+};
+
+__global__ void k_advance_step(unsigned long long* base, unsigned int n) { *base += n; }
+
+// Reduces the workgroup partials in block order, then normalises (magnetization.cpp:150-178).
+__global__ void k_mag_finish(int nblk, const double* __restrict__ partial, int nbins, double* cur,
+                             double* mean, const int* __restrict__ empty, int nempty,
+                             long long* counter) {
+    const int W = 4 * nbins;
+    for (int k = threadIdx.x; k < W; k += blockDim.x) {
+        double acc = 0.0;
+        for (int b = 0; b < nblk; ++b) acc += partial[(size_t)b * W + k];
+        cur[k] = acc;
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int id = threadIdx.x; id < nbins - 1; id += blockDim.x) {
+        const double msat = cur[4 * id + 3];
+        const double magm = sqrt(cur[4 * id + 0] * cur[4 * id + 0] + cur[4 * id + 1] * cur[4 * id + 1] +
+                                 cur[4 * id + 2] * cur[4 * id + 2]);
+        cur[4 * id + 0] = cur[4 * id + 0] / magm;
+        cur[4 * id + 1] = cur[4 * id + 1] / magm;
+        cur[4 * id + 2] = cur[4 * id + 2] / magm;
+        cur[4 * id + 3] = magm / msat;
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int e = threadIdx.x; e < nempty; e += blockDim.x) {
+        const int id = empty[e];
+        cur[4 * id + 0] = 0.0; cur[4 * id + 1] = 0.0; cur[4 * id + 2] = 0.0; cur[4 * id + 3] = 0.0;
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int k = threadIdx.x; k < W; k += blockDim.x) mean[k] += cur[k];
+    if (threadIdx.x == 0) *counter += 1;
+}
+
+// Energies [5][nbins]: total, exchange, anisotropy, applied, magnetostatic
+// (src/statistics/energy.cpp:141-258; no dipole field, so magnetostatic is zero).
+__global__ void k_energy_finish(int nblk, const double* __restrict__ partial, int nbins, double* cur,
+                                double* mean, const int* __restrict__ empty, int nempty,
+                                long long* counter) {
+    const int W = 3 * nbins;
+    for (int k = threadIdx.x; k < W; k += blockDim.x) {
+        double acc = 0.0;
+        for (int b = 0; b < nblk; ++b) acc += partial[(size_t)b * W + k];
+        const int id = k / 3, term = k % 3;     // term: 0 exchange, 1 anisotropy, 2 applied
+        cur[(size_t)(1 + term) * nbins + id] = acc;
+    }
+    for (int id = threadIdx.x; id < nbins; id += blockDim.x) { cur[id] = 0.0; cur[(size_t)4 * nbins + id] = 0.0; }
+    __threadfence_block();
+    __syncthreads();
+    const int ms = nbins - 1;
+    for (int id = threadIdx.x; id < ms; id += blockDim.x) {
+        cur[(size_t)1 * nbins + id] = 0.5 * cur[(size_t)1 * nbins + id];
+        cur[(size_t)4 * nbins + id] = 0.5 * cur[(size_t)4 * nbins + id];
+        cur[id] = cur[(size_t)1 * nbins + id] + cur[(size_t)2 * nbins + id] + cur[(size_t)3 * nbins + id] +
+                  cur[(size_t)4 * nbins + id];
+        for (int t = 0; t < 5; ++t) mean[(size_t)t * nbins + id] += cur[(size_t)t * nbins + id];
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int e = threadIdx.x; e < nempty; e += blockDim.x) {
+        const int id = empty[e];
+        for (int t = 0; t < 5; ++t) cur[(size_t)t * nbins + id] = 0.0;
+    }
+    if (threadIdx.x == 0) *counter += 1;
+}
+
+// Adaptive cone width after a sweep (src/montecarlo/mc.cpp:112-120) and running totals.
+__global__ void k_mc_adapt(int nrep, const int* __restrict__ rep_off, double* sigma, double* rejects,
+                           double* tot_moves, double* tot_rej) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrep) return;
+    const double moves = (double)(rep_off[r + 1] - rep_off[r]);
+    const double rej = rejects[r];
+    const double last_rejection_rate = rej / moves;
+    const double factor = 0.5 / last_rejection_rate;
+    double s = sigma[r] * factor;
+    if (s > 60.0 || s < 1e-5) s = 60.0;
+    sigma[r] = s;
+    tot_moves[r] += moves;
+    tot_rej[r] += rej;
+    rejects[r] = 0.0;
+}
+
+}  // namespace
+
+// ============================================================================================
+struct vmx_ctx {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int precision = VMX_FP64;
+    int kernel = VMX_KERNEL_FUSED;
+    int block = 256;
+    int xcd_remap = 1;
+    uint64_t seed = 0;
+
+    int n = 0, nmat = 0, nrep = 1;
+    int64_t nnz = 0;
+    int et = 0, jm = 0, K = 1;
+    uint32_t ani = 0, fld = 0;
+    double dt = 0.0, half_dt = 0.0;
+    uint64_t step = 0;       // host mirror of *step_base
+    uint64_t sweep = 0;
+
+    std::vector<vmx_material> mats;
+    std::vector<int> rep_off_h;
+    std::vector<double> T_h;      // per replica
+    std::vector<double> H_h;      // per replica [3]
+
+    // device state
+    DevBuf row, col, jidx, jtab, type, mat, cond, blk, rep_off, rep_key, step_base;
+    DevBuf spin[3];               // three buffers of 3N R each (S, P, O roles rotate)
+    int s_idx = 0, o_idx = 2;     // current spin buffer and the corrector's output buffer
+    DevBuf plainF, plainE, plainD;
+    DevBuf mm, atom_rep;
+    DevBuf staging;               // 3N doubles for host<->device spin copies and noise
+    DevBuf noise_dev;             // 3N doubles
+    DevBuf partial;               // statistics partial sums
+    int nblocks = 0;
+    std::vector<MagStat*> mags;
+    std::vector<EnergyStat*> energies;
+
+    // Monte Carlo
+    bool coloured = false;
+    int ncolours = 0;
+    std::vector<int> colour_start;
+    DevBuf colour_list, mc_sigma, mc_rej, mc_tot_moves, mc_tot_rej;
+
+    // graphs: (nsteps) -> executable
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    long long graph_key = -1;     // nsteps, block size, XCD mapping and buffer roles at capture
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int timed_steps = 0;
+    PinnedRing ring;
+
+    ~vmx_ctx() {
+        for (auto* m : mags) delete m;
+        for (auto* e : energies) delete e;
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    size_t rsize() const { return precision == VMX_FP64 ? sizeof(double) : sizeof(float); }
+
+    int use_device() const {
+        if (device >= 0) VMX_HIP(hipSetDevice(device));
+        return VMX_OK;
+    }
+
+    template <typename R>
+    Spin3<R> spins(int idx) const {
+        R* base = spin[idx].as<R>();
+        return Spin3<R>{base, base + n, base + 2 * (size_t)n};
+    }
+    template <typename R>
+    Spin3<R> arr3(const DevBuf& b) const {
+        R* base = b.as<R>();
+        return Spin3<R>{base, base + n, base + 2 * (size_t)n};
+    }
+
+    template <typename R>
+    SysDev<R> sys() const {
+        SysDev<R> s;
+        s.n = n; s.nmat = nmat; s.nblocks = nblocks; s.xcd_remap = xcd_remap;
+        s.row = row.as<int>(); s.col = col.as<int>();
+        s.jidx = jm == JM_UNIFORM ? nullptr : jidx.p;
+        s.jtab = jtab.as<R>();
+        s.type = type.as<uint8_t>();
+        s.mat = mat.as<MatConst<R>>();
+        s.cond = cond.as<Cond<R>>();
+        s.blk = blk.as<int2>();
+        s.rep_off = rep_off.as<int>();
+        s.rep_key = rep_key.as<uint2>();
+        s.step_base = step_base.as<unsigned long long>();
+        s.ani = ani; s.fld = fld;
+        s.dt = (R)dt; s.half_dt = (R)half_dt;
+        return s;
+    }
+
+    long long graph_key_of(int nsteps) const {
+        return (((long long)nsteps * 2048 + block) * 2 + xcd_remap) * 16 + s_idx * 4 + o_idx;
+    }
+
+    Launch launch() const {
+        int g = nblocks;
+        if (xcd_remap) g = (g + 7) / 8 * 8;
+        return Launch{g, block, stream};
+    }
+
+    int build_blocks() {
+        std::vector<int2> tab;
+        for (int r = 0; r < nrep; ++r)
+            for (int a = rep_off_h[r]; a < rep_off_h[r + 1]; a += block) tab.push_back(make_int2(a, r));
+        if (tab.empty()) tab.push_back(make_int2(0, 0));
+        nblocks = (int)tab.size();
+        VMX_TRY(blk.alloc(tab.size() * sizeof(int2)));
+        VMX_HIP(hipMemcpy(blk.p, tab.data(), tab.size() * sizeof(int2), hipMemcpyHostToDevice));
+        return VMX_OK;
+    }
+
+    template <typename R>
+    int upload_materials() {
+        std::vector<MatConst<R>> mc(nmat);
+        for (int m = 0; m < nmat; ++m) {
+            const vmx_material& a = mats[m];
+            MatConst<R>& c = mc[m];
+            c.one_opa2 = (R)a.one_oneplusalpha_sq;
+            c.alpha_opa2 = (R)a.alpha_oneplusalpha_sq;
+            c.ku2 = (R)a.ku2;
+            for (int k = 0; k < 3; ++k) {
+                c.ku[k] = (R)a.ku_vector[k];
+                c.e1[k] = (R)a.kc_vector1[k];
+                c.e2[k] = (R)a.kc_vector2[k];
+                c.e3[k] = (R)a.kc_vector3[k];
+            }
+            c.kc4 = (R)a.kc4;
+            c.mu_s_SI = (R)a.mu_s_SI;
+            c.pad = (R)0;
+        }
+        VMX_TRY(mat.alloc(mc.size() * sizeof(MatConst<R>)));
+        VMX_HIP(hipMemcpy(mat.p, mc.data(), mc.size() * sizeof(MatConst<R>), hipMemcpyHostToDevice));
+        return VMX_OK;
+    }
+
+    // Conditions of every (replica, material); temperature rescaling and thermal width as in
+    // src/simulate/fields.cpp:258-272, applied field as in fields.cpp:181-210, Metropolis
+    // factor as in src/montecarlo/mc.cpp:45-53.
+    template <typename R>
+    int upload_conditions() {
+        std::vector<Cond<R>> cv((size_t)nrep * nmat);
+        for (int r = 0; r < nrep; ++r) {
+            const double T = T_h[r];
+            const double Hx = H_h[3 * r + 0], Hy = H_h[3 * r + 1], Hz = H_h[3 * r + 2];
+            for (int m = 0; m < nmat; ++m) {
+                const vmx_material& a = mats[m];
+                const double alpha = a.temperature_rescaling_alpha;
+                const double Tc = a.temperature_rescaling_Tc;
+                const double rescaled_temperature = T < Tc ? Tc * pow(T / Tc, alpha) : T;
+                const double sqrt_T = sqrt(rescaled_temperature);
+                Cond<R>& c = cv[(size_t)r * nmat + m];
+                c.sigma = (R)(sqrt_T * a.H_th_sigma);
+                double h[3] = {Hx, Hy, Hz};
+                for (int k = 0; k < 3; ++k) {
+                    double hl = h[k];
+                    if (fld & FLD_LOCAL) hl = h[k] + a.applied_field_strength * a.applied_field_unit_vector[k];
+                    c.happ[k] = (R)hl;
+                    c.hglob[k] = (R)h[k];
+                }
+                c.kBTBohr = (R)(9.27400915e-24 / (rescaled_temperature * 1.3806503e-23));
+            }
+        }
+        VMX_TRY(cond.alloc(cv.size() * sizeof(Cond<R>)));
+        if (ring.bytes < cv.size() * sizeof(Cond<R>)) return fail(VMX_ERR_STATE, "condition table exceeds staging");
+        return ring.copy(cond.p, cv.data(), cv.size() * sizeof(Cond<R>), stream);
+    }
+    int upload_conditions_any() {
+        return precision == VMX_FP64 ? upload_conditions<double>() : upload_conditions<float>();
+    }
+
+    template <typename R>
+    int heun_steps(int nsteps, const double* noise, double* kernel_ms = nullptr);
+    template <typename R>
+    int capture_graph(int nsteps);
+    template <typename R>
+    int stats_update();
+    template <typename R>
+    int mc_sweeps(int nsweeps);
+    int colour();
+};
+
+// --------------------------------------------------------------------------------------------
+template <typename R>
+int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
+    const SysDev<R> s = sys<R>();
+    const Launch L = launch();
+    const double* nd = nullptr;
+    if (noise) {
+        VMX_TRY(noise_dev.alloc(3 * (size_t)n * sizeof(double)));
+        nd = noise_dev.as<double>();
+    }
+    if (kernel == VMX_KERNEL_PLAIN) {
+        VMX_TRY(plainF.alloc(3 * (size_t)n * rsize()));
+        VMX_TRY(plainE.alloc(3 * (size_t)n * rsize()));
+        VMX_TRY(plainD.alloc(3 * (size_t)n * rsize()));
+    }
+    // per-kernel timing: one event before each kernel and one after the last
+    hipEvent_t kev[6] = {};
+    if (kernel_ms) {
+        for (int e = 0; e < 6; ++e) VMX_HIP(hipEventCreate(&kev[e]));
+        for (int e = 0; e < 5; ++e) kernel_ms[e] = 0.0;
+    }
+    auto mark = [&](int e) -> hipError_t { return kernel_ms ? hipEventRecord(kev[e], stream) : hipSuccess; };
+    VMX_HIP(hipEventRecord(ev0, stream));
+    for (int k = 0; k < nsteps; ++k) {
+        if (noise)
+            VMX_HIP(hipMemcpyAsync(noise_dev.p, noise + (size_t)k * 3 * n, 3 * (size_t)n * sizeof(double),
+                                   hipMemcpyHostToDevice, stream));
+        const int p_idx = 3 - s_idx - o_idx;
+        int nk = 0;
+        if (kernel == VMX_KERNEL_FUSED) {
+            VMX_HIP(mark(0));
+            VMX_HIP(Ops<R>::fused_predictor(L, et, jm, s, spins<R>(s_idx), spins<R>(p_idx), nd, (uint32_t)k));
+            VMX_HIP(mark(1));
+            VMX_HIP(Ops<R>::fused_corrector(L, et, jm, s, spins<R>(s_idx), spins<R>(p_idx), spins<R>(o_idx), nd,
+                                            (uint32_t)k));
+            VMX_HIP(mark(2));
+            nk = 2;
+            std::swap(s_idx, o_idx);
+        } else {
+            const Spin3<R> F = arr3<R>(plainF), E = arr3<R>(plainE), D = arr3<R>(plainD);
+            VMX_HIP(mark(0));
+            VMX_HIP(Ops<R>::spin_fields(L, et, jm, s, spins<R>(s_idx), F));
+            VMX_HIP(mark(1));
+            VMX_HIP(Ops<R>::ext_fields(L, s, E, nd, (uint32_t)k));
+            VMX_HIP(mark(2));
+            VMX_HIP(Ops<R>::plain_predictor(L, s, spins<R>(s_idx), F, E, D, spins<R>(p_idx)));
+            VMX_HIP(mark(3));
+            VMX_HIP(Ops<R>::spin_fields(L, et, jm, s, spins<R>(p_idx), F));
+            VMX_HIP(mark(4));
+            VMX_HIP(Ops<R>::plain_corrector(L, s, spins<R>(s_idx), spins<R>(p_idx), F, E, D));
+            VMX_HIP(mark(5));
+            nk = 5;
+        }
+        if (kernel_ms) {
+            VMX_HIP(hipEventSynchronize(kev[nk]));
+            for (int e = 0; e < nk; ++e) {
+                float t = 0.f;
+                VMX_HIP(hipEventElapsedTime(&t, kev[e], kev[e + 1]));
+                kernel_ms[e] += (double)t / nsteps;
+            }
+        }
+    }
+    if (kernel_ms)
+        for (int e = 0; e < 6; ++e) (void)hipEventDestroy(kev[e]);
+    VMX_HIP(hipEventRecord(ev1, stream));
+    timed_steps = nsteps;
+    hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(1), 0, stream, step_base.as<unsigned long long>(),
+                       (unsigned)nsteps);
+    VMX_HIP(hipGetLastError());
+    step += (uint64_t)nsteps;
+    return VMX_OK;
+}
+
+template <typename R>
+int vmx_ctx::capture_graph(int nsteps) {
+    if (graph_exec) { VMX_HIP(hipGraphExecDestroy(graph_exec)); graph_exec = nullptr; }
+    if (graph) { VMX_HIP(hipGraphDestroy(graph)); graph = nullptr; }
+    const SysDev<R> s = sys<R>();
+    const Launch L = launch();
+    VMX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int si = s_idx, oi = o_idx;
+    for (int k = 0; k < nsteps; ++k) {
+        const int pi = 3 - si - oi;
+        hipError_t e = Ops<R>::fused_predictor(L, et, jm, s, spins<R>(si), spins<R>(pi), nullptr, (uint32_t)k);
+        if (e == hipSuccess)
+            e = Ops<R>::fused_corrector(L, et, jm, s, spins<R>(si), spins<R>(pi), spins<R>(oi), nullptr, (uint32_t)k);
+        if (e != hipSuccess) {
+            hipGraph_t g;
+            (void)hipStreamEndCapture(stream, &g);
+            return fail(VMX_ERR_HIP, "kernel launch during graph capture failed: %s", hipGetErrorString(e));
+        }
+        std::swap(si, oi);
+    }
+    hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(1), 0, stream, step_base.as<unsigned long long>(),
+                       (unsigned)nsteps);
+    VMX_HIP(hipStreamEndCapture(stream, &graph));
+    VMX_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+    graph_key = graph_key_of(nsteps);
+    return VMX_OK;
+}
+
+template <typename R>
+int vmx_ctx::stats_update() {
+    const Spin3<R> S = spins<R>(s_idx);
+    const int grid = std::max(1, std::min((n + 255) / 256, 512));
+    for (MagStat* st : mags) {
+        VMX_TRY(partial.alloc((size_t)grid * 4 * st->nbins * sizeof(double)));
+        VMX_HIP(Ops<R>::mag_partial(grid, stream, n, S, st->mask.as<int>(), mm.as<double>(), st->nbins,
+                                    partial.as<double>()));
+        hipLaunchKernelGGL(k_mag_finish, dim3(1), dim3(256), 0, stream, grid, partial.as<double>(), st->nbins,
+                           st->cur.as<double>(), st->mean.as<double>(), st->empty_dev.as<int>(),
+                           (int)st->empty.size(), st->counter.as<long long>());
+        VMX_HIP(hipGetLastError());
+    }
+    if (!energies.empty()) {
+        const SysDev<R> s = sys<R>();
+        if (nrep > 1 && !atom_rep.p) {
+            std::vector<int> ar(n);
+            for (int r = 0; r < nrep; ++r)
+                for (int a = rep_off_h[r]; a < rep_off_h[r + 1]; ++a) ar[a] = r;
+            VMX_TRY(atom_rep.alloc((size_t)n * sizeof(int)));
+            VMX_HIP(hipMemcpy(atom_rep.p, ar.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+        }
+        for (EnergyStat* st : energies) {
+            VMX_TRY(partial.alloc((size_t)grid * 3 * st->nbins * sizeof(double)));
+            VMX_HIP(Ops<R>::energy_partial(grid, stream, et, jm, s, S, st->mask.as<int>(), mm.as<double>(),
+                                           nrep > 1 ? atom_rep.as<int>() : nullptr, st->nbins,
+                                           partial.as<double>()));
+            hipLaunchKernelGGL(k_energy_finish, dim3(1), dim3(256), 0, stream, grid, partial.as<double>(),
+                               st->nbins, st->cur.as<double>(), st->mean.as<double>(), st->empty_dev.as<int>(),
+                               (int)st->empty.size(), st->counter.as<long long>());
+            VMX_HIP(hipGetLastError());
+        }
+    }
+    return VMX_OK;
+}
+
+// Greedy graph colouring in atom order over the symmetrised neighbour graph, so that no two
+// atoms of one colour interact in either direction.
+int vmx_ctx::colour() {
+    std::vector<int> rowh(n + 1), colh((size_t)nnz);
+    VMX_HIP(hipMemcpy(rowh.data(), row.p, (n + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    VMX_HIP(hipMemcpy(colh.data(), col.p, (size_t)nnz * sizeof(int), hipMemcpyDeviceToHost));
+    // reverse adjacency
+    std::vector<int> rrow(n + 1, 0), rcol((size_t)nnz);
+    for (int64_t e = 0; e < nnz; ++e) rrow[colh[e] + 1]++;
+    for (int i = 0; i < n; ++i) rrow[i + 1] += rrow[i];
+    {
+        std::vector<int> fill(rrow.begin(), rrow.end() - 1);
+        for (int i = 0; i < n; ++i)
+            for (int e = rowh[i]; e < rowh[i + 1]; ++e) rcol[fill[colh[e]]++] = i;
+    }
+    std::vector<int> c(n, -1);
+    std::vector<int> seen;  // seen[colour] == i marks colours used around atom i
+    int nc = 0;
+    for (int i = 0; i < n; ++i) {
+        auto mark = [&](int j) {
+            if (j != i && c[j] >= 0) {
+                if ((int)seen.size() <= c[j]) seen.resize(c[j] + 1, -1);
+                seen[c[j]] = i;
+            }
+        };
+        for (int e = rowh[i]; e < rowh[i + 1]; ++e) mark(colh[e]);
+        for (int e = rrow[i]; e < rrow[i + 1]; ++e) mark(rcol[e]);
+        int k = 0;
+        while (k < (int)seen.size() && seen[k] == i) ++k;
+        c[i] = k;
+        nc = std::max(nc, k + 1);
+    }
+    std::vector<int> list;
+    list.reserve(n);
+    colour_start.assign(nc + 1, 0);
+    for (int k = 0; k < nc; ++k) {
+        colour_start[k] = (int)list.size();
+        for (int i = 0; i < n; ++i)
+            if (c[i] == k) list.push_back(i);
+    }
+    colour_start[nc] = (int)list.size();
+    VMX_TRY(colour_list.alloc((size_t)n * sizeof(int)));
+    VMX_HIP(hipMemcpy(colour_list.p, list.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+    ncolours = nc;
+    coloured = true;
+    return VMX_OK;
+}
+
+template <typename R>
+int vmx_ctx::mc_sweeps(int nsweeps) {
+    if (!coloured) VMX_TRY(colour());
+    const SysDev<R> s = sys<R>();
+    const Spin3<R> S = spins<R>(s_idx);
+    for (int w = 0; w < nsweeps; ++w) {
+        for (int k = 0; k < ncolours; ++k) {
+            VMX_HIP(Ops<R>::mc_colour(stream, et, jm, s, S, colour_list.as<int>() + colour_start[k],
+                                      colour_start[k + 1] - colour_start[k], nrep, mc_sigma.as<double>(), sweep,
+                                      mc_rej.as<double>()));
+        }
+        hipLaunchKernelGGL(k_mc_adapt, dim3((nrep + 255) / 256), dim3(256), 0, stream, nrep, rep_off.as<int>(),
+                           mc_sigma.as<double>(), mc_rej.as<double>(), mc_tot_moves.as<double>(),
+                           mc_tot_rej.as<double>());
+        VMX_HIP(hipGetLastError());
+        ++sweep;
+    }
+    return VMX_OK;
+}
+
+// ============================================================================================
+// Exchange table compression: exact (bitwise) deduplication of the J values edges refer to.
+namespace {
+
+struct KeyHash {
+    size_t operator()(const std::string& k) const { return std::hash<std::string>()(k); }
+};
+
+int compress_exchange(const vmx_system* sys, const std::vector<int>& iid_csr, int K, int& jm,
+                      std::vector<double>& table, std::vector<uint16_t>& idx16, std::vector<int>& idx32) {
+    const int64_t nnz = (int64_t)iid_csr.size();
+    const double* ex = sys->exchange;
+    // fast path: every referenced entry equal to the first one
+    bool uniform = true;
+    if (nnz > 0) {
+        const double* j0 = ex + (size_t)K * iid_csr[0];
+        for (int64_t e = 1; e < nnz && uniform; ++e) {
+            const double* je = ex + (size_t)K * iid_csr[e];
+            if (std::memcmp(j0, je, K * sizeof(double)) != 0) uniform = false;
+        }
+        if (uniform) {
+            table.assign(j0, j0 + K);
+            jm = JM_UNIFORM;
+            return VMX_OK;
+        }
+    } else {
+        table.assign(K, 0.0);
+        jm = JM_UNIFORM;
+        return VMX_OK;
+    }
+    // dedup the interaction ids that are referenced
+    std::unordered_map<std::string, int, KeyHash> uniq;
+    std::vector<int> remap((size_t)sys->num_interactions, -1);
+    bool small = true;
+    for (int64_t e = 0; e < nnz; ++e) {
+        const int id = iid_csr[e];
+        if (remap[id] >= 0) continue;
+        std::string key((const char*)(ex + (size_t)K * id), K * sizeof(double));
+        auto it = uniq.find(key);
+        if (it == uniq.end()) {
+            const int u = (int)uniq.size();
+            uniq.emplace(std::move(key), u);
+            table.insert(table.end(), ex + (size_t)K * id, ex + (size_t)K * id + K);
+            remap[id] = u;
+            if (u >= 65536) { small = false; break; }
+        } else {
+            remap[id] = it->second;
+        }
+    }
+    if (small) {
+        jm = JM_IDX16;
+        idx16.resize(nnz);
+        for (int64_t e = 0; e < nnz; ++e) idx16[e] = (uint16_t)remap[iid_csr[e]];
+        return VMX_OK;
+    }
+    jm = JM_IDX32;
+    table.assign(ex, ex + (size_t)K * sys->num_interactions);
+    idx32 = iid_csr;
+    return VMX_OK;
+}
+
+}  // namespace
+
+// ============================================================================================
+extern "C" {
+
+const char* vmx_last_error(void) { return g_last_error.c_str(); }
+
+const char* vmx_version(void) { return "vmx 0.1 gfx950"; }
+
+int vmx_device_count(int32_t* count) {
+    if (!count) return fail(VMX_ERR_ARG, "count is null");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    *count = (e == hipSuccess) ? c : 0;
+    return VMX_OK;
+}
+
+int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
+    g_last_error.clear();
+    if (!sys || !opt || !out) return fail(VMX_ERR_ARG, "null argument");
+    *out = nullptr;
+    const int n = sys->num_atoms;
+    if (n <= 0) return fail(VMX_ERR_ARG, "num_atoms must be positive (got %d)", n);
+    if (sys->num_materials < 1 || sys->num_materials > 255)
+        return fail(VMX_ERR_ARG, "num_materials must be in [1,255] (got %d)", sys->num_materials);
+    if (!sys->type || !sys->nbr_start || !sys->nbr_end || !sys->materials || !sys->moment)
+        return fail(VMX_ERR_ARG, "type, moment, nbr_start, nbr_end and materials are required");
+    if (sys->num_nbr > 0 && (!sys->nbr || !sys->nbr_iid)) return fail(VMX_ERR_ARG, "nbr and nbr_iid are required");
+    if (sys->exchange_type < 0 || sys->exchange_type > 2)
+        return fail(VMX_ERR_ARG, "exchange_type must be 0, 1 or 2 (got %d)", sys->exchange_type);
+    if (sys->num_nbr > 0 && (sys->num_interactions <= 0 || !sys->exchange))
+        return fail(VMX_ERR_ARG, "exchange table is required");
+    if (opt->precision != VMX_FP64 && opt->precision != VMX_FP32) return fail(VMX_ERR_ARG, "bad precision");
+    if (opt->kernel != VMX_KERNEL_PLAIN && opt->kernel != VMX_KERNEL_FUSED) return fail(VMX_ERR_ARG, "bad kernel");
+    const int bs = opt->block_size ? opt->block_size : 256;
+    if (bs < 64 || bs > 1024 || bs % 64) return fail(VMX_ERR_ARG, "block_size must be a multiple of 64 in [64,1024]");
+    const int nrep = sys->num_replicas < 1 ? 1 : sys->num_replicas;
+    std::vector<int> roff(nrep + 1);
+    if (nrep == 1) {
+        roff[0] = 0; roff[1] = n;
+        if (sys->replica_offset && (sys->replica_offset[0] != 0 || sys->replica_offset[1] != n))
+            return fail(VMX_ERR_ARG, "replica_offset must be {0, num_atoms} for one replica");
+    } else {
+        if (!sys->replica_offset) return fail(VMX_ERR_ARG, "replica_offset is required for several replicas");
+        for (int r = 0; r <= nrep; ++r) roff[r] = sys->replica_offset[r];
+        if (roff[0] != 0 || roff[nrep] != n) return fail(VMX_ERR_ARG, "replica_offset must run from 0 to num_atoms");
+        for (int r = 0; r < nrep; ++r)
+            if (roff[r + 1] <= roff[r]) return fail(VMX_ERR_ARG, "replica %d is empty or offsets decrease", r);
+    }
+    const int K = sys->exchange_type == 0 ? 1 : (sys->exchange_type == 1 ? 3 : 9);
+
+    // CSR from inclusive start/end index arrays, with every index checked on the host
+    std::vector<int> rowh(n + 1);
+    rowh[0] = 0;
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const int64_t s0 = sys->nbr_start[i], e0 = sys->nbr_end[i];
+        const int64_t cnt = e0 - s0 + 1;
+        if (cnt < 0 || (cnt > 0 && (s0 < 0 || e0 >= sys->num_nbr)))
+            return fail(VMX_ERR_ARG, "neighbour range of atom %d [%lld,%lld] outside [0,%lld)", i, (long long)s0,
+                        (long long)e0, (long long)sys->num_nbr);
+        total += cnt;
+        if (total > INT32_MAX) return fail(VMX_ERR_ARG, "more than 2^31-1 neighbour entries");
+        rowh[i + 1] = (int)total;
+    }
+    std::vector<int> colh((size_t)total), iidh((size_t)total);
+    std::vector<int> atom_rep_h(n);
+    for (int r = 0; r < nrep; ++r)
+        for (int a = roff[r]; a < roff[r + 1]; ++a) atom_rep_h[a] = r;
+    for (int i = 0; i < n; ++i) {
+        const int64_t s0 = sys->nbr_start[i];
+        const int t = sys->type[i];
+        if (t < 0 || t >= sys->num_materials) return fail(VMX_ERR_ARG, "type of atom %d is %d, outside materials", i, t);
+        for (int k = rowh[i]; k < rowh[i + 1]; ++k) {
+            const int j = sys->nbr[s0 + (k - rowh[i])];
+            const int id = sys->nbr_iid[s0 + (k - rowh[i])];
+            if (j < 0 || j >= n) return fail(VMX_ERR_ARG, "neighbour %d of atom %d is outside [0,%d)", j, i, n);
+            if (id < 0 || id >= sys->num_interactions)
+                return fail(VMX_ERR_ARG, "interaction id %d of atom %d is outside the exchange table", id, i);
+            if (atom_rep_h[j] != atom_rep_h[i])
+                return fail(VMX_ERR_ARG, "atom %d interacts with atom %d of another replica", i, j);
+            colh[k] = j;
+            iidh[k] = id;
+        }
+    }
+
+    std::unique_ptr<vmx_ctx> c(new vmx_ctx());
+    c->device = opt->device;
+    VMX_TRY(c->use_device());
+    c->precision = opt->precision;
+    c->kernel = opt->kernel;
+    c->block = bs;
+    c->seed = opt->seed;
+    c->n = n;
+    c->nmat = sys->num_materials;
+    c->nrep = nrep;
+    c->nnz = total;
+    c->et = sys->exchange_type;
+    c->K = K;
+    c->ani = sys->anisotropy_flags & 0x7u;
+    c->mats.assign(sys->materials, sys->materials + sys->num_materials);
+    c->rep_off_h = roff;
+    c->T_h.assign(nrep, 0.0);
+    c->H_h.assign(3 * (size_t)nrep, 0.0);
+    VMX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    VMX_HIP(hipEventCreate(&c->ev0));
+    VMX_HIP(hipEventCreate(&c->ev1));
+
+    std::vector<double> table;
+    std::vector<uint16_t> idx16;
+    std::vector<int> idx32;
+    VMX_TRY(compress_exchange(sys, iidh, K, c->jm, table, idx16, idx32));
+
+    const size_t rs = c->rsize();
+    VMX_TRY(c->row.alloc((n + 1) * sizeof(int)));
+    VMX_HIP(hipMemcpy(c->row.p, rowh.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+    VMX_TRY(c->col.alloc(std::max<size_t>(1, colh.size()) * sizeof(int)));
+    if (!colh.empty()) VMX_HIP(hipMemcpy(c->col.p, colh.data(), colh.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (c->jm == JM_IDX16) {
+        VMX_TRY(c->jidx.alloc(idx16.size() * sizeof(uint16_t)));
+        VMX_HIP(hipMemcpy(c->jidx.p, idx16.data(), idx16.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    } else if (c->jm == JM_IDX32) {
+        VMX_TRY(c->jidx.alloc(idx32.size() * sizeof(int)));
+        VMX_HIP(hipMemcpy(c->jidx.p, idx32.data(), idx32.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    VMX_TRY(c->jtab.alloc(table.size() * rs));
+    if (c->precision == VMX_FP64) {
+        VMX_HIP(hipMemcpy(c->jtab.p, table.data(), table.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tf(table.begin(), table.end());
+        VMX_HIP(hipMemcpy(c->jtab.p, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    std::vector<uint8_t> t8(n);
+    for (int i = 0; i < n; ++i) t8[i] = (uint8_t)sys->type[i];
+    VMX_TRY(c->type.alloc(n));
+    VMX_HIP(hipMemcpy(c->type.p, t8.data(), n, hipMemcpyHostToDevice));
+    VMX_TRY(c->mm.alloc((size_t)n * sizeof(double)));
+    VMX_HIP(hipMemcpy(c->mm.p, sys->moment, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    VMX_TRY(c->rep_off.alloc((nrep + 1) * sizeof(int)));
+    VMX_HIP(hipMemcpy(c->rep_off.p, roff.data(), (nrep + 1) * sizeof(int), hipMemcpyHostToDevice));
+    std::vector<uint2> keys(nrep);
+    for (int r = 0; r < nrep; ++r) {
+        const uint64_t k = opt->seed + (uint64_t)r;
+        keys[r] = make_uint2((uint32_t)k, (uint32_t)(k >> 32));
+    }
+    VMX_TRY(c->rep_key.alloc(nrep * sizeof(uint2)));
+    VMX_HIP(hipMemcpy(c->rep_key.p, keys.data(), nrep * sizeof(uint2), hipMemcpyHostToDevice));
+    VMX_TRY(c->step_base.alloc(sizeof(unsigned long long)));
+    VMX_HIP(hipMemset(c->step_base.p, 0, sizeof(unsigned long long)));
+    if (c->precision == VMX_FP64) VMX_TRY(c->upload_materials<double>());
+    else VMX_TRY(c->upload_materials<float>());
+    VMX_TRY(c->build_blocks());
+    for (int b = 0; b < 3; ++b) {
+        VMX_TRY(c->spin[b].alloc(3 * (size_t)n * rs));
+        VMX_HIP(hipMemset(c->spin[b].p, 0, 3 * (size_t)n * rs));
+    }
+    VMX_TRY(c->staging.alloc(3 * (size_t)n * sizeof(double)));
+    const size_t cond_bytes = (size_t)nrep * c->nmat * sizeof(Cond<double>) + 64;
+    VMX_TRY(c->ring.init(std::max<size_t>(cond_bytes, 8 * (size_t)nrep + 64)));
+    VMX_TRY(c->upload_conditions_any());
+    // Monte Carlo state: adaptive sigma starts at 60 (src/montecarlo/data.cpp:46)
+    VMX_TRY(c->mc_sigma.alloc(nrep * sizeof(double)));
+    VMX_TRY(c->mc_rej.alloc(nrep * sizeof(double)));
+    VMX_TRY(c->mc_tot_moves.alloc(nrep * sizeof(double)));
+    VMX_TRY(c->mc_tot_rej.alloc(nrep * sizeof(double)));
+    {
+        std::vector<double> s60(nrep, 60.0);
+        VMX_HIP(hipMemcpy(c->mc_sigma.p, s60.data(), nrep * sizeof(double), hipMemcpyHostToDevice));
+        VMX_HIP(hipMemset(c->mc_rej.p, 0, nrep * sizeof(double)));
+        VMX_HIP(hipMemset(c->mc_tot_moves.p, 0, nrep * sizeof(double)));
+        VMX_HIP(hipMemset(c->mc_tot_rej.p, 0, nrep * sizeof(double)));
+    }
+    VMX_HIP(hipDeviceSynchronize());
+    *out = c.release();
+    return VMX_OK;
+}
+
+int vmx_destroy(vmx_ctx* ctx) {
+    if (!ctx) return VMX_OK;
+    (void)ctx->use_device();
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    delete ctx;
+    return VMX_OK;
+}
+
+int vmx_set_spins(vmx_ctx* c, const double* sx, const double* sy, const double* sz) {
+    if (!c || !sx || !sy || !sz) return fail(VMX_ERR_ARG, "null argument");
+    VMX_TRY(c->use_device());
+    const size_t nb = (size_t)c->n * sizeof(double);
+    double* st = c->staging.as<double>();
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    VMX_HIP(hipMemcpy(st, sx, nb, hipMemcpyHostToDevice));
+    VMX_HIP(hipMemcpy(st + c->n, sy, nb, hipMemcpyHostToDevice));
+    VMX_HIP(hipMemcpy(st + 2 * (size_t)c->n, sz, nb, hipMemcpyHostToDevice));
+    if (c->precision == VMX_FP64) {
+        VMX_HIP(hipMemcpyAsync(c->spin[c->s_idx].p, st, 3 * nb, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        VMX_HIP(Ops<float>::from_double(c->stream, 3 * c->n, st, c->spin[c->s_idx].as<float>()));
+    }
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    return VMX_OK;
+}
+
+int vmx_get_spins(vmx_ctx* c, double* sx, double* sy, double* sz) {
+    if (!c || !sx || !sy || !sz) return fail(VMX_ERR_ARG, "null argument");
+    VMX_TRY(c->use_device());
+    const size_t nb = (size_t)c->n * sizeof(double);
+    double* st = c->staging.as<double>();
+    if (c->precision == VMX_FP64) {
+        VMX_HIP(hipMemcpyAsync(st, c->spin[c->s_idx].p, 3 * nb, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        VMX_HIP(Ops<float>::to_double(c->stream, 3 * c->n, c->spin[c->s_idx].as<float>(), st));
+    }
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    VMX_HIP(hipMemcpy(sx, st, nb, hipMemcpyDeviceToHost));
+    VMX_HIP(hipMemcpy(sy, st + c->n, nb, hipMemcpyDeviceToHost));
+    VMX_HIP(hipMemcpy(sz, st + 2 * (size_t)c->n, nb, hipMemcpyDeviceToHost));
+    return VMX_OK;
+}
+
+int vmx_set_time_step(vmx_ctx* c, double dt, double half_dt) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(VMX_ERR_ARG, "dt must be positive and finite");
+    c->dt = dt;
+    c->half_dt = half_dt;
+    return VMX_OK;
+}
+
+int vmx_set_conditions(vmx_ctx* c, int32_t replica, double temperature, const double H[3],
+                       uint32_t field_flags) {
+    if (!c || !H) return fail(VMX_ERR_ARG, "null argument");
+    if (replica < -1 || replica >= c->nrep) return fail(VMX_ERR_ARG, "replica %d outside [-1,%d)", replica, c->nrep);
+    if (!(temperature >= 0.0) || !std::isfinite(temperature)) return fail(VMX_ERR_ARG, "temperature must be >= 0");
+    VMX_TRY(c->use_device());
+    const int r0 = replica < 0 ? 0 : replica, r1 = replica < 0 ? c->nrep : replica + 1;
+    for (int r = r0; r < r1; ++r) {
+        c->T_h[r] = temperature;
+        for (int k = 0; k < 3; ++k) c->H_h[3 * r + k] = H[k];
+    }
+    c->fld = field_flags & 0x7u;
+    return c->upload_conditions_any();
+}
+
+int vmx_llg_heun(vmx_ctx* c, int32_t nsteps, const double* noise) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    if (nsteps < 0) return fail(VMX_ERR_ARG, "nsteps must be >= 0");
+    if (!(c->dt > 0.0)) return fail(VMX_ERR_STATE, "time step not set (vmx_set_time_step)");
+    if (nsteps == 0) return VMX_OK;
+    VMX_TRY(c->use_device());
+    return c->precision == VMX_FP64 ? c->heun_steps<double>(nsteps, noise) : c->heun_steps<float>(nsteps, noise);
+}
+
+int vmx_llg_heun_graph(vmx_ctx* c, int32_t nsteps, int32_t nrepeat) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    if (nsteps <= 0 || nsteps % 2) return fail(VMX_ERR_ARG, "graph step count must be positive and even");
+    if (nrepeat < 0) return fail(VMX_ERR_ARG, "nrepeat must be >= 0");
+    if (c->kernel != VMX_KERNEL_FUSED) return fail(VMX_ERR_STATE, "graphs are recorded from the fused kernels");
+    if (!(c->dt > 0.0)) return fail(VMX_ERR_STATE, "time step not set (vmx_set_time_step)");
+    VMX_TRY(c->use_device());
+    if (!c->graph_exec || c->graph_key != c->graph_key_of(nsteps)) {
+        VMX_TRY(c->precision == VMX_FP64 ? c->capture_graph<double>(nsteps) : c->capture_graph<float>(nsteps));
+    }
+    VMX_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < nrepeat; ++r) VMX_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+    VMX_HIP(hipEventRecord(c->ev1, c->stream));
+    c->timed_steps = nsteps * nrepeat;
+    c->step += (uint64_t)nsteps * nrepeat;
+    return VMX_OK;
+}
+
+int vmx_get_step(vmx_ctx* c, int64_t* step) {
+    if (!c || !step) return fail(VMX_ERR_ARG, "null argument");
+    *step = (int64_t)c->step;
+    return VMX_OK;
+}
+
+int vmx_set_step(vmx_ctx* c, int64_t step) {
+    if (!c || step < 0) return fail(VMX_ERR_ARG, "bad argument");
+    VMX_TRY(c->use_device());
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    const unsigned long long v = (unsigned long long)step;
+    VMX_HIP(hipMemcpy(c->step_base.p, &v, sizeof v, hipMemcpyHostToDevice));
+    c->step = (uint64_t)step;
+    return VMX_OK;
+}
+
+int vmx_mc_sweep(vmx_ctx* c, int32_t nsweeps) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    if (nsweeps < 0) return fail(VMX_ERR_ARG, "nsweeps must be >= 0");
+    VMX_TRY(c->use_device());
+    return c->precision == VMX_FP64 ? c->mc_sweeps<double>(nsweeps) : c->mc_sweeps<float>(nsweeps);
+}
+
+int vmx_mc_get_state(vmx_ctx* c, double* sigma, double* moves, double* rejects) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    VMX_TRY(c->use_device());
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    const size_t b = c->nrep * sizeof(double);
+    if (sigma) VMX_HIP(hipMemcpy(sigma, c->mc_sigma.p, b, hipMemcpyDeviceToHost));
+    if (moves) VMX_HIP(hipMemcpy(moves, c->mc_tot_moves.p, b, hipMemcpyDeviceToHost));
+    if (rejects) VMX_HIP(hipMemcpy(rejects, c->mc_tot_rej.p, b, hipMemcpyDeviceToHost));
+    return VMX_OK;
+}
+
+int vmx_mc_set_sigma(vmx_ctx* c, int32_t replica, double sigma) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    if (replica < -1 || replica >= c->nrep) return fail(VMX_ERR_ARG, "replica out of range");
+    VMX_TRY(c->use_device());
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    const int r0 = replica < 0 ? 0 : replica, r1 = replica < 0 ? c->nrep : replica + 1;
+    for (int r = r0; r < r1; ++r)
+        VMX_HIP(hipMemcpy(c->mc_sigma.as<double>() + r, &sigma, sizeof(double), hipMemcpyHostToDevice));
+    return VMX_OK;
+}
+
+int vmx_mc_num_colours(vmx_ctx* c, int32_t* ncolours) {
+    if (!c || !ncolours) return fail(VMX_ERR_ARG, "null argument");
+    VMX_TRY(c->use_device());
+    if (!c->coloured) VMX_TRY(c->colour());
+    *ncolours = c->ncolours;
+    return VMX_OK;
+}
+
+static int check_mask(vmx_ctx* c, int32_t mask_size, const int32_t* mask, std::vector<int>& empty) {
+    if (mask_size < 1 || mask_size > 2048) return fail(VMX_ERR_ARG, "mask_size must be in [1,2048]");
+    if (!mask) return fail(VMX_ERR_ARG, "mask is null");
+    std::vector<int> count(mask_size, 0);
+    for (int i = 0; i < c->n; ++i) {
+        if (mask[i] < 0 || mask[i] >= mask_size)
+            return fail(VMX_ERR_ARG, "mask id %d of atom %d outside [0,%d)", mask[i], i, mask_size);
+        count[mask[i]]++;
+    }
+    for (int id = 0; id < mask_size - 1; ++id)
+        if (count[id] == 0) empty.push_back(id);
+    return VMX_OK;
+}
+
+int vmx_stats_add_magnetization(vmx_ctx* c, int32_t mask_size, const int32_t* mask, int32_t* handle) {
+    if (!c || !handle) return fail(VMX_ERR_ARG, "null argument");
+    VMX_TRY(c->use_device());
+    std::unique_ptr<MagStat> st(new MagStat());
+    VMX_TRY(check_mask(c, mask_size, mask, st->empty));
+    st->nbins = mask_size;
+    VMX_TRY(st->mask.alloc((size_t)c->n * sizeof(int)));
+    VMX_HIP(hipMemcpy(st->mask.p, mask, (size_t)c->n * sizeof(int), hipMemcpyHostToDevice));
+    VMX_TRY(st->cur.alloc(4 * (size_t)mask_size * sizeof(double)));
+    VMX_TRY(st->mean.alloc(4 * (size_t)mask_size * sizeof(double)));
+    VMX_HIP(hipMemset(st->cur.p, 0, 4 * (size_t)mask_size * sizeof(double)));
+    VMX_HIP(hipMemset(st->mean.p, 0, 4 * (size_t)mask_size * sizeof(double)));
+    VMX_TRY(st->empty_dev.alloc(std::max<size_t>(1, st->empty.size()) * sizeof(int)));
+    if (!st->empty.empty())
+        VMX_HIP(hipMemcpy(st->empty_dev.p, st->empty.data(), st->empty.size() * sizeof(int), hipMemcpyHostToDevice));
+    VMX_TRY(st->counter.alloc(sizeof(long long)));
+    VMX_HIP(hipMemset(st->counter.p, 0, sizeof(long long)));
+    *handle = (int32_t)c->mags.size();
+    c->mags.push_back(st.release());
+    return VMX_OK;
+}
+
+int vmx_stats_add_energy(vmx_ctx* c, int32_t mask_size, const int32_t* mask, int32_t* handle) {
+    if (!c || !handle) return fail(VMX_ERR_ARG, "null argument");
+    VMX_TRY(c->use_device());
+    std::unique_ptr<EnergyStat> st(new EnergyStat());
+    VMX_TRY(check_mask(c, mask_size, mask, st->empty));
+    st->nbins = mask_size;
+    VMX_TRY(st->mask.alloc((size_t)c->n * sizeof(int)));
+    VMX_HIP(hipMemcpy(st->mask.p, mask, (size_t)c->n * sizeof(int), hipMemcpyHostToDevice));
+    VMX_TRY(st->cur.alloc(5 * (size_t)mask_size * sizeof(double)));
+    VMX_TRY(st->mean.alloc(5 * (size_t)mask_size * sizeof(double)));
+    VMX_HIP(hipMemset(st->cur.p, 0, 5 * (size_t)mask_size * sizeof(double)));
+    VMX_HIP(hipMemset(st->mean.p, 0, 5 * (size_t)mask_size * sizeof(double)));
+    VMX_TRY(st->empty_dev.alloc(std::max<size_t>(1, st->empty.size()) * sizeof(int)));
+    if (!st->empty.empty())
+        VMX_HIP(hipMemcpy(st->empty_dev.p, st->empty.data(), st->empty.size() * sizeof(int), hipMemcpyHostToDevice));
+    VMX_TRY(st->counter.alloc(sizeof(long long)));
+    VMX_HIP(hipMemset(st->counter.p, 0, sizeof(long long)));
+    *handle = (int32_t)c->energies.size();
+    c->energies.push_back(st.release());
+    return VMX_OK;
+}
+
+int vmx_stats_update(vmx_ctx* c) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    VMX_TRY(c->use_device());
+    return c->precision == VMX_FP64 ? c->stats_update<double>() : c->stats_update<float>();
+}
+
+int vmx_stats_get_magnetization(vmx_ctx* c, int32_t handle, double* magnetization, double* mean_sum,
+                                int64_t* counter) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    if (handle < 0 || handle >= (int)c->mags.size()) return fail(VMX_ERR_ARG, "bad magnetization handle %d", handle);
+    VMX_TRY(c->use_device());
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    MagStat* st = c->mags[handle];
+    const size_t b = 4 * (size_t)st->nbins * sizeof(double);
+    if (magnetization) VMX_HIP(hipMemcpy(magnetization, st->cur.p, b, hipMemcpyDeviceToHost));
+    if (mean_sum) VMX_HIP(hipMemcpy(mean_sum, st->mean.p, b, hipMemcpyDeviceToHost));
+    if (counter) {
+        long long v = 0;
+        VMX_HIP(hipMemcpy(&v, st->counter.p, sizeof v, hipMemcpyDeviceToHost));
+        *counter = v;
+    }
+    return VMX_OK;
+}
+
+int vmx_stats_get_energy(vmx_ctx* c, int32_t handle, double* energy, double* mean_sum, int64_t* counter) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    if (handle < 0 || handle >= (int)c->energies.size()) return fail(VMX_ERR_ARG, "bad energy handle %d", handle);
+    VMX_TRY(c->use_device());
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    EnergyStat* st = c->energies[handle];
+    const size_t b = 5 * (size_t)st->nbins * sizeof(double);
+    if (energy) VMX_HIP(hipMemcpy(energy, st->cur.p, b, hipMemcpyDeviceToHost));
+    if (mean_sum) VMX_HIP(hipMemcpy(mean_sum, st->mean.p, b, hipMemcpyDeviceToHost));
+    if (counter) {
+        long long v = 0;
+        VMX_HIP(hipMemcpy(&v, st->counter.p, sizeof v, hipMemcpyDeviceToHost));
+        *counter = v;
+    }
+    return VMX_OK;
+}
+
+int vmx_stats_reset(vmx_ctx* c) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    VMX_TRY(c->use_device());
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    for (MagStat* st : c->mags) {
+        VMX_HIP(hipMemset(st->mean.p, 0, 4 * (size_t)st->nbins * sizeof(double)));
+        VMX_HIP(hipMemset(st->counter.p, 0, sizeof(long long)));
+    }
+    for (EnergyStat* st : c->energies) {
+        VMX_HIP(hipMemset(st->mean.p, 0, 5 * (size_t)st->nbins * sizeof(double)));
+        VMX_HIP(hipMemset(st->counter.p, 0, sizeof(long long)));
+    }
+    VMX_HIP(hipDeviceSynchronize());
+    return VMX_OK;
+}
+
+extern "C++" template <typename R>
+static int compute_fields_impl(vmx_ctx* c, const double* noise, double* h_spin, double* h_ext) {
+    VMX_TRY(c->plainF.alloc(3 * (size_t)c->n * c->rsize()));
+    VMX_TRY(c->plainE.alloc(3 * (size_t)c->n * c->rsize()));
+    const SysDev<R> s = c->sys<R>();
+    const Launch L = c->launch();
+    const double* nd = nullptr;
+    if (noise) {
+        VMX_TRY(c->noise_dev.alloc(3 * (size_t)c->n * sizeof(double)));
+        VMX_HIP(hipMemcpyAsync(c->noise_dev.p, noise, 3 * (size_t)c->n * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream));
+        nd = c->noise_dev.as<double>();
+    }
+    VMX_HIP(Ops<R>::spin_fields(L, c->et, c->jm, s, c->spins<R>(c->s_idx), c->arr3<R>(c->plainF)));
+    VMX_HIP(Ops<R>::ext_fields(L, s, c->arr3<R>(c->plainE), nd, 0u));
+    const size_t nb = 3 * (size_t)c->n * sizeof(double);
+    double* st = c->staging.as<double>();
+    for (int which = 0; which < 2; ++which) {
+        double* dst = which == 0 ? h_spin : h_ext;
+        if (!dst) continue;
+        const DevBuf& src = which == 0 ? c->plainF : c->plainE;
+        VMX_HIP(Ops<R>::to_double(c->stream, 3 * c->n, src.as<R>(), st));
+        VMX_HIP(hipStreamSynchronize(c->stream));
+        VMX_HIP(hipMemcpy(dst, st, nb, hipMemcpyDeviceToHost));
+    }
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    return VMX_OK;
+}
+
+int vmx_compute_fields(vmx_ctx* c, const double* noise, double* h_spin, double* h_ext) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    VMX_TRY(c->use_device());
+    return c->precision == VMX_FP64 ? compute_fields_impl<double>(c, noise, h_spin, h_ext)
+                                    : compute_fields_impl<float>(c, noise, h_spin, h_ext);
+}
+
+extern "C++" template <typename R>
+static int thermal_noise_impl(vmx_ctx* c, int64_t step, double* noise) {
+    const SysDev<R> s = c->sys<R>();
+    double* st = c->staging.as<double>();
+    VMX_HIP(Ops<R>::thermal_noise(c->launch(), s, (uint64_t)step, st));
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    VMX_HIP(hipMemcpy(noise, st, 3 * (size_t)c->n * sizeof(double), hipMemcpyDeviceToHost));
+    return VMX_OK;
+}
+
+int vmx_thermal_noise(vmx_ctx* c, int64_t step, double* noise) {
+    if (!c || !noise || step < 0) return fail(VMX_ERR_ARG, "bad argument");
+    VMX_TRY(c->use_device());
+    return c->precision == VMX_FP64 ? thermal_noise_impl<double>(c, step, noise)
+                                    : thermal_noise_impl<float>(c, step, noise);
+}
+
+int vmx_synchronize(vmx_ctx* c) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    VMX_TRY(c->use_device());
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    return VMX_OK;
+}
+
+int vmx_last_step_time_ms(vmx_ctx* c, double* ms) {
+    if (!c || !ms) return fail(VMX_ERR_ARG, "null argument");
+    VMX_TRY(c->use_device());
+    *ms = 0.0;
+    if (c->timed_steps <= 0) return VMX_OK;
+    VMX_HIP(hipEventSynchronize(c->ev1));
+    float t = 0.f;
+    VMX_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
+    *ms = (double)t / c->timed_steps;
+    return VMX_OK;
+}
+
+int vmx_time_kernels(vmx_ctx* c, int32_t nsteps, double* avg_ms, int32_t* nkernels) {
+    if (!c || !avg_ms || !nkernels) return fail(VMX_ERR_ARG, "null argument");
+    if (nsteps <= 0) return fail(VMX_ERR_ARG, "nsteps must be positive");
+    if (!(c->dt > 0.0)) return fail(VMX_ERR_STATE, "time step not set (vmx_set_time_step)");
+    VMX_TRY(c->use_device());
+    *nkernels = c->kernel == VMX_KERNEL_FUSED ? 2 : 5;
+    return c->precision == VMX_FP64 ? c->heun_steps<double>(nsteps, nullptr, avg_ms)
+                                    : c->heun_steps<float>(nsteps, nullptr, avg_ms);
+}
+
+int vmx_autotune(vmx_ctx* c, int32_t steps_per_trial, int32_t* chosen) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    if (steps_per_trial <= 0) return fail(VMX_ERR_ARG, "steps_per_trial must be positive");
+    if (!(c->dt > 0.0)) return fail(VMX_ERR_STATE, "time step not set (vmx_set_time_step)");
+    VMX_TRY(c->use_device());
+    // save spins and step counter
+    const size_t bytes = 3 * (size_t)c->n * c->rsize();
+    DevBuf save;
+    VMX_TRY(save.alloc(bytes));
+    VMX_HIP(hipMemcpyAsync(save.p, c->spin[c->s_idx].p, bytes, hipMemcpyDeviceToDevice, c->stream));
+    const uint64_t step0 = c->step;
+    const int s0 = c->s_idx, o0 = c->o_idx;
+    const int blocks[] = {64, 128, 256, 512, 1024};
+    double best = 1e300;
+    int best_b = c->block, best_x = c->xcd_remap;
+    for (int x = 0; x <= 1; ++x) {
+        for (int b : blocks) {
+            c->block = b;
+            c->xcd_remap = x;
+            VMX_TRY(c->build_blocks());
+            VMX_TRY(vmx_llg_heun(c, 2, nullptr));   // warm-up
+            VMX_TRY(vmx_llg_heun(c, steps_per_trial, nullptr));
+            double ms = 0.0;
+            VMX_TRY(vmx_last_step_time_ms(c, &ms));
+            if (ms > 0.0 && ms < best) { best = ms; best_b = b; best_x = x; }
+        }
+    }
+    c->block = best_b;
+    c->xcd_remap = best_x;
+    VMX_TRY(c->build_blocks());
+    c->s_idx = s0; c->o_idx = o0;
+    VMX_HIP(hipMemcpyAsync(c->spin[c->s_idx].p, save.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    VMX_TRY(vmx_set_step(c, (int64_t)step0));
+    if (chosen) *chosen = best_b;
+    return VMX_OK;
+}
+
+}  // extern "C"

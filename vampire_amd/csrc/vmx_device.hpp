@@ -1,0 +1,581 @@
+// vmx_device.hpp — device-side data structures and kernels of the spin-dynamics hot path.
+//
+// Included by vmx_f64.hip (R = double, compiled with -ffp-contract=off so every multiply and
+// add rounds exactly where the reference's C++ rounds) and vmx_f32.hip (R = float, FMA
+// contraction allowed).  Only the launchers declared in vmx_internal.hpp leave these units.
+//
+// Layout in HBM (one context):
+//   spins         three SoA arrays of R per buffer (x[N], y[N], z[N]); buffers S, S' and S_next
+//   topology      row[N+1] int32 CSR offsets, col[nnz] int32 neighbour ids (row order as given)
+//   exchange      deduplicated table of 1/3/9 R per distinct J; per-edge uint16 or int32 index
+//                 into it, or no per-edge data at all when every edge carries the same J
+//   type          uint8 material id per atom
+//   tables        material constants [M], conditions [replicas][M], block table, replica offsets
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vmx {
+
+// ---- plain-old-data shared with the host ------------------------------------------------
+enum { ET_ISO = 0, ET_VEC = 1, ET_TEN = 2 };      // exchange type (exchange::exchange_t)
+enum { JM_UNIFORM = 0, JM_IDX16 = 1, JM_IDX32 = 2 };  // how an edge finds its J
+
+enum : uint32_t {
+    ANI_UNI2 = 0x1u, ANI_CUB4 = 0x2u, ANI_CUB4R = 0x4u,
+    FLD_THERMAL = 0x1u, FLD_APPLIED = 0x2u, FLD_LOCAL = 0x4u
+};
+
+template <typename R>
+struct MatConst {            // per material, set once (mp::material after set_derived_parameters)
+    R one_opa2;              // -gamma_rel/(1+alpha^2)
+    R alpha_opa2;            // alpha * one_opa2
+    R ku2, ku[3];            // uniaxial constant (T) and axis
+    R kc4;                   // cubic constant (T)
+    R e1[3], e2[3], e3[3];   // rotated cubic basis
+    R mu_s_SI;               // for the Monte Carlo energy scale
+    R pad;
+};
+
+template <typename R>
+struct Cond {                // per (replica, material); rewritten when conditions change
+    R sigma;                 // sqrt(T_rescaled) * H_th_sigma   (src/simulate/fields.cpp:262-271)
+    R happ[3];               // applied field of the LLG step: H or H + H_local (fields.cpp:171-210)
+    R hglob[3];              // H of the applied-field energy (src/simulate/energy.cpp:94-98)
+    R kBTBohr;               // mu_B / (kB T_rescaled)            (src/montecarlo/mc.cpp:45-53)
+};
+
+template <typename R>
+struct SysDev {
+    int n;                                   // atoms over all replicas
+    int nmat;
+    int nblocks;                             // logical workgroups covering all replicas
+    int xcd_remap;                           // 1: contiguous logical blocks per XCD
+    const int* __restrict__ row;             // [n+1]
+    const int* __restrict__ col;             // [nnz]
+    const void* __restrict__ jidx;           // per-edge table index (uint16 / int32) or null
+    const R* __restrict__ jtab;              // exchange table
+    const uint8_t* __restrict__ type;        // [n]
+    const MatConst<R>* __restrict__ mat;     // [nmat]
+    const Cond<R>* __restrict__ cond;        // [nrep*nmat]
+    const int2* __restrict__ blk;            // [nblocks] {first atom, replica}
+    const int* __restrict__ rep_off;         // [nrep+1]
+    const uint2* __restrict__ rep_key;       // [nrep] Philox key
+    const unsigned long long* __restrict__ step_base;  // Heun steps completed before this launch batch
+    uint32_t ani;                            // ANI_* flags
+    uint32_t fld;                            // FLD_* flags
+    R dt, half_dt;
+};
+
+template <typename R>
+struct Spin3 {
+    R* __restrict__ x;
+    R* __restrict__ y;
+    R* __restrict__ z;
+};
+
+// ---- Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as
+// 1, 2, 3", SC'11): 10 rounds of the 4x32 S-box with Weyl key schedule ---------------------
+struct U4 { uint32_t v[4]; };
+
+__device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                            uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(M0, c0), lo0 = M0 * c0;
+        const uint32_t hi1 = __umulhi(M1, c2), lo1 = M1 * c2;
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += W0; k1 += W1;
+    }
+    U4 o; o.v[0] = c0; o.v[1] = c1; o.v[2] = c2; o.v[3] = c3;
+    return o;
+}
+
+// Stream ids: the fourth counter word separates independent uses of one (atom, step) pair.
+enum : uint32_t { STREAM_THERMAL = 0u, STREAM_MC_MOVE = 1u, STREAM_MC_ACCEPT = 2u };
+
+// Three standard normals from one Philox block by Box-Muller on (x0,x1) and (x2,x3);
+// uniforms are (k + 0.5) * 2^-32, never 0 or 1.
+__device__ __forceinline__ void box_muller3(const U4& u, double& g0, double& g1, double& g2) {
+    const double s = 2.3283064365386963e-10;  // 2^-32
+    const double twopi = 6.283185307179586;
+    const double u1 = ((double)u.v[0] + 0.5) * s, u2 = ((double)u.v[1] + 0.5) * s;
+    const double u3 = ((double)u.v[2] + 0.5) * s, u4 = ((double)u.v[3] + 0.5) * s;
+    const double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+    double sn, cs;
+    sincos(twopi * u2, &sn, &cs);
+    g0 = r1 * cs;
+    g1 = r1 * sn;
+    g2 = r2 * cos(twopi * u4);
+}
+
+// FP32 variant on the hardware transcendental units: v_log_f32 is log2, v_sin/v_cos_f32 take
+// their argument in revolutions, so sin(2 pi u) is one instruction.
+__device__ __forceinline__ void box_muller3(const U4& u, float& g0, float& g1, float& g2) {
+    const float u1 = ((float)(u.v[0] >> 8) + 0.5f) * 5.9604644775390625e-08f;  // 24-bit
+    const float u2 = ((float)(u.v[1] >> 8) + 0.5f) * 5.9604644775390625e-08f;
+    const float u3 = ((float)(u.v[2] >> 8) + 0.5f) * 5.9604644775390625e-08f;
+    const float u4 = ((float)(u.v[3] >> 8) + 0.5f) * 5.9604644775390625e-08f;
+    const float ln2x2 = 1.3862943611198906f;  // 2 ln 2
+    const float r1 = __builtin_amdgcn_sqrtf(-ln2x2 * __builtin_amdgcn_logf(u1));
+    const float r2 = __builtin_amdgcn_sqrtf(-ln2x2 * __builtin_amdgcn_logf(u3));
+    g0 = r1 * __builtin_amdgcn_cosf(u2);
+    g1 = r1 * __builtin_amdgcn_sinf(u2);
+    g2 = r2 * __builtin_amdgcn_cosf(u4);
+}
+
+template <typename R>
+__device__ __forceinline__ void thermal_normals(const SysDev<R>& s, int rep, int local, uint64_t step,
+                                                R& gx, R& gy, R& gz) {
+    const uint2 key = s.rep_key[rep];
+    const U4 u = philox4x32_10((uint32_t)local, (uint32_t)step, (uint32_t)(step >> 32),
+                               STREAM_THERMAL, key.x, key.y);
+    box_muller3(u, gx, gy, gz);
+}
+
+// ---- workgroup -> atoms ------------------------------------------------------------------
+// Workgroups are dispatched round-robin over the 8 XCDs.  With xcd_remap the grid is padded to
+// a multiple of 8 and hardware block b takes logical block (b%8)*(grid/8) + b/8, so each XCD
+// walks one contiguous stretch of atoms and the neighbour rows it gathers stay in its own L2.
+__device__ __forceinline__ int logical_block(int xcd_remap) {
+    const int b = (int)blockIdx.x;
+    if (!xcd_remap) return b;
+    const int per = (int)gridDim.x >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+template <typename R>
+__device__ __forceinline__ bool my_atom(const SysDev<R>& s, int& i, int& rep, int& local) {
+    const int lb = logical_block(s.xcd_remap);
+    if (lb >= s.nblocks) return false;
+    const int2 b = s.blk[lb];
+    rep = b.y;
+    i = b.x + (int)threadIdx.x;
+    const int r0 = s.rep_off[rep];
+    if (i >= s.rep_off[rep + 1]) return false;
+    local = i - r0;
+    return true;
+}
+
+// ---- spin-dependent field H_spin(S) at atom i ------------------------------------------------
+// Exchange: src/exchange/exchange_fields.cpp:52-78 (isotropic), :85-116 (vectorial),
+// :122-162 (tensorial); the sum starts at 0 and adds one neighbour at a time in row order.
+// Anisotropy terms are then added in the order of src/anisotropy/fields.cpp:44-59.
+template <typename R, int ET, int JM>
+__device__ __forceinline__ void spin_field(const SysDev<R>& s, const R* __restrict__ X,
+                                           const R* __restrict__ Y, const R* __restrict__ Z,
+                                           int i, int m, R sx, R sy, R sz,
+                                           R& fx, R& fy, R& fz) {
+    constexpr int K = (ET == ET_ISO) ? 1 : (ET == ET_VEC ? 3 : 9);
+    R hx = R(0), hy = R(0), hz = R(0);
+    const int beg = s.row[i], end = s.row[i + 1];
+    R ju[K];
+    if (JM == JM_UNIFORM) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) ju[k] = s.jtab[k];
+    }
+    for (int nn = beg; nn < end; ++nn) {
+        const int j = s.col[nn];
+        const R* J;
+        if (JM == JM_UNIFORM) {
+            J = ju;
+        } else if (JM == JM_IDX16) {
+            J = s.jtab + (size_t)K * ((const uint16_t*)s.jidx)[nn];
+        } else {
+            J = s.jtab + (size_t)K * ((const int*)s.jidx)[nn];
+        }
+        const R Sx = X[j], Sy = Y[j], Sz = Z[j];
+        if (ET == ET_ISO) {
+            const R Jij = J[0];
+            hx += Jij * Sx;
+            hy += Jij * Sy;
+            hz += Jij * Sz;
+        } else if (ET == ET_VEC) {
+            hx += J[0] * Sx;
+            hy += J[1] * Sy;
+            hz += J[2] * Sz;
+        } else {
+            hx += (J[0] * Sx + J[1] * Sy + J[2] * Sz);
+            hy += (J[3] * Sx + J[4] * Sy + J[5] * Sz);
+            hz += (J[6] * Sx + J[7] * Sy + J[8] * Sz);
+        }
+    }
+    // field array starts at zero (fields.cpp:79-81), exchange is added to it
+    fx = R(0); fy = R(0); fz = R(0);
+    fx += hx; fy += hy; fz += hz;
+    if (s.ani) {
+        const MatConst<R>& mc = s.mat[m];
+        if (s.ani & ANI_UNI2) {  // uniaxial_second_order.cpp:68-94, scale 2
+            const R ex = mc.ku[0], ey = mc.ku[1], ez = mc.ku[2];
+            const R sdote = (sx * ex + sy * ey + sz * ez);
+            const R k2 = R(2.0) * mc.ku2 * sdote;
+            fx += ex * k2; fy += ey * k2; fz += ez * k2;
+        }
+        if (s.ani & ANI_CUB4) {  // cubic_fourth_order.cpp:48-68, scale 0.5*4
+            const R k4 = R(2.0) * mc.kc4;
+            fx += sx * sx * sx * k4;
+            fy += sy * sy * sy * k4;
+            fz += sz * sz * sz * k4;
+        }
+        if (s.ani & ANI_CUB4R) {  // cubic_fourth_order_rotation.cpp:48-95
+            const R k4 = R(2.0) * mc.kc4;
+            const R d1 = sx * mc.e1[0] + sy * mc.e1[1] + sz * mc.e1[2];
+            const R d2 = sx * mc.e2[0] + sy * mc.e2[1] + sz * mc.e2[2];
+            const R d3 = sx * mc.e3[0] + sy * mc.e3[1] + sz * mc.e3[2];
+            const R d1c = d1 * d1 * d1, d2c = d2 * d2 * d2, d3c = d3 * d3 * d3;
+            fx += k4 * (d1c * mc.e1[0] + d2c * mc.e2[0] + d3c * mc.e3[0]);
+            fy += k4 * (d1c * mc.e1[1] + d2c * mc.e2[1] + d3c * mc.e3[1]);
+            fz += k4 * (d1c * mc.e1[2] + d2c * mc.e2[2] + d3c * mc.e3[2]);
+        }
+    }
+}
+
+// ---- external field at atom i (thermal + applied), src/simulate/fields.cpp:120-289 ----------
+// The array starts at 0; the thermal term overwrites it with N(0,1)*sigma (generate, then *=),
+// then the applied field is added.
+template <typename R>
+__device__ __forceinline__ void ext_field(const SysDev<R>& s, int rep, int m, int i, int local,
+                                          uint64_t step, const double* __restrict__ noise,
+                                          R& ex, R& ey, R& ez) {
+    const Cond<R>& c = s.cond[rep * s.nmat + m];
+    ex = R(0); ey = R(0); ez = R(0);
+    if (s.fld & FLD_THERMAL) {
+        R gx, gy, gz;
+        if (noise) {
+            gx = (R)noise[i];
+            gy = (R)noise[(size_t)s.n + i];
+            gz = (R)noise[2 * (size_t)s.n + i];
+        } else {
+            thermal_normals(s, rep, local, step, gx, gy, gz);
+        }
+        ex = gx * c.sigma; ey = gy * c.sigma; ez = gz * c.sigma;
+    }
+    if (s.fld & FLD_APPLIED) {
+        ex += c.happ[0]; ey += c.happ[1]; ez += c.happ[2];
+    }
+}
+
+// ---- LLG right-hand side, src/simulate/LLGHeun.cpp:201-203 (term order kept) --------------
+template <typename R>
+__device__ __forceinline__ void llg_rhs(R a, R b, R S0, R S1, R S2, R H0, R H1, R H2,
+                                        R& d0, R& d1, R& d2) {
+    d0 = (a) * (S1 * H2 - S2 * H1) + (b) * (S1 * (S0 * H1 - S1 * H0) - S2 * (S2 * H0 - S0 * H2));
+    d1 = (a) * (S2 * H0 - S0 * H2) + (b) * (S2 * (S1 * H2 - S2 * H1) - S0 * (S0 * H1 - S1 * H0));
+    d2 = (a) * (S0 * H1 - S1 * H0) + (b) * (S0 * (S2 * H0 - S0 * H2) - S1 * (S1 * H2 - S2 * H1));
+}
+
+__device__ __forceinline__ double inv_norm(double x, double y, double z) {
+    return 1.0 / sqrt(x * x + y * y + z * z);   // LLGHeun.cpp:216, :269
+}
+__device__ __forceinline__ float inv_norm(float x, float y, float z) {
+    return __builtin_amdgcn_rsqf(x * x + y * y + z * z);
+}
+
+// ======================= fused kernels: two launches per Heun step ========================
+// Predictor: S' = |S + dt f(S, H_spin(S) + H_ext)|, written to P.
+template <typename R, int ET, int JM>
+__global__ void __launch_bounds__(1024)
+k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, const double* __restrict__ noise, uint32_t step_off) {
+    int i, rep, local;
+    if (!my_atom(s, i, rep, local)) return;
+    const uint64_t step = *s.step_base + step_off;
+    const int m = s.type[i];
+    const R sx = S.x[i], sy = S.y[i], sz = S.z[i];
+    R fx, fy, fz, ex, ey, ez;
+    spin_field<R, ET, JM>(s, S.x, S.y, S.z, i, m, sx, sy, sz, fx, fy, fz);
+    ext_field(s, rep, m, i, local, step, noise, ex, ey, ez);
+    const MatConst<R>& mc = s.mat[m];
+    R d0, d1, d2;
+    llg_rhs(mc.one_opa2, mc.alpha_opa2, sx, sy, sz, fx + ex, fy + ey, fz + ez, d0, d1, d2);
+    R n0 = sx + d0 * s.dt, n1 = sy + d1 * s.dt, n2 = sz + d2 * s.dt;
+    const R r = inv_norm(n0, n1, n2);
+    P.x[i] = n0 * r; P.y[i] = n1 * r; P.z[i] = n2 * r;
+}
+
+// Corrector: recomputes dS at S (same noise), dS' at S', S_next = |S + dt/2 (dS + dS')|.
+template <typename R, int ET, int JM>
+__global__ void __launch_bounds__(1024)
+k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, const double* __restrict__ noise,
+                  uint32_t step_off) {
+    int i, rep, local;
+    if (!my_atom(s, i, rep, local)) return;
+    const uint64_t step = *s.step_base + step_off;
+    const int m = s.type[i];
+    const MatConst<R>& mc = s.mat[m];
+    R ex, ey, ez;
+    ext_field(s, rep, m, i, local, step, noise, ex, ey, ez);
+    const R sx = S.x[i], sy = S.y[i], sz = S.z[i];
+    R fx, fy, fz;
+    spin_field<R, ET, JM>(s, S.x, S.y, S.z, i, m, sx, sy, sz, fx, fy, fz);
+    R e0, e1, e2;
+    llg_rhs(mc.one_opa2, mc.alpha_opa2, sx, sy, sz, fx + ex, fy + ey, fz + ez, e0, e1, e2);
+    const R px = P.x[i], py = P.y[i], pz = P.z[i];
+    spin_field<R, ET, JM>(s, P.x, P.y, P.z, i, m, px, py, pz, fx, fy, fz);
+    R h0, h1, h2;
+    llg_rhs(mc.one_opa2, mc.alpha_opa2, px, py, pz, fx + ex, fy + ey, fz + ez, h0, h1, h2);
+    R n0 = sx + s.half_dt * (e0 + h0);
+    R n1 = sy + s.half_dt * (e1 + h1);
+    R n2 = sz + s.half_dt * (e2 + h2);
+    const R r = inv_norm(n0, n1, n2);
+    O.x[i] = n0 * r; O.y[i] = n1 * r; O.z[i] = n2 * r;
+}
+
+// ======================= plain kernels: the reference dataflow ==============================
+template <typename R, int ET, int JM>
+__global__ void __launch_bounds__(1024)
+k_plain_spin_fields(SysDev<R> s, Spin3<R> S, Spin3<R> F) {
+    int i, rep, local;
+    if (!my_atom(s, i, rep, local)) return;
+    const int m = s.type[i];
+    R fx, fy, fz;
+    spin_field<R, ET, JM>(s, S.x, S.y, S.z, i, m, S.x[i], S.y[i], S.z[i], fx, fy, fz);
+    F.x[i] = fx; F.y[i] = fy; F.z[i] = fz;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(1024)
+k_plain_ext_fields(SysDev<R> s, Spin3<R> E, const double* __restrict__ noise, uint32_t step_off) {
+    int i, rep, local;
+    if (!my_atom(s, i, rep, local)) return;
+    const uint64_t step = *s.step_base + step_off;
+    R ex, ey, ez;
+    ext_field(s, rep, (int)s.type[i], i, local, step, noise, ex, ey, ez);
+    E.x[i] = ex; E.y[i] = ey; E.z[i] = ez;
+}
+
+// Euler predictor (LLGHeun.cpp:188-226): stores dS in D and S' in P.
+template <typename R>
+__global__ void __launch_bounds__(1024)
+k_plain_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> F, Spin3<R> E, Spin3<R> D, Spin3<R> P) {
+    int i, rep, local;
+    if (!my_atom(s, i, rep, local)) return;
+    const MatConst<R>& mc = s.mat[s.type[i]];
+    const R sx = S.x[i], sy = S.y[i], sz = S.z[i];
+    R d0, d1, d2;
+    llg_rhs(mc.one_opa2, mc.alpha_opa2, sx, sy, sz, F.x[i] + E.x[i], F.y[i] + E.y[i], F.z[i] + E.z[i],
+            d0, d1, d2);
+    D.x[i] = d0; D.y[i] = d1; D.z[i] = d2;
+    R n0 = sx + d0 * s.dt, n1 = sy + d1 * s.dt, n2 = sz + d2 * s.dt;
+    const R r = inv_norm(n0, n1, n2);
+    P.x[i] = n0 * r; P.y[i] = n1 * r; P.z[i] = n2 * r;
+}
+
+// Heun corrector (LLGHeun.cpp:239-279): S <- |S0 + dt/2 (dS + dS')| in place.
+template <typename R>
+__global__ void __launch_bounds__(1024)
+k_plain_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> F, Spin3<R> E, Spin3<R> D) {
+    int i, rep, local;
+    if (!my_atom(s, i, rep, local)) return;
+    const MatConst<R>& mc = s.mat[s.type[i]];
+    const R px = P.x[i], py = P.y[i], pz = P.z[i];
+    R h0, h1, h2;
+    llg_rhs(mc.one_opa2, mc.alpha_opa2, px, py, pz, F.x[i] + E.x[i], F.y[i] + E.y[i], F.z[i] + E.z[i],
+            h0, h1, h2);
+    R n0 = S.x[i] + s.half_dt * (D.x[i] + h0);
+    R n1 = S.y[i] + s.half_dt * (D.y[i] + h1);
+    R n2 = S.z[i] + s.half_dt * (D.z[i] + h2);
+    const R r = inv_norm(n0, n1, n2);
+    S.x[i] = n0 * r; S.y[i] = n1 * r; S.z[i] = n2 * r;
+}
+
+// ======================= statistics ==========================================================
+// Per-mask sums in FP64.  Each thread keeps a register accumulator for the mask id of its
+// current run of atoms and flushes it into the workgroup's LDS histogram when the id changes,
+// so a system-wide mask (one id) costs one LDS atomic per thread.  Each workgroup writes its
+// histogram to partial[block][nbins*W]; k_stats_finish reduces the partials in block order.
+
+// W = 4: sum(m s_x), sum(m s_y), sum(m s_z), sum(m)   (magnetization.cpp:141-147)
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_mag_partial(int n, Spin3<R> S, const int* __restrict__ mask, const double* __restrict__ mm,
+              int nbins, double* __restrict__ partial) {
+    extern __shared__ double hist[];
+    for (int k = threadIdx.x; k < 4 * nbins; k += blockDim.x) hist[k] = 0.0;
+    __syncthreads();
+    int cur = -1;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int id = mask[i];
+        if (id != cur) {
+            if (cur >= 0) {
+                atomicAdd(&hist[4 * cur + 0], a0); atomicAdd(&hist[4 * cur + 1], a1);
+                atomicAdd(&hist[4 * cur + 2], a2); atomicAdd(&hist[4 * cur + 3], a3);
+            }
+            cur = id; a0 = a1 = a2 = a3 = 0.0;
+        }
+        const double m = mm[i];
+        a0 += (double)S.x[i] * m;
+        a1 += (double)S.y[i] * m;
+        a2 += (double)S.z[i] * m;
+        a3 += m;
+    }
+    if (cur >= 0) {
+        atomicAdd(&hist[4 * cur + 0], a0); atomicAdd(&hist[4 * cur + 1], a1);
+        atomicAdd(&hist[4 * cur + 2], a2); atomicAdd(&hist[4 * cur + 3], a3);
+    }
+    __syncthreads();
+    double* out = partial + (size_t)blockIdx.x * 4 * nbins;
+    for (int k = threadIdx.x; k < 4 * nbins; k += blockDim.x) out[k] = hist[k];
+}
+
+// Single-spin energies (tesla): exchange (src/exchange/energy.cpp:27-104), anisotropy
+// (src/anisotropy/energy.cpp:31-50 with uniaxial_second_order.cpp:104-121,
+// cubic_fourth_order.cpp:80-95, cubic_fourth_order_rotation.cpp:104-140) and applied field
+// (src/simulate/energy.cpp:94-98).
+template <typename R, int ET, int JM>
+__device__ __forceinline__ R exchange_energy(const SysDev<R>& s, const R* __restrict__ X,
+                                             const R* __restrict__ Y, const R* __restrict__ Z,
+                                             int i, R sx, R sy, R sz) {
+    constexpr int K = (ET == ET_ISO) ? 1 : (ET == ET_VEC ? 3 : 9);
+    R energy = R(0);
+    const int beg = s.row[i], end = s.row[i + 1];
+    for (int nn = beg; nn < end; ++nn) {
+        const int j = s.col[nn];
+        const R* J;
+        if (JM == JM_UNIFORM) J = s.jtab;
+        else if (JM == JM_IDX16) J = s.jtab + (size_t)K * ((const uint16_t*)s.jidx)[nn];
+        else J = s.jtab + (size_t)K * ((const int*)s.jidx)[nn];
+        const R Sx = X[j], Sy = Y[j], Sz = Z[j];
+        if (ET == ET_ISO) {
+            energy -= J[0] * (Sx * sx + Sy * sy + Sz * sz);
+        } else if (ET == ET_VEC) {
+            energy -= (J[0] * Sx * sx + J[1] * Sy * sy + J[2] * Sz * sz);
+        } else {
+            energy -= (J[0] * Sx * sx + J[1] * Sy * sx + J[2] * Sz * sx +
+                       J[3] * Sx * sy + J[4] * Sy * sy + J[5] * Sz * sy +
+                       J[6] * Sx * sz + J[7] * Sy * sz + J[8] * Sz * sz);
+        }
+    }
+    return energy;
+}
+
+template <typename R>
+__device__ __forceinline__ R anisotropy_energy(const SysDev<R>& s, int m, R sx, R sy, R sz) {
+    R energy = R(0);
+    if (!s.ani) return energy;
+    const MatConst<R>& mc = s.mat[m];
+    if (s.ani & ANI_UNI2) {
+        const R sdote = (sx * mc.ku[0] + sy * mc.ku[1] + sz * mc.ku[2]);
+        energy += -mc.ku2 * (sdote * sdote);
+    }
+    if (s.ani & ANI_CUB4) {
+        const R sx4 = sx * sx * sx * sx, sy4 = sy * sy * sy * sy, sz4 = sz * sz * sz * sz;
+        energy += R(-0.5) * mc.kc4 * (sx4 + sy4 + sz4);
+    }
+    if (s.ani & ANI_CUB4R) {
+        const R d1 = sx * mc.e1[0] + sy * mc.e1[1] + sz * mc.e1[2];
+        const R d2 = sx * mc.e2[0] + sy * mc.e2[1] + sz * mc.e2[2];
+        const R d3 = sx * mc.e3[0] + sy * mc.e3[1] + sz * mc.e3[2];
+        const R d1f = d1 * d1 * d1 * d1, d2f = d2 * d2 * d2 * d2, d3f = d3 * d3 * d3 * d3;
+        energy += R(-0.5) * mc.kc4 * (d1f + d2f + d3f);
+    }
+    return energy;
+}
+
+template <typename R>
+__device__ __forceinline__ R applied_energy(const Cond<R>& c, R sx, R sy, R sz) {
+    return -(c.hglob[0] * sx + c.hglob[1] * sy + c.hglob[2] * sz);
+}
+
+// W = 3: exchange, anisotropy, applied, each times m_i (energy.cpp:155-196)
+template <typename R, int ET, int JM>
+__global__ void __launch_bounds__(256)
+k_energy_partial(SysDev<R> s, Spin3<R> S, const int* __restrict__ mask, const double* __restrict__ mm,
+                 const int* __restrict__ atom_rep, int nbins, double* __restrict__ partial) {
+    extern __shared__ double hist[];
+    for (int k = threadIdx.x; k < 3 * nbins; k += blockDim.x) hist[k] = 0.0;
+    __syncthreads();
+    int cur = -1;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
+        const int id = mask[i];
+        if (id != cur) {
+            if (cur >= 0) {
+                atomicAdd(&hist[3 * cur + 0], a0); atomicAdd(&hist[3 * cur + 1], a1);
+                atomicAdd(&hist[3 * cur + 2], a2);
+            }
+            cur = id; a0 = a1 = a2 = 0.0;
+        }
+        const R sx = S.x[i], sy = S.y[i], sz = S.z[i];
+        const int m = s.type[i];
+        const int rep = atom_rep ? atom_rep[i] : 0;
+        const double w = mm[i];
+        a0 += (double)exchange_energy<R, ET, JM>(s, S.x, S.y, S.z, i, sx, sy, sz) * w;
+        a1 += (double)anisotropy_energy(s, m, sx, sy, sz) * w;
+        a2 += (double)applied_energy(s.cond[rep * s.nmat + m], sx, sy, sz) * w;
+    }
+    if (cur >= 0) {
+        atomicAdd(&hist[3 * cur + 0], a0); atomicAdd(&hist[3 * cur + 1], a1);
+        atomicAdd(&hist[3 * cur + 2], a2);
+    }
+    __syncthreads();
+    double* out = partial + (size_t)blockIdx.x * 3 * nbins;
+    for (int k = threadIdx.x; k < 3 * nbins; k += blockDim.x) out[k] = hist[k];
+}
+
+// ======================= checkerboard Metropolis =============================================
+// One launch per colour.  Atoms of a colour have no neighbour of the same colour, so each move
+// sees fixed neighbours and the updates of one launch commute.  Move and acceptance follow
+// montecarlo::mc_step (src/montecarlo/mc.cpp:59-108) with the adaptive cone of
+// mc_moves.cpp:71-87,158-162.
+template <typename R>
+__device__ __forceinline__ int replica_of(const SysDev<R>& s, int nrep, int i) {
+    int lo = 0, hi = nrep;                 // rep_off[lo] <= i < rep_off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s.rep_off[mid] <= i) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <typename R, int ET, int JM>
+__global__ void __launch_bounds__(256)
+k_mc_colour(SysDev<R> s, Spin3<R> S, const int* __restrict__ list, int count, int nrep,
+            const double* __restrict__ sigma, uint64_t sweep, double* __restrict__ rejects) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double rej = 0.0;
+    int myrep = -1;
+    if (t < count) {
+        const int i = list[t];
+        const int rep = replica_of(s, nrep, i);
+        myrep = rep;
+        const int local = i - s.rep_off[rep];
+        const int m = s.type[i];
+        const Cond<R>& c = s.cond[rep * s.nmat + m];
+        const uint2 key = s.rep_key[rep];
+        const U4 um = philox4x32_10((uint32_t)local, (uint32_t)sweep, (uint32_t)(sweep >> 32),
+                                    STREAM_MC_MOVE, key.x, key.y);
+        R g0, g1, g2;
+        box_muller3(um, g0, g1, g2);
+        const R angle = (R)sigma[rep];
+        const R ox = S.x[i], oy = S.y[i], oz = S.z[i];
+        R nx = ox + g0 * angle, ny = oy + g1 * angle, nz = oz + g2 * angle;
+        const R r = inv_norm(nx, ny, nz);
+        nx *= r; ny *= r; nz *= r;
+        const R e_old = (R(0) + exchange_energy<R, ET, JM>(s, S.x, S.y, S.z, i, ox, oy, oz)
+                         + anisotropy_energy(s, m, ox, oy, oz)) + applied_energy(c, ox, oy, oz);
+        const R e_new = (R(0) + exchange_energy<R, ET, JM>(s, S.x, S.y, S.z, i, nx, ny, nz)
+                         + anisotropy_energy(s, m, nx, ny, nz)) + applied_energy(c, nx, ny, nz);
+        const R de = (e_new - e_old) * s.mat[m].mu_s_SI * R(1.07828231e23);
+        bool accept = de < R(0);
+        if (!accept) {
+            const U4 ua = philox4x32_10((uint32_t)local, (uint32_t)sweep, (uint32_t)(sweep >> 32),
+                                        STREAM_MC_ACCEPT, key.x, key.y);
+            const double u = (double)ua.v[0] * 2.3283064365386963e-10;   // [0,1)
+            accept = exp((double)(-de * c.kBTBohr)) >= u;
+        }
+        if (accept) {
+            S.x[i] = nx; S.y[i] = ny; S.z[i] = nz;
+        } else {
+            rej = 1.0;
+        }
+    }
+    // rejects per replica: a workgroup spans at most a few replicas; reduce when uniform
+    if (myrep >= 0 && rej != 0.0) atomicAdd(&rejects[myrep], rej);
+}
+
+}  // namespace vmx

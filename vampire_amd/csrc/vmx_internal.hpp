@@ -1,0 +1,54 @@
+// vmx_internal.hpp — launchers exported by the precision units (vmx_f64.hip, vmx_f32.hip) to
+// the context code (vmx_api.hip).  Each launcher picks the kernel instantiation for the
+// exchange type and J-storage mode of the context and queues it on `stream`.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vmx_device.hpp"
+
+namespace vmx {
+
+struct Launch {
+    int grid;        // hardware workgroups (padded to a multiple of 8 when XCD remapping)
+    int block;       // threads per workgroup
+    hipStream_t stream;
+};
+
+template <typename R>
+struct Ops {
+    // fused Heun step: predictor writes P from S; corrector writes O from S and P.  The Philox
+    // step of a launch is *s.step_base + step_off.
+    static hipError_t fused_predictor(const Launch& L, int et, int jm, const SysDev<R>& s, Spin3<R> S,
+                                      Spin3<R> P, const double* noise, uint32_t step_off);
+    static hipError_t fused_corrector(const Launch& L, int et, int jm, const SysDev<R>& s, Spin3<R> S,
+                                      Spin3<R> P, Spin3<R> O, const double* noise, uint32_t step_off);
+    // plain Heun step
+    static hipError_t spin_fields(const Launch& L, int et, int jm, const SysDev<R>& s, Spin3<R> S, Spin3<R> F);
+    static hipError_t ext_fields(const Launch& L, const SysDev<R>& s, Spin3<R> E, const double* noise,
+                                 uint32_t step_off);
+    static hipError_t plain_predictor(const Launch& L, const SysDev<R>& s, Spin3<R> S, Spin3<R> F,
+                                      Spin3<R> E, Spin3<R> D, Spin3<R> P);
+    static hipError_t plain_corrector(const Launch& L, const SysDev<R>& s, Spin3<R> S, Spin3<R> P,
+                                      Spin3<R> F, Spin3<R> E, Spin3<R> D);
+    // statistics partial sums (grid-stride, `grid` workgroups of 256)
+    static hipError_t mag_partial(int grid, hipStream_t st, int n, Spin3<R> S, const int* mask,
+                                  const double* mm, int nbins, double* partial);
+    static hipError_t energy_partial(int grid, hipStream_t st, int et, int jm, const SysDev<R>& s, Spin3<R> S,
+                                     const int* mask, const double* mm, const int* atom_rep, int nbins,
+                                     double* partial);
+    // Monte Carlo: one colour
+    static hipError_t mc_colour(hipStream_t st, int et, int jm, const SysDev<R>& s, Spin3<R> S, const int* list,
+                                int count, int nrep, const double* sigma, uint64_t sweep, double* rejects);
+    // Philox normals of `step` as double [3][n]
+    static hipError_t thermal_noise(const Launch& L, const SysDev<R>& s, uint64_t step, double* out);
+    // conversions between the double host/staging layout and R device arrays
+    static hipError_t from_double(hipStream_t st, int n, const double* in, R* out);
+    static hipError_t to_double(hipStream_t st, int n, const R* in, double* out);
+};
+
+extern template struct Ops<double>;
+extern template struct Ops<float>;
+
+}  // namespace vmx
