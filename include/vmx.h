@@ -117,12 +117,26 @@ typedef struct vmx_system {
 typedef struct vmx_options {
     int32_t precision;    /* VMX_FP64 or VMX_FP32                                              */
     int32_t kernel;       /* VMX_KERNEL_PLAIN or VMX_KERNEL_FUSED                              */
-    int32_t block_size;   /* threads per workgroup, 64..1024 in multiples of 64; 0 = 256       */
+    int32_t block_size;   /* threads per workgroup, 64..512 in multiples of 64; 0 = 256        */
     int32_t device;       /* HIP device ordinal; -1 = current device                            */
     uint64_t seed;        /* Philox key for thermal noise and Monte Carlo moves                 */
+    int32_t topology;     /* VMX_TOPOLOGY_*: device layout of the neighbour list                */
 } vmx_options;
 
+/* Device layout of the neighbour list.  Every layout keeps each row's neighbour order, so all
+ * give identical results; they differ only in bytes moved per row. */
+enum {
+    VMX_TOPOLOGY_AUTO = 0,     /* stencil if the rows fit a small table, else ELL, else CSR      */
+    VMX_TOPOLOGY_CSR = 1,      /* row offsets + neighbour ids, as handed over                    */
+    VMX_TOPOLOGY_ELL = 2,      /* neighbour ids transposed to [slot][atom] (coalesced)            */
+    VMX_TOPOLOGY_STENCIL = 3   /* per-atom id of a (j - i offsets, J ids) stencil: rows of a
+                                  periodic lattice cost 2 bytes                                   */
+};
+
 typedef struct vmx_ctx vmx_ctx;
+
+/* Layout chosen for a context (VMX_TOPOLOGY_CSR/ELL/STENCIL) and the number of stencils. */
+int vmx_get_topology(vmx_ctx* ctx, int32_t* topology, int32_t* num_stencils);
 
 /* Text of the last error on this thread ("" if none). */
 const char* vmx_last_error(void);

@@ -14,8 +14,21 @@ pytestmark = pytest.mark.gpu
 GOLDEN_CASES = ["sc6_uniaxial", "fcc3_cubic", "bcc4_rotated", "film_dmi"]
 
 
-def _ctx(system, precision=FP64, kernel=FUSED, seed=0, block=0):
-    return Context(system, precision=precision, kernel=kernel, seed=seed, block_size=block)
+TOPOLOGIES = ["csr", "ell", "stencil"]
+
+
+def _ctx(system, precision=FP64, kernel=FUSED, seed=0, block=0, topology="auto"):
+    return Context(system, precision=precision, kernel=kernel, seed=seed, block_size=block, topology=topology)
+
+
+@pytest.mark.parametrize("topology", TOPOLOGIES)
+def test_topology_is_honoured(gpu, topology):
+    s = S.ferromagnet("fcc", 5)
+    with _ctx(s, topology=topology) as c:
+        assert c.topology[0] == topology
+    with _ctx(s) as c:   # a periodic lattice: automatic choice is the stencil layout
+        name, npat = c.topology
+        assert name == "stencil" and 4 <= npat <= 4 * 27
 
 
 def test_device_and_library(gpu):
@@ -52,16 +65,18 @@ def test_fields_match_reference(gpu, name):
     assert np.abs(he - ref[3:]).max() <= 2.3e-16 * scale
 
 
+@pytest.mark.parametrize("topology", TOPOLOGIES)
 @pytest.mark.parametrize("kernel", [PLAIN, FUSED])
 @pytest.mark.parametrize("name", GOLDEN_CASES)
-def test_trajectory_matches_reference(gpu, name, kernel):
-    """Device FP64 fed the reference's noise stream reproduces sim::LLG_Heun: <= 1e-10 per spin."""
+def test_trajectory_matches_reference(gpu, name, kernel, topology):
+    """Device FP64 fed the reference's noise stream reproduces sim::LLG_Heun: <= 1e-10 per spin,
+    for every neighbour-list layout."""
     f = RC.load(name)
     s = RC.system_of(name)
     T, H = RC.conditions(name)
     seed = int(f["noise_seed"])
     steps = [int(v) for v in f["snap_steps"]]
-    with _ctx(s, kernel=kernel) as c:
+    with _ctx(s, kernel=kernel, topology=topology) as c:
         c.set_conditions(T, H)
         c.set_spins(f["spins_in"])
         last = 0

@@ -11,15 +11,16 @@ from vampire_amd.context import FP32, FP64, FUSED, Context
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("topology", ["csr", "ell", "stencil"])
 @pytest.mark.parametrize("name", ["sc6_uniaxial", "fcc3_cubic", "bcc4_rotated", "film_dmi"])
-def test_stats_match_reference(gpu, name):
+def test_stats_match_reference(gpu, name, topology):
     """System and material magnetisation and system energies after the reference's steps
     (stats_after = system M [8], material M [4(nmat+1)], system E [5x2])."""
     f = RC.load(name)
     s = RC.system_of(name)
     T, H = RC.conditions(name)
     nmat = len(s.materials)
-    with Context(s, precision=FP64) as c:
+    with Context(s, precision=FP64, topology=topology) as c:
         c.set_conditions(T, H)
         c.set_spins(f["snaps"][-1])
         hs = c.add_magnetization(np.zeros(s.n, np.int32), 2)
@@ -72,7 +73,8 @@ def test_stats_running_means(gpu, precision):
         assert cnt == 0 and not mean.any()
 
 
-def test_mc_checkerboard_matches_oracle(gpu):
+@pytest.mark.parametrize("topology", ["csr", "ell", "stencil"])
+def test_mc_checkerboard_matches_oracle(gpu, topology):
     """FP64 checkerboard sweeps on the device reproduce the oracle's colour-ordered Metropolis
     (same colouring, Philox streams and acceptance), including the adaptive cone width."""
     s = S.ferromagnet("sc", 6)
@@ -80,7 +82,7 @@ def test_mc_checkerboard_matches_oracle(gpu):
     o.set_conditions(400.0, (0.0, 0.0, 0.2))
     spins = S.random_spins(s.n, 21)
     ref, sigma, moves, rej = o.mc_checkerboard(spins, 8, seed=77, sweep0=0, sigma=60.0)
-    with Context(s, precision=FP64, seed=77) as c:
+    with Context(s, precision=FP64, seed=77, topology=topology) as c:
         c.set_conditions(400.0, (0.0, 0.0, 0.2))
         c.set_spins(spins)
         assert c.mc_num_colours() == 2

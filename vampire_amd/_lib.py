@@ -17,6 +17,7 @@ VMX_OK, VMX_ERR_ARG, VMX_ERR_HIP, VMX_ERR_STATE, VMX_ERR_NOMEM = 0, 1, 2, 3, 4
 VMX_EXCHANGE_ISOTROPIC, VMX_EXCHANGE_VECTORIAL, VMX_EXCHANGE_TENSORIAL = 0, 1, 2
 VMX_FP64, VMX_FP32 = 0, 1
 VMX_KERNEL_PLAIN, VMX_KERNEL_FUSED = 0, 1
+VMX_TOPOLOGY_AUTO, VMX_TOPOLOGY_CSR, VMX_TOPOLOGY_ELL, VMX_TOPOLOGY_STENCIL = 0, 1, 2, 3
 VMX_ANISOTROPY_UNIAXIAL_SECOND_ORDER = 0x1
 VMX_ANISOTROPY_CUBIC_FOURTH_ORDER = 0x2
 VMX_ANISOTROPY_CUBIC_FOURTH_ROTATION = 0x4
@@ -87,6 +88,7 @@ class vmx_options(C.Structure):
         ("block_size", C.c_int32),
         ("device", C.c_int32),
         ("seed", C.c_uint64),
+        ("topology", C.c_int32),
     ]
 
 
@@ -100,6 +102,7 @@ SIGNATURES = [
     ("vmx_device_count", C.c_int, [_P(C.c_int32)]),
     ("vmx_create", C.c_int, [_P(vmx_system), _P(vmx_options), _P(_VP)]),
     ("vmx_destroy", C.c_int, [_VP]),
+    ("vmx_get_topology", C.c_int, [_VP, _P(C.c_int32), _P(C.c_int32)]),
     ("vmx_set_spins", C.c_int, [_VP, _DP, _DP, _DP]),
     ("vmx_get_spins", C.c_int, [_VP, _DP, _DP, _DP]),
     ("vmx_set_time_step", C.c_int, [_VP, C.c_double, C.c_double]),

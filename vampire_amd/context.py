@@ -18,6 +18,7 @@ FP64 = "fp64"
 FP32 = "fp32"
 PLAIN = "plain"
 FUSED = "fused"
+TOPOLOGIES = {"auto": 0, "csr": 1, "ell": 2, "stencil": 3}
 
 
 def device_count() -> int:
@@ -43,7 +44,7 @@ class Context:
     """A device context for one system (or a batch of replicas)."""
 
     def __init__(self, system: System, precision: str = FP64, kernel: str = FUSED, block_size: int = 0,
-                 seed: int = 0, device: int = -1):
+                 seed: int = 0, device: int = -1, topology: str = "auto"):
         self.lib = L.load()
         self.system = system
         self.n = system.n
@@ -65,7 +66,8 @@ class Context:
             num_replicas=s.num_replicas, replica_offset=None if roff is None else L.iptr(roff))
         opt = L.vmx_options(precision=L.VMX_FP64 if precision == FP64 else L.VMX_FP32,
                             kernel=L.VMX_KERNEL_FUSED if kernel == FUSED else L.VMX_KERNEL_PLAIN,
-                            block_size=block_size, device=device, seed=seed)
+                            block_size=block_size, device=device, seed=seed,
+                            topology=TOPOLOGIES[topology])
         h = C.c_void_p()
         L.check(self.lib.vmx_create(C.byref(sysc), C.byref(opt), C.byref(h)))
         self.h = h
@@ -112,6 +114,13 @@ class Context:
 
     def set_time_step(self, dt: float, half_dt: Optional[float] = None) -> None:
         L.check(self.lib.vmx_set_time_step(self.h, dt, 0.5 * dt if half_dt is None else half_dt))
+
+    @property
+    def topology(self):
+        """(layout name, number of stencils) chosen for the neighbour list."""
+        t, p = C.c_int32(0), C.c_int32(0)
+        L.check(self.lib.vmx_get_topology(self.h, C.byref(t), C.byref(p)))
+        return {1: "csr", 2: "ell", 3: "stencil"}[t.value], int(p.value)
 
     @property
     def step(self) -> int:

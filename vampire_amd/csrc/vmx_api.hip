@@ -210,6 +210,9 @@ struct vmx_ctx {
     int n = 0, nmat = 0, nrep = 1;
     int64_t nnz = 0;
     int et = 0, jm = 0, K = 1;
+    int tp = TP_CSR, zmax = 0, npat = 0, wsel = 0;   // zmax: row stride W for ELL / stencil
+    int am() const { return am_of(tp, jm, wsel); }
+    std::vector<int> rowh_, colh_;   // host CSR (graph colouring for Monte Carlo)
     uint32_t ani = 0, fld = 0;
     double dt = 0.0, half_dt = 0.0;
     uint64_t step = 0;       // host mirror of *step_base
@@ -222,6 +225,7 @@ struct vmx_ctx {
 
     // device state
     DevBuf row, col, jidx, jtab, type, mat, cond, blk, rep_off, rep_key, step_base;
+    DevBuf deg, pat, pat_off, pat_jid, pat_deg;
     DevBuf spin[3];               // three buffers of 3N R each (S, P, O roles rotate)
     int s_idx = 0, o_idx = 2;     // current spin buffer and the corrector's output buffer
     DevBuf plainF, plainE, plainD;
@@ -280,8 +284,15 @@ struct vmx_ctx {
     SysDev<R> sys() const {
         SysDev<R> s;
         s.n = n; s.nmat = nmat; s.nblocks = nblocks; s.xcd_remap = xcd_remap;
+        s.zmax = zmax;
+        s.am = am();
         s.row = row.as<int>(); s.col = col.as<int>();
         s.jidx = jm == JM_UNIFORM ? nullptr : jidx.p;
+        s.deg = deg.as<uint8_t>();
+        s.pat = pat.as<uint16_t>();
+        s.pat_off = pat_off.as<int>();
+        s.pat_jid = pat_jid.as<uint16_t>();
+        s.pat_deg = pat_deg.as<uint8_t>();
         s.jtab = jtab.as<R>();
         s.type = type.as<uint8_t>();
         s.mat = mat.as<MatConst<R>>();
@@ -417,9 +428,9 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
         int nk = 0;
         if (kernel == VMX_KERNEL_FUSED) {
             VMX_HIP(mark(0));
-            VMX_HIP(Ops<R>::fused_predictor(L, et, jm, s, spins<R>(s_idx), spins<R>(p_idx), nd, (uint32_t)k));
+            VMX_HIP(Ops<R>::fused_predictor(L, et, am(),s, spins<R>(s_idx), spins<R>(p_idx), nd, (uint32_t)k));
             VMX_HIP(mark(1));
-            VMX_HIP(Ops<R>::fused_corrector(L, et, jm, s, spins<R>(s_idx), spins<R>(p_idx), spins<R>(o_idx), nd,
+            VMX_HIP(Ops<R>::fused_corrector(L, et, am(),s, spins<R>(s_idx), spins<R>(p_idx), spins<R>(o_idx), nd,
                                             (uint32_t)k));
             VMX_HIP(mark(2));
             nk = 2;
@@ -427,13 +438,13 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
         } else {
             const Spin3<R> F = arr3<R>(plainF), E = arr3<R>(plainE), D = arr3<R>(plainD);
             VMX_HIP(mark(0));
-            VMX_HIP(Ops<R>::spin_fields(L, et, jm, s, spins<R>(s_idx), F));
+            VMX_HIP(Ops<R>::spin_fields(L, et, am(),s, spins<R>(s_idx), F));
             VMX_HIP(mark(1));
             VMX_HIP(Ops<R>::ext_fields(L, s, E, nd, (uint32_t)k));
             VMX_HIP(mark(2));
             VMX_HIP(Ops<R>::plain_predictor(L, s, spins<R>(s_idx), F, E, D, spins<R>(p_idx)));
             VMX_HIP(mark(3));
-            VMX_HIP(Ops<R>::spin_fields(L, et, jm, s, spins<R>(p_idx), F));
+            VMX_HIP(Ops<R>::spin_fields(L, et, am(),s, spins<R>(p_idx), F));
             VMX_HIP(mark(4));
             VMX_HIP(Ops<R>::plain_corrector(L, s, spins<R>(s_idx), spins<R>(p_idx), F, E, D));
             VMX_HIP(mark(5));
@@ -469,9 +480,9 @@ int vmx_ctx::capture_graph(int nsteps) {
     int si = s_idx, oi = o_idx;
     for (int k = 0; k < nsteps; ++k) {
         const int pi = 3 - si - oi;
-        hipError_t e = Ops<R>::fused_predictor(L, et, jm, s, spins<R>(si), spins<R>(pi), nullptr, (uint32_t)k);
+        hipError_t e = Ops<R>::fused_predictor(L, et, am(),s, spins<R>(si), spins<R>(pi), nullptr, (uint32_t)k);
         if (e == hipSuccess)
-            e = Ops<R>::fused_corrector(L, et, jm, s, spins<R>(si), spins<R>(pi), spins<R>(oi), nullptr, (uint32_t)k);
+            e = Ops<R>::fused_corrector(L, et, am(),s, spins<R>(si), spins<R>(pi), spins<R>(oi), nullptr, (uint32_t)k);
         if (e != hipSuccess) {
             hipGraph_t g;
             (void)hipStreamEndCapture(stream, &g);
@@ -511,7 +522,7 @@ int vmx_ctx::stats_update() {
         }
         for (EnergyStat* st : energies) {
             VMX_TRY(partial.alloc((size_t)grid * 3 * st->nbins * sizeof(double)));
-            VMX_HIP(Ops<R>::energy_partial(grid, stream, et, jm, s, S, st->mask.as<int>(), mm.as<double>(),
+            VMX_HIP(Ops<R>::energy_partial(grid, stream, et, am(),s, S, st->mask.as<int>(), mm.as<double>(),
                                            nrep > 1 ? atom_rep.as<int>() : nullptr, st->nbins,
                                            partial.as<double>()));
             hipLaunchKernelGGL(k_energy_finish, dim3(1), dim3(256), 0, stream, grid, partial.as<double>(),
@@ -526,9 +537,8 @@ int vmx_ctx::stats_update() {
 // Greedy graph colouring in atom order over the symmetrised neighbour graph, so that no two
 // atoms of one colour interact in either direction.
 int vmx_ctx::colour() {
-    std::vector<int> rowh(n + 1), colh((size_t)nnz);
-    VMX_HIP(hipMemcpy(rowh.data(), row.p, (n + 1) * sizeof(int), hipMemcpyDeviceToHost));
-    VMX_HIP(hipMemcpy(colh.data(), col.p, (size_t)nnz * sizeof(int), hipMemcpyDeviceToHost));
+    const std::vector<int>& rowh = rowh_;
+    const std::vector<int>& colh = colh_;
     // reverse adjacency
     std::vector<int> rrow(n + 1, 0), rcol((size_t)nnz);
     for (int64_t e = 0; e < nnz; ++e) rrow[colh[e] + 1]++;
@@ -578,7 +588,7 @@ int vmx_ctx::mc_sweeps(int nsweeps) {
     const Spin3<R> S = spins<R>(s_idx);
     for (int w = 0; w < nsweeps; ++w) {
         for (int k = 0; k < ncolours; ++k) {
-            VMX_HIP(Ops<R>::mc_colour(stream, et, jm, s, S, colour_list.as<int>() + colour_start[k],
+            VMX_HIP(Ops<R>::mc_colour(stream, et, am(),s, S, colour_list.as<int>() + colour_start[k],
                                       colour_start[k + 1] - colour_start[k], nrep, mc_sigma.as<double>(), sweep,
                                       mc_rej.as<double>()));
         }
@@ -652,7 +662,132 @@ int compress_exchange(const vmx_system* sys, const std::vector<int>& iid_csr, in
     return VMX_OK;
 }
 
+// Stencils: a row is (degree, j-i for each slot, J id for each slot).  Rows of a periodic lattice
+// repeat a few of them (per basis atom, times the boundary-wrap cases), so a row can be coded as
+// a 16-bit stencil id.  Returns false when the rows do not fit `max_pat` stencils.
+constexpr int kMaxStencils = 4096;
+
+bool build_stencils(int n, int zmax, const std::vector<int>& rowh, const std::vector<int>& colh,
+                    const std::vector<uint16_t>& jid, std::vector<uint16_t>& pat, std::vector<int>& off,
+                    std::vector<uint16_t>& pjid, std::vector<uint8_t>& pdeg) {
+    std::unordered_map<std::string, int> ids;
+    pat.resize(n);
+    std::string key;
+    std::vector<int> row_off(zmax);
+    std::vector<uint16_t> row_jid(zmax);
+    for (int i = 0; i < n; ++i) {
+        const int b = rowh[i], d = rowh[i + 1] - rowh[i];
+        key.assign(1, (char)d);
+        for (int k = 0; k < d; ++k) {
+            row_off[k] = colh[b + k] - i;
+            row_jid[k] = jid.empty() ? 0 : jid[b + k];
+        }
+        key.append((const char*)row_off.data(), d * sizeof(int));
+        key.append((const char*)row_jid.data(), d * sizeof(uint16_t));
+        auto it = ids.find(key);
+        int p;
+        if (it == ids.end()) {
+            p = (int)ids.size();
+            if (p >= kMaxStencils) return false;
+            ids.emplace(key, p);
+            off.resize((size_t)(p + 1) * zmax, 0);
+            pjid.resize((size_t)(p + 1) * zmax, 0);
+            pdeg.push_back((uint8_t)d);
+            for (int k = 0; k < d; ++k) {
+                off[(size_t)p * zmax + k] = row_off[k];
+                pjid[(size_t)p * zmax + k] = row_jid[k];
+            }
+        } else {
+            p = it->second;
+        }
+        pat[i] = (uint16_t)p;
+    }
+    return true;
+}
+
+template <typename T>
+int upload(DevBuf& b, const std::vector<T>& v) {
+    VMX_TRY(b.alloc(std::max<size_t>(1, v.size()) * sizeof(T)));
+    if (!v.empty()) VMX_HIP(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return VMX_OK;
+}
+
 }  // namespace
+
+// Chooses and uploads the device layout of the neighbour list (VMX_TOPOLOGY_*).
+static int build_topology(vmx_ctx* c, int request, const std::vector<int>& rowh, const std::vector<int>& colh,
+                          const std::vector<uint16_t>& idx16, const std::vector<int>& idx32) {
+    const int n = c->n;
+    int zmax = 0;
+    for (int i = 0; i < n; ++i) zmax = std::max(zmax, rowh[i + 1] - rowh[i]);
+    if (request < VMX_TOPOLOGY_AUTO || request > VMX_TOPOLOGY_STENCIL)
+        return fail(VMX_ERR_ARG, "unknown topology %d", request);
+    // slot width: the narrowest compiled width that holds the widest row
+    int wsel = -1;
+    for (int w = 0; w < 4; ++w)
+        if (zmax <= kSlotWidths[w]) { wsel = w; break; }
+    const bool slotted = wsel >= 0;
+    const int W = slotted ? kSlotWidths[wsel] : 0;
+    c->zmax = zmax;
+    if ((request == VMX_TOPOLOGY_AUTO || request == VMX_TOPOLOGY_STENCIL) && c->jm != JM_IDX32 && slotted) {
+        std::vector<uint16_t> pat, pjid;
+        std::vector<int> off;
+        std::vector<uint8_t> pdeg;
+        if (build_stencils(n, W, rowh, colh, c->jm == JM_IDX16 ? idx16 : std::vector<uint16_t>(), pat, off, pjid,
+                           pdeg)) {
+            c->tp = TP_PAT;
+            c->wsel = wsel;
+            c->zmax = W;
+            c->npat = (int)pdeg.size();
+            VMX_TRY(upload(c->pat, pat));
+            VMX_TRY(upload(c->pat_off, off));
+            VMX_TRY(upload(c->pat_jid, pjid));
+            VMX_TRY(upload(c->pat_deg, pdeg));
+            return VMX_OK;
+        }
+        if (request == VMX_TOPOLOGY_STENCIL)
+            return fail(VMX_ERR_ARG, "rows do not fit %d stencils", kMaxStencils);
+    } else if (request == VMX_TOPOLOGY_STENCIL) {
+        return fail(VMX_ERR_ARG, "stencil layout needs at most 65536 distinct J values and rows of <= %d",
+                    kSlotWidths[3]);
+    }
+    // ELL when padding every row to W slots costs at most half again the CSR entries
+    const int64_t nnz = c->nnz;
+    const bool ell_ok = slotted && (int64_t)W * n <= nnz + nnz / 2 + n;
+    if ((request == VMX_TOPOLOGY_AUTO && ell_ok) || request == VMX_TOPOLOGY_ELL) {
+        if (!slotted) return fail(VMX_ERR_ARG, "ELL layout needs rows of at most %d neighbours", kSlotWidths[3]);
+        const size_t slots = (size_t)W * n;
+        std::vector<int> col(slots, 0);
+        std::vector<uint8_t> deg(n);
+        std::vector<uint16_t> j16(c->jm == JM_IDX16 ? slots : 0, 0);
+        std::vector<int> j32(c->jm == JM_IDX32 ? slots : 0, 0);
+        for (int i = 0; i < n; ++i) {
+            const int b = rowh[i], d = rowh[i + 1] - rowh[i];
+            deg[i] = (uint8_t)d;
+            for (int k = 0; k < W; ++k) {
+                const size_t e = (size_t)k * n + i;
+                col[e] = k < d ? colh[b + k] : i;   // padding: the atom itself (in bounds, cached)
+                if (c->jm == JM_IDX16 && k < d) j16[e] = idx16[b + k];
+                if (c->jm == JM_IDX32 && k < d) j32[e] = idx32[b + k];
+            }
+        }
+        c->tp = TP_ELL;
+        c->wsel = wsel;
+        c->zmax = W;
+        VMX_TRY(upload(c->col, col));
+        VMX_TRY(upload(c->deg, deg));
+        if (c->jm == JM_IDX16) VMX_TRY(upload(c->jidx, j16));
+        if (c->jm == JM_IDX32) VMX_TRY(upload(c->jidx, j32));
+        return VMX_OK;
+    }
+    c->tp = TP_CSR;
+    c->wsel = 0;
+    VMX_TRY(upload(c->row, rowh));
+    VMX_TRY(upload(c->col, colh));
+    if (c->jm == JM_IDX16) VMX_TRY(upload(c->jidx, idx16));
+    if (c->jm == JM_IDX32) VMX_TRY(upload(c->jidx, idx32));
+    return VMX_OK;
+}
 
 // ============================================================================================
 extern "C" {
@@ -687,7 +822,7 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
     if (opt->precision != VMX_FP64 && opt->precision != VMX_FP32) return fail(VMX_ERR_ARG, "bad precision");
     if (opt->kernel != VMX_KERNEL_PLAIN && opt->kernel != VMX_KERNEL_FUSED) return fail(VMX_ERR_ARG, "bad kernel");
     const int bs = opt->block_size ? opt->block_size : 256;
-    if (bs < 64 || bs > 1024 || bs % 64) return fail(VMX_ERR_ARG, "block_size must be a multiple of 64 in [64,1024]");
+    if (bs < 64 || bs > 512 || bs % 64) return fail(VMX_ERR_ARG, "block_size must be a multiple of 64 in [64,512]");
     const int nrep = sys->num_replicas < 1 ? 1 : sys->num_replicas;
     std::vector<int> roff(nrep + 1);
     if (nrep == 1) {
@@ -766,17 +901,9 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
     VMX_TRY(compress_exchange(sys, iidh, K, c->jm, table, idx16, idx32));
 
     const size_t rs = c->rsize();
-    VMX_TRY(c->row.alloc((n + 1) * sizeof(int)));
-    VMX_HIP(hipMemcpy(c->row.p, rowh.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
-    VMX_TRY(c->col.alloc(std::max<size_t>(1, colh.size()) * sizeof(int)));
-    if (!colh.empty()) VMX_HIP(hipMemcpy(c->col.p, colh.data(), colh.size() * sizeof(int), hipMemcpyHostToDevice));
-    if (c->jm == JM_IDX16) {
-        VMX_TRY(c->jidx.alloc(idx16.size() * sizeof(uint16_t)));
-        VMX_HIP(hipMemcpy(c->jidx.p, idx16.data(), idx16.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
-    } else if (c->jm == JM_IDX32) {
-        VMX_TRY(c->jidx.alloc(idx32.size() * sizeof(int)));
-        VMX_HIP(hipMemcpy(c->jidx.p, idx32.data(), idx32.size() * sizeof(int), hipMemcpyHostToDevice));
-    }
+    VMX_TRY(build_topology(c.get(), opt->topology, rowh, colh, idx16, idx32));
+    c->rowh_.swap(rowh);
+    c->colh_.swap(colh);
     VMX_TRY(c->jtab.alloc(table.size() * rs));
     if (c->precision == VMX_FP64) {
         VMX_HIP(hipMemcpy(c->jtab.p, table.data(), table.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -826,6 +953,13 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
     }
     VMX_HIP(hipDeviceSynchronize());
     *out = c.release();
+    return VMX_OK;
+}
+
+int vmx_get_topology(vmx_ctx* c, int32_t* topology, int32_t* num_stencils) {
+    if (!c || !topology) return fail(VMX_ERR_ARG, "null argument");
+    *topology = c->tp == TP_PAT ? VMX_TOPOLOGY_STENCIL : (c->tp == TP_ELL ? VMX_TOPOLOGY_ELL : VMX_TOPOLOGY_CSR);
+    if (num_stencils) *num_stencils = c->tp == TP_PAT ? c->npat : 0;
     return VMX_OK;
 }
 
@@ -1103,7 +1237,7 @@ static int compute_fields_impl(vmx_ctx* c, const double* noise, double* h_spin, 
                                c->stream));
         nd = c->noise_dev.as<double>();
     }
-    VMX_HIP(Ops<R>::spin_fields(L, c->et, c->jm, s, c->spins<R>(c->s_idx), c->arr3<R>(c->plainF)));
+    VMX_HIP(Ops<R>::spin_fields(L, c->et, c->am(), s, c->spins<R>(c->s_idx), c->arr3<R>(c->plainF)));
     VMX_HIP(Ops<R>::ext_fields(L, s, c->arr3<R>(c->plainE), nd, 0u));
     const size_t nb = 3 * (size_t)c->n * sizeof(double);
     double* st = c->staging.as<double>();
@@ -1184,7 +1318,7 @@ int vmx_autotune(vmx_ctx* c, int32_t steps_per_trial, int32_t* chosen) {
     VMX_HIP(hipMemcpyAsync(save.p, c->spin[c->s_idx].p, bytes, hipMemcpyDeviceToDevice, c->stream));
     const uint64_t step0 = c->step;
     const int s0 = c->s_idx, o0 = c->o_idx;
-    const int blocks[] = {64, 128, 256, 512, 1024};
+    const int blocks[] = {64, 128, 256, 512};
     double best = 1e300;
     int best_b = c->block, best_x = c->xcd_remap;
     for (int x = 0; x <= 1; ++x) {

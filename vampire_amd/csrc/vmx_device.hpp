@@ -21,6 +21,28 @@ namespace vmx {
 // ---- plain-old-data shared with the host ------------------------------------------------
 enum { ET_ISO = 0, ET_VEC = 1, ET_TEN = 2 };      // exchange type (exchange::exchange_t)
 enum { JM_UNIFORM = 0, JM_IDX16 = 1, JM_IDX32 = 2 };  // how an edge finds its J
+// How a row finds its neighbours.  All three keep the reference's per-row neighbour order.
+//   TP_CSR  row offsets + neighbour ids (the reference layout)
+//   TP_ELL  neighbour ids transposed to [slot][atom] with a per-atom degree: a wavefront's 64
+//           rows read one coalesced 256-byte line per slot
+//   TP_PAT  per-atom stencil id into a small table of (degree, j - i offsets, J ids): rows of a
+//           periodic lattice share a handful of stencils, so a row costs 2 bytes of topology
+enum { TP_CSR = 0, TP_ELL = 1, TP_PAT = 2 };
+// ELL and stencil rows have a compile-time slot width W (rows padded with in-bounds entries);
+// CSR rows are walked with a run-time loop.
+constexpr int kSlotWidths[4] = {6, 8, 12, 16};
+constexpr int kNumModes = 23;
+// Access mode (a kernel template parameter): 0..2 CSR x J mode; 3..14 ELL x J mode x width;
+// 15..22 stencil x {uniform, 16-bit J id} x width.
+constexpr int am_of(int tp, int jm, int wsel) {
+    return tp == TP_CSR ? jm : (tp == TP_ELL ? 3 + jm * 4 + wsel : 15 + jm * 4 + wsel);
+}
+template <int AM>
+struct Access {
+    static constexpr int TP = AM < 3 ? TP_CSR : (AM < 15 ? TP_ELL : TP_PAT);
+    static constexpr int JM = AM < 3 ? AM : (AM < 15 ? (AM - 3) / 4 : (AM - 15) / 4);
+    static constexpr int W = AM < 3 ? 0 : kSlotWidths[AM < 15 ? (AM - 3) % 4 : (AM - 15) % 4];
+};
 
 enum : uint32_t {
     ANI_UNI2 = 0x1u, ANI_CUB4 = 0x2u, ANI_CUB4R = 0x4u,
@@ -52,9 +74,16 @@ struct SysDev {
     int nmat;
     int nblocks;                             // logical workgroups covering all replicas
     int xcd_remap;                           // 1: contiguous logical blocks per XCD
-    const int* __restrict__ row;             // [n+1]
-    const int* __restrict__ col;             // [nnz]
-    const void* __restrict__ jidx;           // per-edge table index (uint16 / int32) or null
+    int zmax;                                // widest row (ELL slots, stencil width)
+    int am;                                  // access mode the arrays below were built for
+    const int* __restrict__ row;             // CSR: [n+1]
+    const int* __restrict__ col;             // CSR: [nnz]; ELL: [zmax][n]
+    const void* __restrict__ jidx;           // per-edge table index (uint16 / int32), CSR or ELL layout
+    const uint8_t* __restrict__ deg;         // ELL: [n] row lengths
+    const uint16_t* __restrict__ pat;        // PAT: [n] stencil id
+    const int* __restrict__ pat_off;         // PAT: [npat][zmax] j - i
+    const uint16_t* __restrict__ pat_jid;    // PAT: [npat][zmax] J table index
+    const uint8_t* __restrict__ pat_deg;     // PAT: [npat] row lengths
     const R* __restrict__ jtab;              // exchange table
     const uint8_t* __restrict__ type;        // [n]
     const MatConst<R>* __restrict__ mat;     // [nmat]
@@ -161,34 +190,95 @@ __device__ __forceinline__ bool my_atom(const SysDev<R>& s, int& i, int& rep, in
     return true;
 }
 
-// ---- spin-dependent field H_spin(S) at atom i ------------------------------------------------
-// Exchange: src/exchange/exchange_fields.cpp:52-78 (isotropic), :85-116 (vectorial),
-// :122-162 (tensorial); the sum starts at 0 and adds one neighbour at a time in row order.
-// Anisotropy terms are then added in the order of src/anisotropy/fields.cpp:44-59.
-template <typename R, int ET, int JM>
-__device__ __forceinline__ void spin_field(const SysDev<R>& s, const R* __restrict__ X,
-                                           const R* __restrict__ Y, const R* __restrict__ Z,
-                                           int i, int m, R sx, R sy, R sz,
-                                           R& fx, R& fy, R& fz) {
+// ---- neighbours of atom i, in row order ------------------------------------------------------
+// Calls f(Sx, Sy, Sz, J) for every neighbour with its spin and a pointer to its 1/3/9 exchange
+// reals.  ELL and stencil rows first issue every index load of the row, then every spin load,
+// and only then accumulate (in row order, slots k < degree): the latency of a row is paid once
+// instead of once per neighbour.  Padding slots load in-bounds entries and are not accumulated.
+template <typename R, int ET, int AM, typename F>
+__device__ __forceinline__ void for_neighbours(const SysDev<R>& s, const R* __restrict__ X,
+                                               const R* __restrict__ Y, const R* __restrict__ Z,
+                                               int i, F&& f) {
     constexpr int K = (ET == ET_ISO) ? 1 : (ET == ET_VEC ? 3 : 9);
-    R hx = R(0), hy = R(0), hz = R(0);
-    const int beg = s.row[i], end = s.row[i + 1];
+    constexpr int TP = Access<AM>::TP, JM = Access<AM>::JM, W = Access<AM>::W;
     R ju[K];
     if (JM == JM_UNIFORM) {
 #pragma unroll
         for (int k = 0; k < K; ++k) ju[k] = s.jtab[k];
     }
-    for (int nn = beg; nn < end; ++nn) {
-        const int j = s.col[nn];
-        const R* J;
-        if (JM == JM_UNIFORM) {
-            J = ju;
-        } else if (JM == JM_IDX16) {
-            J = s.jtab + (size_t)K * ((const uint16_t*)s.jidx)[nn];
-        } else {
-            J = s.jtab + (size_t)K * ((const int*)s.jidx)[nn];
+    if constexpr (TP == TP_CSR) {
+        const int beg = s.row[i], end = s.row[i + 1];
+        for (int nn = beg; nn < end; ++nn) {
+            const R* J = ju;
+            if (JM == JM_IDX16) J = s.jtab + (size_t)K * ((const uint16_t*)s.jidx)[nn];
+            if (JM == JM_IDX32) J = s.jtab + (size_t)K * ((const int*)s.jidx)[nn];
+            const int j = s.col[nn];
+            f(X[j], Y[j], Z[j], J);
         }
-        const R Sx = X[j], Sy = Y[j], Sz = Z[j];
+    } else {
+        int jj[W];
+        int jd[W];
+        int deg;
+        if constexpr (TP == TP_ELL) {
+            deg = s.deg[i];
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                const size_t e = (size_t)k * (size_t)s.n + (size_t)i;
+                jj[k] = s.col[e];
+                if (JM == JM_IDX16) jd[k] = ((const uint16_t*)s.jidx)[e];
+                if (JM == JM_IDX32) jd[k] = ((const int*)s.jidx)[e];
+            }
+        } else {
+            const int p = s.pat[i];
+            deg = s.pat_deg[p];
+            const int* __restrict__ off = s.pat_off + p * W;
+            if constexpr (W % 4 == 0) {
+#pragma unroll
+                for (int q = 0; q < W / 4; ++q) {
+                    const int4 o = reinterpret_cast<const int4*>(off)[q];
+                    jj[4 * q + 0] = i + o.x; jj[4 * q + 1] = i + o.y;
+                    jj[4 * q + 2] = i + o.z; jj[4 * q + 3] = i + o.w;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < W / 2; ++q) {
+                    const int2 o = reinterpret_cast<const int2*>(off)[q];
+                    jj[2 * q + 0] = i + o.x; jj[2 * q + 1] = i + o.y;
+                }
+            }
+            if (JM != JM_UNIFORM) {
+#pragma unroll
+                for (int k = 0; k < W; ++k) jd[k] = s.pat_jid[p * W + k];
+            }
+        }
+        R xs[W], ys[W], zs[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            xs[k] = X[jj[k]];
+            ys[k] = Y[jj[k]];
+            zs[k] = Z[jj[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            if (k < deg) {
+                const R* J = JM == JM_UNIFORM ? ju : s.jtab + (size_t)K * jd[k];
+                f(xs[k], ys[k], zs[k], J);
+            }
+        }
+    }
+}
+
+// ---- spin-dependent field H_spin(S) at atom i ------------------------------------------------
+// Exchange: src/exchange/exchange_fields.cpp:52-78 (isotropic), :85-116 (vectorial),
+// :122-162 (tensorial); the sum starts at 0 and adds one neighbour at a time in row order.
+// Anisotropy terms are then added in the order of src/anisotropy/fields.cpp:44-59.
+template <typename R, int ET, int AM>
+__device__ __forceinline__ void spin_field(const SysDev<R>& s, const R* __restrict__ X,
+                                           const R* __restrict__ Y, const R* __restrict__ Z,
+                                           int i, int m, R sx, R sy, R sz,
+                                           R& fx, R& fy, R& fz) {
+    R hx = R(0), hy = R(0), hz = R(0);
+    for_neighbours<R, ET, AM>(s, X, Y, Z, i, [&](const R Sx, const R Sy, const R Sz, const R* J) {
         if (ET == ET_ISO) {
             const R Jij = J[0];
             hx += Jij * Sx;
@@ -203,7 +293,7 @@ __device__ __forceinline__ void spin_field(const SysDev<R>& s, const R* __restri
             hy += (J[3] * Sx + J[4] * Sy + J[5] * Sz);
             hz += (J[6] * Sx + J[7] * Sy + J[8] * Sz);
         }
-    }
+    });
     // field array starts at zero (fields.cpp:79-81), exchange is added to it
     fx = R(0); fy = R(0); fz = R(0);
     fx += hx; fy += hy; fz += hz;
@@ -277,8 +367,8 @@ __device__ __forceinline__ float inv_norm(float x, float y, float z) {
 
 // ======================= fused kernels: two launches per Heun step ========================
 // Predictor: S' = |S + dt f(S, H_spin(S) + H_ext)|, written to P.
-template <typename R, int ET, int JM>
-__global__ void __launch_bounds__(1024)
+template <typename R, int ET, int AM>
+__global__ void __launch_bounds__(512)
 k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, const double* __restrict__ noise, uint32_t step_off) {
     int i, rep, local;
     if (!my_atom(s, i, rep, local)) return;
@@ -286,7 +376,7 @@ k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, const double* __restrict_
     const int m = s.type[i];
     const R sx = S.x[i], sy = S.y[i], sz = S.z[i];
     R fx, fy, fz, ex, ey, ez;
-    spin_field<R, ET, JM>(s, S.x, S.y, S.z, i, m, sx, sy, sz, fx, fy, fz);
+    spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m, sx, sy, sz, fx, fy, fz);
     ext_field(s, rep, m, i, local, step, noise, ex, ey, ez);
     const MatConst<R>& mc = s.mat[m];
     R d0, d1, d2;
@@ -297,8 +387,8 @@ k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, const double* __restrict_
 }
 
 // Corrector: recomputes dS at S (same noise), dS' at S', S_next = |S + dt/2 (dS + dS')|.
-template <typename R, int ET, int JM>
-__global__ void __launch_bounds__(1024)
+template <typename R, int ET, int AM>
+__global__ void __launch_bounds__(512)
 k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, const double* __restrict__ noise,
                   uint32_t step_off) {
     int i, rep, local;
@@ -310,11 +400,11 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, const double*
     ext_field(s, rep, m, i, local, step, noise, ex, ey, ez);
     const R sx = S.x[i], sy = S.y[i], sz = S.z[i];
     R fx, fy, fz;
-    spin_field<R, ET, JM>(s, S.x, S.y, S.z, i, m, sx, sy, sz, fx, fy, fz);
+    spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m, sx, sy, sz, fx, fy, fz);
     R e0, e1, e2;
     llg_rhs(mc.one_opa2, mc.alpha_opa2, sx, sy, sz, fx + ex, fy + ey, fz + ez, e0, e1, e2);
     const R px = P.x[i], py = P.y[i], pz = P.z[i];
-    spin_field<R, ET, JM>(s, P.x, P.y, P.z, i, m, px, py, pz, fx, fy, fz);
+    spin_field<R, ET, AM>(s, P.x, P.y, P.z, i, m, px, py, pz, fx, fy, fz);
     R h0, h1, h2;
     llg_rhs(mc.one_opa2, mc.alpha_opa2, px, py, pz, fx + ex, fy + ey, fz + ez, h0, h1, h2);
     R n0 = sx + s.half_dt * (e0 + h0);
@@ -325,19 +415,19 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, const double*
 }
 
 // ======================= plain kernels: the reference dataflow ==============================
-template <typename R, int ET, int JM>
-__global__ void __launch_bounds__(1024)
+template <typename R, int ET, int AM>
+__global__ void __launch_bounds__(512)
 k_plain_spin_fields(SysDev<R> s, Spin3<R> S, Spin3<R> F) {
     int i, rep, local;
     if (!my_atom(s, i, rep, local)) return;
     const int m = s.type[i];
     R fx, fy, fz;
-    spin_field<R, ET, JM>(s, S.x, S.y, S.z, i, m, S.x[i], S.y[i], S.z[i], fx, fy, fz);
+    spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m, S.x[i], S.y[i], S.z[i], fx, fy, fz);
     F.x[i] = fx; F.y[i] = fy; F.z[i] = fz;
 }
 
 template <typename R>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(512)
 k_plain_ext_fields(SysDev<R> s, Spin3<R> E, const double* __restrict__ noise, uint32_t step_off) {
     int i, rep, local;
     if (!my_atom(s, i, rep, local)) return;
@@ -349,7 +439,7 @@ k_plain_ext_fields(SysDev<R> s, Spin3<R> E, const double* __restrict__ noise, ui
 
 // Euler predictor (LLGHeun.cpp:188-226): stores dS in D and S' in P.
 template <typename R>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(512)
 k_plain_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> F, Spin3<R> E, Spin3<R> D, Spin3<R> P) {
     int i, rep, local;
     if (!my_atom(s, i, rep, local)) return;
@@ -366,7 +456,7 @@ k_plain_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> F, Spin3<R> E, Spin3<R> D, S
 
 // Heun corrector (LLGHeun.cpp:239-279): S <- |S0 + dt/2 (dS + dS')| in place.
 template <typename R>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(512)
 k_plain_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> F, Spin3<R> E, Spin3<R> D) {
     int i, rep, local;
     if (!my_atom(s, i, rep, local)) return;
@@ -426,20 +516,12 @@ k_mag_partial(int n, Spin3<R> S, const int* __restrict__ mask, const double* __r
 // (src/anisotropy/energy.cpp:31-50 with uniaxial_second_order.cpp:104-121,
 // cubic_fourth_order.cpp:80-95, cubic_fourth_order_rotation.cpp:104-140) and applied field
 // (src/simulate/energy.cpp:94-98).
-template <typename R, int ET, int JM>
+template <typename R, int ET, int AM>
 __device__ __forceinline__ R exchange_energy(const SysDev<R>& s, const R* __restrict__ X,
                                              const R* __restrict__ Y, const R* __restrict__ Z,
                                              int i, R sx, R sy, R sz) {
-    constexpr int K = (ET == ET_ISO) ? 1 : (ET == ET_VEC ? 3 : 9);
     R energy = R(0);
-    const int beg = s.row[i], end = s.row[i + 1];
-    for (int nn = beg; nn < end; ++nn) {
-        const int j = s.col[nn];
-        const R* J;
-        if (JM == JM_UNIFORM) J = s.jtab;
-        else if (JM == JM_IDX16) J = s.jtab + (size_t)K * ((const uint16_t*)s.jidx)[nn];
-        else J = s.jtab + (size_t)K * ((const int*)s.jidx)[nn];
-        const R Sx = X[j], Sy = Y[j], Sz = Z[j];
+    for_neighbours<R, ET, AM>(s, X, Y, Z, i, [&](const R Sx, const R Sy, const R Sz, const R* J) {
         if (ET == ET_ISO) {
             energy -= J[0] * (Sx * sx + Sy * sy + Sz * sz);
         } else if (ET == ET_VEC) {
@@ -449,7 +531,7 @@ __device__ __forceinline__ R exchange_energy(const SysDev<R>& s, const R* __rest
                        J[3] * Sx * sy + J[4] * Sy * sy + J[5] * Sz * sy +
                        J[6] * Sx * sz + J[7] * Sy * sz + J[8] * Sz * sz);
         }
-    }
+    });
     return energy;
 }
 
@@ -482,7 +564,7 @@ __device__ __forceinline__ R applied_energy(const Cond<R>& c, R sx, R sy, R sz) 
 }
 
 // W = 3: exchange, anisotropy, applied, each times m_i (energy.cpp:155-196)
-template <typename R, int ET, int JM>
+template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(256)
 k_energy_partial(SysDev<R> s, Spin3<R> S, const int* __restrict__ mask, const double* __restrict__ mm,
                  const int* __restrict__ atom_rep, int nbins, double* __restrict__ partial) {
@@ -504,7 +586,7 @@ k_energy_partial(SysDev<R> s, Spin3<R> S, const int* __restrict__ mask, const do
         const int m = s.type[i];
         const int rep = atom_rep ? atom_rep[i] : 0;
         const double w = mm[i];
-        a0 += (double)exchange_energy<R, ET, JM>(s, S.x, S.y, S.z, i, sx, sy, sz) * w;
+        a0 += (double)exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, sx, sy, sz) * w;
         a1 += (double)anisotropy_energy(s, m, sx, sy, sz) * w;
         a2 += (double)applied_energy(s.cond[rep * s.nmat + m], sx, sy, sz) * w;
     }
@@ -532,7 +614,7 @@ __device__ __forceinline__ int replica_of(const SysDev<R>& s, int nrep, int i) {
     return lo;
 }
 
-template <typename R, int ET, int JM>
+template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(256)
 k_mc_colour(SysDev<R> s, Spin3<R> S, const int* __restrict__ list, int count, int nrep,
             const double* __restrict__ sigma, uint64_t sweep, double* __restrict__ rejects) {
@@ -556,9 +638,9 @@ k_mc_colour(SysDev<R> s, Spin3<R> S, const int* __restrict__ list, int count, in
         R nx = ox + g0 * angle, ny = oy + g1 * angle, nz = oz + g2 * angle;
         const R r = inv_norm(nx, ny, nz);
         nx *= r; ny *= r; nz *= r;
-        const R e_old = (R(0) + exchange_energy<R, ET, JM>(s, S.x, S.y, S.z, i, ox, oy, oz)
+        const R e_old = (R(0) + exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, ox, oy, oz)
                          + anisotropy_energy(s, m, ox, oy, oz)) + applied_energy(c, ox, oy, oz);
-        const R e_new = (R(0) + exchange_energy<R, ET, JM>(s, S.x, S.y, S.z, i, nx, ny, nz)
+        const R e_new = (R(0) + exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, nx, ny, nz)
                          + anisotropy_energy(s, m, nx, ny, nz)) + applied_energy(c, nx, ny, nz);
         const R de = (e_new - e_old) * s.mat[m].mu_s_SI * R(1.07828231e23);
         bool accept = de < R(0);

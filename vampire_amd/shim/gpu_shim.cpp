@@ -1,0 +1,272 @@
+// gpu_shim.cpp — VAMPIRE's accelerator boundary (namespace gpu, hdr/gpu.hpp:33-56) implemented
+// on the C ABI of libvmx.so (include/vmx.h).
+//
+// In a VAMPIRE tree this file replaces src/gpu/initialize.cpp, llg_heun.cpp, finalize.cpp,
+// statistics.cpp, transfer.cpp and config.cpp (the #ifdef CUDA / OPENCL dispatchers); data.cpp
+// (the flags) and interface.cpp (the gpu: input keys) stay.  Nothing else in VAMPIRE changes:
+// sim::integrate_serial already calls gpu::llg_heun() when gpu::acceleration is set
+// (src/simulate/sim.cpp:600) and stats::update/reset call gpu::stats::update/reset
+// (src/statistics/statistics.cpp:35-37,71-72).  Build: compile with the VAMPIRE headers
+// (-I hdr -I src) and link with -lvmx.
+//
+// Behaviour kept from the reference backend:
+//  * gpu::initialize copies the system from the atoms::, mp::, exchange:: and anisotropy::
+//    globals once (src/cuda/initialize.cu:43-151); the device spins are authoritative afterwards.
+//  * gpu::llg_heun reads sim::temperature, sim::H_vec * sim::H_applied and the Hamiltonian flags
+//    at every call (src/cuda/external_fields.cu:52-56) and takes one Heun step.
+//  * gpu::stats::get replaces the host statistics with the device ones: reset the averages, then
+//    set_magnetization(current, mean sums, counter) (src/cuda/statistics.cu:252-284).
+//  * errors end the run through err::vexit(), as a failed CUDA call does (cuda_utils.hpp:32-43).
+#include <cmath>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "atoms.hpp"
+#include "errors.hpp"
+#include "exchange.hpp"
+#include "gpu.hpp"
+#include "material.hpp"
+#include "random.hpp"
+#include "sim.hpp"
+#include "stats.hpp"
+#include "vio.hpp"
+#include "anisotropy/internal.hpp"
+
+#include "vmx.h"
+
+namespace {
+
+vmx_ctx* g_ctx = nullptr;
+
+struct MagHandle {
+    stats::magnetization_statistic_t* stat;
+    int mask_size;
+    int handle;
+};
+std::vector<MagHandle> g_mags;
+
+void check(int rc, const char* what) {
+    if (rc != VMX_OK) {
+        std::cerr << "Error: GPU backend call " << what << " failed: " << vmx_last_error() << std::endl;
+        zlog << zTs() << "Error: GPU backend call " << what << " failed: " << vmx_last_error() << std::endl;
+        err::vexit();
+    }
+}
+
+uint32_t field_flags() {
+    uint32_t f = 0;
+    if (sim::hamiltonian_simulation_flags[3] == 1) f |= VMX_FIELD_THERMAL;
+    if (sim::hamiltonian_simulation_flags[2] == 1) f |= VMX_FIELD_APPLIED;
+    if (sim::local_applied_field) f |= VMX_FIELD_LOCAL_APPLIED;
+    return f;
+}
+
+void push_conditions() {
+    const double H[3] = {sim::H_vec[0] * sim::H_applied, sim::H_vec[1] * sim::H_applied,
+                         sim::H_vec[2] * sim::H_applied};
+    check(vmx_set_conditions(g_ctx, -1, sim::temperature, H, field_flags()), "vmx_set_conditions");
+}
+
+void add_magnetization(stats::magnetization_statistic_t& st, bool enabled) {
+    if (!enabled || !st.is_initialized()) return;
+    std::vector<int> mask;
+    std::vector<double> saturation;
+    st.get_mask(mask, saturation);
+    const int mask_size = (int)saturation.size();
+    int h = -1;
+    check(vmx_stats_add_magnetization(g_ctx, mask_size, mask.data(), &h), "vmx_stats_add_magnetization");
+    g_mags.push_back(MagHandle{&st, mask_size, h});
+}
+
+}  // namespace
+
+namespace gpu {
+
+void initialize() {
+    const int n = atoms::num_atoms;
+    const int nmat = mp::num_materials;
+    namespace ai = anisotropy::internal;
+
+    // exchange table in the representation the neighbour list refers to
+    const int et = (int)exchange::get_exchange_type();
+    std::vector<double> table;
+    if (et == 0) {
+        for (const zval_t& z : atoms::i_exchange_list) table.push_back(z.Jij);
+    } else if (et == 1) {
+        for (const zvec_t& z : atoms::v_exchange_list)
+            for (int k = 0; k < 3; ++k) table.push_back(z.Jij[k]);
+    } else {
+        for (const zten_t& z : atoms::t_exchange_list)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) table.push_back(z.Jij[a][b]);
+    }
+    const int K = et == 0 ? 1 : (et == 1 ? 3 : 9);
+
+    std::vector<vmx_material> mats(nmat);
+    for (int m = 0; m < nmat; ++m) {
+        const mp::materials_t& a = mp::material[m];
+        vmx_material& v = mats[m];
+        v.alpha = a.alpha;
+        v.gamma_rel = a.gamma_rel;
+        v.mu_s_SI = a.mu_s_SI;
+        v.one_oneplusalpha_sq = a.one_oneplusalpha_sq;
+        v.alpha_oneplusalpha_sq = a.alpha_oneplusalpha_sq;
+        v.H_th_sigma = a.H_th_sigma;
+        v.temperature_rescaling_alpha = a.temperature_rescaling_alpha;
+        v.temperature_rescaling_Tc = a.temperature_rescaling_Tc;
+        v.ku2 = ai::enable_uniaxial_second_order ? ai::ku2[m] : 0.0;
+        v.ku_vector[0] = ai::ku_vector[m].x;
+        v.ku_vector[1] = ai::ku_vector[m].y;
+        v.ku_vector[2] = ai::ku_vector[m].z;
+        v.kc4 = (ai::enable_cubic_fourth_order || ai::enable_cubic_fourth_order_rotation) ? ai::kc4[m] : 0.0;
+        for (int k = 0; k < 3; ++k) {
+            v.kc_vector1[k] = ai::mp[m].kc_vector1[k];
+            v.kc_vector2[k] = ai::mp[m].kc_vector2[k];
+            v.kc_vector3[k] = ai::mp[m].kc_vector3[k];
+            v.applied_field_unit_vector[k] = a.applied_field_unit_vector[k];
+        }
+        v.applied_field_strength = a.applied_field_strength;
+    }
+    uint32_t ani = 0;
+    if (ai::enable_uniaxial_second_order) ani |= VMX_ANISOTROPY_UNIAXIAL_SECOND_ORDER;
+    if (ai::enable_cubic_fourth_order) ani |= VMX_ANISOTROPY_CUBIC_FOURTH_ORDER;
+    if (ai::enable_cubic_fourth_order_rotation) ani |= VMX_ANISOTROPY_CUBIC_FOURTH_ROTATION;
+    if (ai::enable_uniaxial_fourth_order || ai::enable_uniaxial_sixth_order || ai::enable_cubic_sixth_order ||
+        ai::enable_neel_anisotropy || ai::enable_lattice_anisotropy || exchange::biquadratic) {
+        std::cerr << "Error: the GPU backend supports second-order uniaxial and fourth-order cubic anisotropy "
+                     "and bilinear exchange only" << std::endl;
+        err::vexit();
+    }
+
+    vmx_system s{};
+    s.num_atoms = n;
+    s.num_materials = nmat;
+    s.type = atoms::type_array.data();
+    s.moment = atoms::m_spin_array.data();
+    s.nbr_start = atoms::neighbour_list_start_index.data();
+    s.nbr_end = atoms::neighbour_list_end_index.data();
+    s.num_nbr = (int64_t)atoms::neighbour_list_array.size();
+    s.nbr = atoms::neighbour_list_array.data();
+    s.nbr_iid = atoms::neighbour_interaction_type_array.data();
+    s.exchange_type = et;
+    s.num_interactions = (int64_t)(table.size() / K);
+    s.exchange = table.data();
+    s.materials = mats.data();
+    s.anisotropy_flags = ani;
+    s.num_replicas = 1;
+    s.replica_offset = nullptr;
+
+    vmx_options o{};
+    o.precision = VMX_FP64;
+    o.kernel = VMX_KERNEL_FUSED;
+    // gpu:num-threads becomes the workgroup size when it is one the kernels accept
+    o.block_size = (gpu::num_threads >= 64 && gpu::num_threads <= 512 && gpu::num_threads % 64 == 0)
+                       ? gpu::num_threads : 0;
+    o.device = gpu::device;
+    o.seed = (uint64_t)(uint32_t)mtrandom::integration_seed;
+    o.topology = VMX_TOPOLOGY_AUTO;
+
+    check(vmx_create(&s, &o, &g_ctx), "vmx_create");
+    check(vmx_set_time_step(g_ctx, mp::dt, mp::half_dt), "vmx_set_time_step");
+    check(vmx_set_spins(g_ctx, atoms::x_spin_array.data(), atoms::y_spin_array.data(), atoms::z_spin_array.data()),
+          "vmx_set_spins");
+    push_conditions();
+
+    // statistics computed on the device unless the input asks for CPU statistics
+    g_mags.clear();
+    if (!gpu::cpu_stats) {
+        add_magnetization(::stats::system_magnetization, ::stats::calculate_system_magnetization);
+        add_magnetization(::stats::material_magnetization, ::stats::calculate_material_magnetization);
+        add_magnetization(::stats::height_magnetization, ::stats::calculate_height_magnetization);
+        add_magnetization(::stats::material_height_magnetization,
+                          ::stats::calculate_material_height_magnetization);
+    }
+    gpu::initialized = true;
+    zlog << zTs() << "GPU backend initialised: " << vmx_version() << ", " << n << " atoms" << std::endl;
+}
+
+void llg_heun() {
+    push_conditions();
+    check(vmx_llg_heun(g_ctx, 1, nullptr), "vmx_llg_heun");
+}
+
+void finalize() {
+    if (g_ctx) {
+        transfer_spin_positions_from_gpu_to_cpu();
+        check(vmx_destroy(g_ctx), "vmx_destroy");
+        g_ctx = nullptr;
+    }
+}
+
+// Without an initialised backend these calls do nothing, like the reference dispatchers in a
+// build without CUDA or OpenCL (config output calls synchronise unconditionally).
+void transfer_spin_positions_from_gpu_to_cpu() {
+    if (!g_ctx) return;
+    check(vmx_get_spins(g_ctx, atoms::x_spin_array.data(), atoms::y_spin_array.data(), atoms::z_spin_array.data()),
+          "vmx_get_spins");
+}
+
+void transfer_dipole_fields_from_cpu_to_gpu() {
+    // the dipole field is not part of this backend; gpu::initialize refuses systems that need it
+}
+
+namespace config {
+void synchronise() { transfer_spin_positions_from_gpu_to_cpu(); }
+}  // namespace config
+
+namespace stats {
+
+// Energy statistics are computed on the host from synchronised spins: energy_statistic_t takes
+// device values only without their update counter (hdr/stats.hpp:108-112), so its means could not
+// be formed from device sums.
+void host_energies() {
+    if (!::stats::calculate_system_energy && !::stats::calculate_material_energy) return;
+    if (!gpu::cpu_stats) transfer_spin_positions_from_gpu_to_cpu();
+    if (::stats::calculate_system_energy)
+        ::stats::system_energy.calculate(atoms::x_spin_array, atoms::y_spin_array, atoms::z_spin_array,
+                                         atoms::m_spin_array, atoms::type_array, sim::temperature);
+    if (::stats::calculate_material_energy)
+        ::stats::material_energy.calculate(atoms::x_spin_array, atoms::y_spin_array, atoms::z_spin_array,
+                                           atoms::m_spin_array, atoms::type_array, sim::temperature);
+}
+
+void update() {
+    if (!g_ctx) return;
+    if (gpu::cpu_stats) {
+        // statistics on the host from freshly copied spins (src/cuda/statistics.cu:34-48)
+        transfer_spin_positions_from_gpu_to_cpu();
+        if (::stats::calculate_system_magnetization)
+            ::stats::system_magnetization.calculate_magnetization(atoms::x_spin_array, atoms::y_spin_array,
+                                                                  atoms::z_spin_array, atoms::m_spin_array);
+        if (::stats::calculate_material_magnetization)
+            ::stats::material_magnetization.calculate_magnetization(atoms::x_spin_array, atoms::y_spin_array,
+                                                                    atoms::z_spin_array, atoms::m_spin_array);
+        host_energies();
+        return;
+    }
+    push_conditions();
+    check(vmx_stats_update(g_ctx), "vmx_stats_update");
+    host_energies();
+}
+
+void get() {
+    if (!g_ctx || gpu::cpu_stats) return;
+    for (const MagHandle& h : g_mags) {
+        std::vector<double> cur(4 * h.mask_size), mean(4 * h.mask_size);
+        int64_t counter = 0;
+        check(vmx_stats_get_magnetization(g_ctx, h.handle, cur.data(), mean.data(), &counter),
+              "vmx_stats_get_magnetization");
+        h.stat->reset_magnetization_averages();
+        h.stat->set_magnetization(cur, mean, (long)counter);
+    }
+}
+
+void reset() {
+    if (!g_ctx || gpu::cpu_stats) return;
+    check(vmx_stats_reset(g_ctx), "vmx_stats_reset");
+}
+
+}  // namespace stats
+
+}  // namespace gpu
