@@ -121,6 +121,10 @@ typedef struct vmx_options {
     int32_t device;       /* HIP device ordinal; -1 = current device                            */
     uint64_t seed;        /* Philox key for thermal noise and Monte Carlo moves                 */
     int32_t topology;     /* VMX_TOPOLOGY_*: device layout of the neighbour list                */
+    int32_t recompute_external_field; /* fused kernels: 0 = the predictor stores the external
+                             (thermal + applied) field for the corrector; 1 = the corrector
+                             regenerates it from the same Philox counter (less memory traffic,
+                             more arithmetic).  Results are identical.                         */
 } vmx_options;
 
 /* Device layout of the neighbour list.  Every layout keeps each row's neighbour order, so all
@@ -250,6 +254,29 @@ int vmx_last_step_time_ms(vmx_ctx* ctx, double* ms);
  * corrector; plain: spin fields of S, external fields, predictor, spin fields of S', corrector)
  * and their number in `nkernels`.  `avg_ms` must hold 5 values. */
 int vmx_time_kernels(vmx_ctx* ctx, int32_t nsteps, double* avg_ms, int32_t* nkernels);
+
+/* ---- differentiation: deterministic trajectory and its adjoint ---------------------------- */
+/* The T = 0 Heun step of sim::LLG_Heun (src/simulate/LLGHeun.cpp:160-282) with explicit
+ * parameters, for a context of one replica, one material, isotropic uniform exchange on a
+ * symmetric neighbour graph.  The field is H_i = J sum_j S_j + 2 ku2 (S_i.e) e + H + h_i
+ * (e = the material's uniaxial axis, h_i an optional per-site field). */
+typedef struct vmx_fit_params {
+    double J;          /* exchange J_ij/mu_s of every bond (T)                              */
+    double ku2;        /* uniaxial Ku/mu_s (T)                                              */
+    double alpha;      /* Gilbert damping; gamma_rel is the material's                       */
+    double H[3];       /* uniform applied field (T)                                         */
+} vmx_fit_params;
+
+/* Runs nsteps steps from spins0 [3][N] and writes <m_z> after every record_every steps to
+ * mz [nsteps/record_every].  Keeps a checkpoint every checkpoint_every steps for the adjoint.
+ * h: per-site field [3][N] or NULL.  nsteps must be a multiple of record_every. */
+int vmx_fit_forward(vmx_ctx* ctx, const vmx_fit_params* p, const double* h, const double* spins0,
+                    int32_t nsteps, int32_t record_every, int32_t checkpoint_every, double* mz);
+
+/* Discrete adjoint of the last vmx_fit_forward: given dL/d<m_z> [nsteps/record_every], writes
+ * grad = {dL/dJ, dL/dku2, dL/dalpha} and, if dL_dh is not NULL, dL/dh_i [3][N].  Segments between
+ * checkpoints are recomputed forward, then swept backward (hand-written adjoint kernels). */
+int vmx_fit_backward(vmx_ctx* ctx, const double* dL_dmz, double* grad, double* dL_dh);
 
 /* Chooses the workgroup size (and for FP32 the dS-storage variant) by timing a few steps of
  * each candidate on this system; the result applies to later calls.  The spins and step

@@ -89,6 +89,16 @@ class vmx_options(C.Structure):
         ("device", C.c_int32),
         ("seed", C.c_uint64),
         ("topology", C.c_int32),
+        ("recompute_external_field", C.c_int32),
+    ]
+
+
+class vmx_fit_params(C.Structure):
+    _fields_ = [
+        ("J", C.c_double),
+        ("ku2", C.c_double),
+        ("alpha", C.c_double),
+        ("H", C.c_double * 3),
     ]
 
 
@@ -126,6 +136,8 @@ SIGNATURES = [
     ("vmx_synchronize", C.c_int, [_VP]),
     ("vmx_last_step_time_ms", C.c_int, [_VP, _DP]),
     ("vmx_time_kernels", C.c_int, [_VP, C.c_int32, _DP, _P(C.c_int32)]),
+    ("vmx_fit_forward", C.c_int, [_VP, _P(vmx_fit_params), _DP, _DP, C.c_int32, C.c_int32, C.c_int32, _DP]),
+    ("vmx_fit_backward", C.c_int, [_VP, _DP, _DP, _DP]),
     ("vmx_autotune", C.c_int, [_VP, C.c_int32, _P(C.c_int32)]),
 ]
 

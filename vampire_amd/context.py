@@ -44,7 +44,7 @@ class Context:
     """A device context for one system (or a batch of replicas)."""
 
     def __init__(self, system: System, precision: str = FP64, kernel: str = FUSED, block_size: int = 0,
-                 seed: int = 0, device: int = -1, topology: str = "auto"):
+                 seed: int = 0, device: int = -1, topology: str = "auto", recompute_external_field: bool = False):
         self.lib = L.load()
         self.system = system
         self.n = system.n
@@ -67,7 +67,8 @@ class Context:
         opt = L.vmx_options(precision=L.VMX_FP64 if precision == FP64 else L.VMX_FP32,
                             kernel=L.VMX_KERNEL_FUSED if kernel == FUSED else L.VMX_KERNEL_PLAIN,
                             block_size=block_size, device=device, seed=seed,
-                            topology=TOPOLOGIES[topology])
+                            topology=TOPOLOGIES[topology],
+                            recompute_external_field=int(bool(recompute_external_field)))
         h = C.c_void_p()
         L.check(self.lib.vmx_create(C.byref(sysc), C.byref(opt), C.byref(h)))
         self.h = h

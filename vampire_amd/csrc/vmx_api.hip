@@ -113,6 +113,19 @@ struct EnergyStat {
 
 __global__ void k_advance_step(unsigned long long* base, unsigned int n) { *base += n; }
 
+// adds dL/dS_z (the same for every atom) to the adjoint of a recorded state
+template <typename R>
+__global__ void k_lam_inject(int n, R* __restrict__ lz, R inj) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) lz[i] += inj;
+}
+
+template <typename R>
+__global__ void k_fill(size_t n, R* __restrict__ a, R v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        a[i] = v;
+}
+
 // Reduces the workgroup partials in block order, then normalises (magnetization.cpp:150-178).
 __global__ void k_mag_finish(int nblk, const double* __restrict__ partial, int nbins, double* cur,
                              double* mean, const int* __restrict__ empty, int nempty,
@@ -229,6 +242,8 @@ struct vmx_ctx {
     DevBuf spin[3];               // three buffers of 3N R each (S, P, O roles rotate)
     int s_idx = 0, o_idx = 2;     // current spin buffer and the corrector's output buffer
     DevBuf plainF, plainE, plainD;
+    int store_hext = 1;           // fused step: predictor stores H_ext for the corrector
+    DevBuf hext_buf;              // 3N R
     DevBuf mm, atom_rep;
     DevBuf staging;               // 3N doubles for host<->device spin copies and noise
     DevBuf noise_dev;             // 3N doubles
@@ -273,6 +288,12 @@ struct vmx_ctx {
     Spin3<R> spins(int idx) const {
         R* base = spin[idx].as<R>();
         return Spin3<R>{base, base + n, base + 2 * (size_t)n};
+    }
+    // external-field arrays of the fused step, or null arrays when the corrector regenerates it
+    template <typename R>
+    Spin3<R> hext() const {
+        if (!store_hext) return Spin3<R>{nullptr, nullptr, nullptr};
+        return arr3<R>(hext_buf);
     }
     template <typename R>
     Spin3<R> arr3(const DevBuf& b) const {
@@ -395,6 +416,22 @@ struct vmx_ctx {
     template <typename R>
     int mc_sweeps(int nsweeps);
     int colour();
+
+    // deterministic fitting trajectory and its adjoint
+    struct Fit {
+        bool ready = false;
+        int nsteps = 0, record_every = 1, K = 100, nrec = 0;
+        double J = 0, k = 0, alpha = 0, gamma_rel = 1, H[3] = {0, 0, 0};
+        bool has_h = false;
+        DevBuf ckpt, seg, P, lam, lH1, lPd, lT, H0, sumS, lH0, lSp, dh, nhat, h, acc, mz;
+    } fit;
+    bool symmetric_checked = false, symmetric = false;
+    template <typename R>
+    FitPrm<R> fit_prm() const;
+    template <typename R>
+    int fit_forward(const double* h, const double* spins0, double* mz_out);
+    template <typename R>
+    int fit_backward(const double* dmz, double* grad, double* dh);
 };
 
 // --------------------------------------------------------------------------------------------
@@ -428,10 +465,11 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
         int nk = 0;
         if (kernel == VMX_KERNEL_FUSED) {
             VMX_HIP(mark(0));
-            VMX_HIP(Ops<R>::fused_predictor(L, et, am(),s, spins<R>(s_idx), spins<R>(p_idx), nd, (uint32_t)k));
-            VMX_HIP(mark(1));
-            VMX_HIP(Ops<R>::fused_corrector(L, et, am(),s, spins<R>(s_idx), spins<R>(p_idx), spins<R>(o_idx), nd,
+            VMX_HIP(Ops<R>::fused_predictor(L, et, am(), s, spins<R>(s_idx), spins<R>(p_idx), hext<R>(), nd,
                                             (uint32_t)k));
+            VMX_HIP(mark(1));
+            VMX_HIP(Ops<R>::fused_corrector(L, et, am(), s, spins<R>(s_idx), spins<R>(p_idx), spins<R>(o_idx),
+                                            hext<R>(), nd, (uint32_t)k));
             VMX_HIP(mark(2));
             nk = 2;
             std::swap(s_idx, o_idx);
@@ -480,9 +518,11 @@ int vmx_ctx::capture_graph(int nsteps) {
     int si = s_idx, oi = o_idx;
     for (int k = 0; k < nsteps; ++k) {
         const int pi = 3 - si - oi;
-        hipError_t e = Ops<R>::fused_predictor(L, et, am(),s, spins<R>(si), spins<R>(pi), nullptr, (uint32_t)k);
+        hipError_t e = Ops<R>::fused_predictor(L, et, am(), s, spins<R>(si), spins<R>(pi), hext<R>(), nullptr,
+                                               (uint32_t)k);
         if (e == hipSuccess)
-            e = Ops<R>::fused_corrector(L, et, am(),s, spins<R>(si), spins<R>(pi), spins<R>(oi), nullptr, (uint32_t)k);
+            e = Ops<R>::fused_corrector(L, et, am(), s, spins<R>(si), spins<R>(pi), spins<R>(oi), hext<R>(), nullptr,
+                                        (uint32_t)k);
         if (e != hipSuccess) {
             hipGraph_t g;
             (void)hipStreamEndCapture(stream, &g);
@@ -602,6 +642,137 @@ int vmx_ctx::mc_sweeps(int nsweeps) {
 }
 
 // ============================================================================================
+// ---- fitting trajectory and adjoint -------------------------------------------------------------
+template <typename R>
+FitPrm<R> vmx_ctx::fit_prm() const {
+    FitPrm<R> p;
+    const double a = -fit.gamma_rel / (1.0 + fit.alpha * fit.alpha);
+    p.J = (R)fit.J;
+    p.k = (R)fit.k;
+    p.a = (R)a;
+    p.b = (R)(fit.alpha * a);
+    p.dt = (R)dt;
+    p.half_dt = (R)half_dt;
+    for (int q = 0; q < 3; ++q) {
+        p.e[q] = (R)mats[0].ku_vector[q];
+        p.H[q] = (R)fit.H[q];
+    }
+    return p;
+}
+
+template <typename R>
+int vmx_ctx::fit_forward(const double* h, const double* spins0, double* mz_out) {
+    const size_t N3 = 3 * (size_t)n;
+    const int nck = fit.nsteps / fit.K + 1;
+    VMX_TRY(fit.ckpt.alloc((size_t)nck * N3 * sizeof(R)));
+    VMX_TRY(fit.seg.alloc((size_t)(fit.K + 1) * N3 * sizeof(R)));
+    VMX_TRY(fit.P.alloc(N3 * sizeof(R)));
+    VMX_TRY(fit.h.alloc(N3 * sizeof(R)));
+    VMX_TRY(fit.mz.alloc((size_t)4 * std::max(fit.nrec, 1) * sizeof(double)));
+    double* st = staging.as<double>();
+    VMX_HIP(hipStreamSynchronize(stream));
+    fit.has_h = h != nullptr;
+    if (h) {
+        VMX_HIP(hipMemcpy(st, h, N3 * sizeof(double), hipMemcpyHostToDevice));
+        VMX_HIP(Ops<R>::from_double(stream, (int)N3, st, fit.h.as<R>()));
+        VMX_HIP(hipStreamSynchronize(stream));
+    }
+    VMX_HIP(hipMemcpy(st, spins0, N3 * sizeof(double), hipMemcpyHostToDevice));
+    R* ck = fit.ckpt.as<R>();
+    VMX_HIP(Ops<R>::from_double(stream, (int)N3, st, ck));
+    VMX_HIP(hipMemsetAsync(fit.mz.p, 0, (size_t)4 * std::max(fit.nrec, 1) * sizeof(double), stream));
+    const SysDev<R> s = sys<R>();
+    const FitPrm<R> p = fit_prm<R>();
+    const Spin3<R> hh = fit.has_h ? arr3<R>(fit.h) : Spin3<R>{nullptr, nullptr, nullptr};
+    auto state = [&](R* base) { return Spin3<R>{base, base + n, base + 2 * (size_t)n}; };
+    R* seg = fit.seg.as<R>();
+    R* cur = seg;
+    R* nxt = seg + N3;
+    VMX_HIP(hipMemcpyAsync(cur, ck, N3 * sizeof(R), hipMemcpyDeviceToDevice, stream));
+    for (int q = 0; q < fit.nsteps; ++q) {
+        VMX_HIP(Ops<R>::fit_predictor(stream, am(), s, p, hh, state(cur), arr3<R>(fit.P)));
+        VMX_HIP(Ops<R>::fit_corrector(stream, am(), s, p, hh, state(cur), arr3<R>(fit.P), state(nxt)));
+        std::swap(cur, nxt);
+        if ((q + 1) % fit.K == 0)
+            VMX_HIP(hipMemcpyAsync(ck + (size_t)((q + 1) / fit.K) * N3, cur, N3 * sizeof(R), hipMemcpyDeviceToDevice,
+                                   stream));
+        if ((q + 1) % fit.record_every == 0)
+            VMX_HIP(Ops<R>::fit_mz(stream, n, state(cur), fit.mz.as<double>() + 4 * (size_t)((q + 1) / fit.record_every - 1)));
+    }
+    std::vector<double> mzh((size_t)4 * std::max(fit.nrec, 1));
+    VMX_HIP(hipStreamSynchronize(stream));
+    VMX_HIP(hipMemcpy(mzh.data(), fit.mz.p, mzh.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int r = 0; r < fit.nrec; ++r) mz_out[r] = mzh[4 * (size_t)r] / n;
+    fit.ready = true;
+    return VMX_OK;
+}
+
+template <typename R>
+int vmx_ctx::fit_backward(const double* dmz, double* grad, double* dh_out) {
+    const size_t N3 = 3 * (size_t)n;
+    for (DevBuf* b : {&fit.lam, &fit.lH1, &fit.lPd, &fit.lT, &fit.H0, &fit.sumS, &fit.lH0, &fit.lSp, &fit.dh})
+        VMX_TRY(b->alloc(N3 * sizeof(R)));
+    VMX_TRY(fit.nhat.alloc((size_t)n * sizeof(R)));
+    VMX_TRY(fit.acc.alloc(4 * sizeof(double)));
+    const SysDev<R> s = sys<R>();
+    const FitPrm<R> p = fit_prm<R>();
+    const Spin3<R> hh = fit.has_h ? arr3<R>(fit.h) : Spin3<R>{nullptr, nullptr, nullptr};
+    auto state = [&](R* base) { return Spin3<R>{base, base + n, base + 2 * (size_t)n}; };
+    const int fill_grid = (int)std::min<size_t>((N3 + 255) / 256, 4096);
+    hipLaunchKernelGGL((k_fill<R>), dim3(fill_grid), dim3(256), 0, stream, N3, fit.lam.as<R>(), (R)0);
+    hipLaunchKernelGGL((k_fill<R>), dim3(fill_grid), dim3(256), 0, stream, N3, fit.dh.as<R>(), (R)0);
+    hipLaunchKernelGGL((k_fill<double>), dim3(1), dim3(64), 0, stream, (size_t)4, fit.acc.as<double>(), 0.0);
+    VMX_HIP(hipGetLastError());
+    R* ck = fit.ckpt.as<R>();
+    R* seg = fit.seg.as<R>();
+    const Spin3<R> lam = arr3<R>(fit.lam);
+    const int nseg = (fit.nsteps + fit.K - 1) / fit.K;
+    for (int c = nseg - 1; c >= 0; --c) {
+        const int start = c * fit.K, end = std::min(start + fit.K, fit.nsteps);
+        // recompute the states of this segment from its checkpoint
+        VMX_HIP(hipMemcpyAsync(seg, ck + (size_t)c * N3, N3 * sizeof(R), hipMemcpyDeviceToDevice, stream));
+        for (int t = 0; t < end - start; ++t) {
+            R* a = seg + (size_t)t * N3;
+            VMX_HIP(Ops<R>::fit_predictor(stream, am(), s, p, hh, state(a), arr3<R>(fit.P)));
+            VMX_HIP(Ops<R>::fit_corrector(stream, am(), s, p, hh, state(a), arr3<R>(fit.P), state(a + N3)));
+        }
+        for (int q = end - 1; q >= start; --q) {
+            if ((q + 1) % fit.record_every == 0) {
+                const double inj = dmz[(q + 1) / fit.record_every - 1] / n;
+                hipLaunchKernelGGL((k_lam_inject<R>), dim3((n + 255) / 256), dim3(256), 0, stream, n, lam.z, (R)inj);
+                VMX_HIP(hipGetLastError());
+            }
+            R* a = seg + (size_t)(q - start) * N3;
+            VMX_HIP(Ops<R>::fit_predictor(stream, am(), s, p, hh, state(a), arr3<R>(fit.P)));
+            VMX_HIP(Ops<R>::adj_b1(stream, am(), s, p, hh, state(a), arr3<R>(fit.P), lam, arr3<R>(fit.lH1),
+                                   arr3<R>(fit.lPd), arr3<R>(fit.lT), arr3<R>(fit.H0), arr3<R>(fit.sumS),
+                                   fit.nhat.as<R>(), fit.acc.as<double>()));
+            VMX_HIP(Ops<R>::adj_b2(stream, am(), s, p, state(a), arr3<R>(fit.P), arr3<R>(fit.lH1), arr3<R>(fit.lPd),
+                                   arr3<R>(fit.lT), arr3<R>(fit.H0), arr3<R>(fit.sumS), fit.nhat.as<R>(),
+                                   arr3<R>(fit.lH0), arr3<R>(fit.lSp), arr3<R>(fit.dh), fit.acc.as<double>()));
+            VMX_HIP(Ops<R>::adj_b3(stream, am(), s, p, arr3<R>(fit.lH0), arr3<R>(fit.lSp), lam, (R)0));
+        }
+    }
+    double acc[4];
+    VMX_HIP(hipStreamSynchronize(stream));
+    VMX_HIP(hipMemcpy(acc, fit.acc.p, sizeof acc, hipMemcpyDeviceToHost));
+    // a = -gamma/(1+alpha^2), b = alpha a
+    const double al = fit.alpha, g = fit.gamma_rel;
+    const double a = -g / (1.0 + al * al);
+    const double da_dal = 2.0 * al * g / ((1.0 + al * al) * (1.0 + al * al));
+    const double db_dal = a + al * da_dal;
+    grad[0] = acc[0];
+    grad[1] = acc[1];
+    grad[2] = acc[2] * da_dal + acc[3] * db_dal;
+    if (dh_out) {
+        double* st = staging.as<double>();
+        VMX_HIP(Ops<R>::to_double(stream, (int)N3, fit.dh.as<R>(), st));
+        VMX_HIP(hipStreamSynchronize(stream));
+        VMX_HIP(hipMemcpy(dh_out, st, N3 * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return VMX_OK;
+}
+
 // Exchange table compression: exact (bitwise) deduplication of the J values edges refer to.
 namespace {
 
@@ -936,6 +1107,8 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
         VMX_HIP(hipMemset(c->spin[b].p, 0, 3 * (size_t)n * rs));
     }
     VMX_TRY(c->staging.alloc(3 * (size_t)n * sizeof(double)));
+    c->store_hext = opt->recompute_external_field ? 0 : 1;
+    if (c->kernel == VMX_KERNEL_FUSED) VMX_TRY(c->hext_buf.alloc(3 * (size_t)n * rs));
     const size_t cond_bytes = (size_t)nrep * c->nmat * sizeof(Cond<double>) + 64;
     VMX_TRY(c->ring.init(std::max<size_t>(cond_bytes, 8 * (size_t)nrep + 64)));
     VMX_TRY(c->upload_conditions_any());
@@ -1304,6 +1477,51 @@ int vmx_time_kernels(vmx_ctx* c, int32_t nsteps, double* avg_ms, int32_t* nkerne
     *nkernels = c->kernel == VMX_KERNEL_FUSED ? 2 : 5;
     return c->precision == VMX_FP64 ? c->heun_steps<double>(nsteps, nullptr, avg_ms)
                                     : c->heun_steps<float>(nsteps, nullptr, avg_ms);
+}
+
+int vmx_fit_forward(vmx_ctx* c, const vmx_fit_params* p, const double* h, const double* spins0, int32_t nsteps,
+                    int32_t record_every, int32_t checkpoint_every, double* mz) {
+    if (!c || !p || !spins0 || !mz) return fail(VMX_ERR_ARG, "null argument");
+    if (nsteps <= 0 || record_every <= 0 || checkpoint_every <= 0 || nsteps % record_every)
+        return fail(VMX_ERR_ARG, "nsteps must be a positive multiple of record_every");
+    if (c->nrep != 1 || c->nmat != 1 || c->et != ET_ISO || c->jm != JM_UNIFORM)
+        return fail(VMX_ERR_STATE, "fitting needs one replica, one material and uniform isotropic exchange");
+    if (!(c->dt > 0.0)) return fail(VMX_ERR_STATE, "time step not set (vmx_set_time_step)");
+    if (!c->symmetric_checked) {
+        // the adjoint gathers lambda_H over N(j) for the transpose: every bond must be listed both ways
+        c->symmetric = true;
+        const std::vector<int>& r = c->rowh_;
+        const std::vector<int>& col = c->colh_;
+        for (int i = 0; i < c->n && c->symmetric; ++i)
+            for (int e = r[i]; e < r[i + 1] && c->symmetric; ++e) {
+                const int j = col[e];
+                bool back = false;
+                for (int f = r[j]; f < r[j + 1]; ++f)
+                    if (col[f] == i) { back = true; break; }
+                c->symmetric = back;
+            }
+        c->symmetric_checked = true;
+    }
+    if (!c->symmetric) return fail(VMX_ERR_STATE, "fitting needs a symmetric neighbour list");
+    VMX_TRY(c->use_device());
+    c->fit.nsteps = nsteps;
+    c->fit.record_every = record_every;
+    c->fit.K = checkpoint_every;
+    c->fit.nrec = nsteps / record_every;
+    c->fit.J = p->J;
+    c->fit.k = p->ku2;
+    c->fit.alpha = p->alpha;
+    c->fit.gamma_rel = c->mats[0].gamma_rel;
+    for (int q = 0; q < 3; ++q) c->fit.H[q] = p->H[q];
+    return c->precision == VMX_FP64 ? c->fit_forward<double>(h, spins0, mz) : c->fit_forward<float>(h, spins0, mz);
+}
+
+int vmx_fit_backward(vmx_ctx* c, const double* dL_dmz, double* grad, double* dL_dh) {
+    if (!c || !dL_dmz || !grad) return fail(VMX_ERR_ARG, "null argument");
+    if (!c->fit.ready) return fail(VMX_ERR_STATE, "vmx_fit_backward needs a vmx_fit_forward first");
+    VMX_TRY(c->use_device());
+    return c->precision == VMX_FP64 ? c->fit_backward<double>(dL_dmz, grad, dL_dh)
+                                    : c->fit_backward<float>(dL_dmz, grad, dL_dh);
 }
 
 int vmx_autotune(vmx_ctx* c, int32_t steps_per_trial, int32_t* chosen) {

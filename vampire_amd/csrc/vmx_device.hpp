@@ -104,6 +104,15 @@ struct Spin3 {
     R* __restrict__ z;
 };
 
+// Parameters of the deterministic fitting step (vmx_fit.inc): exchange J (T) of every bond,
+// uniaxial k (T) along e, LLG prefactors a = -gamma/(1+alpha^2), b = alpha a, uniform field H.
+template <typename R>
+struct FitPrm {
+    R J, k, a, b, dt, half_dt;
+    R e[3];
+    R H[3];
+};
+
 // ---- Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as
 // 1, 2, 3", SC'11): 10 rounds of the 4x32 S-box with Weyl key schedule ---------------------
 struct U4 { uint32_t v[4]; };
@@ -369,7 +378,8 @@ __device__ __forceinline__ float inv_norm(float x, float y, float z) {
 // Predictor: S' = |S + dt f(S, H_spin(S) + H_ext)|, written to P.
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(512)
-k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, const double* __restrict__ noise, uint32_t step_off) {
+k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> E, const double* __restrict__ noise,
+                  uint32_t step_off) {
     int i, rep, local;
     if (!my_atom(s, i, rep, local)) return;
     const uint64_t step = *s.step_base + step_off;
@@ -378,6 +388,9 @@ k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, const double* __restrict_
     R fx, fy, fz, ex, ey, ez;
     spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m, sx, sy, sz, fx, fy, fz);
     ext_field(s, rep, m, i, local, step, noise, ex, ey, ez);
+    if (E.x) {   // keep H_ext for the corrector instead of regenerating the noise there
+        E.x[i] = ex; E.y[i] = ey; E.z[i] = ez;
+    }
     const MatConst<R>& mc = s.mat[m];
     R d0, d1, d2;
     llg_rhs(mc.one_opa2, mc.alpha_opa2, sx, sy, sz, fx + ex, fy + ey, fz + ez, d0, d1, d2);
@@ -386,18 +399,23 @@ k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, const double* __restrict_
     P.x[i] = n0 * r; P.y[i] = n1 * r; P.z[i] = n2 * r;
 }
 
-// Corrector: recomputes dS at S (same noise), dS' at S', S_next = |S + dt/2 (dS + dS')|.
+// Corrector: recomputes dS at S with the same H_ext (read back when the predictor stored it,
+// else regenerated from the same Philox counter), dS' at S', S_next = |S + dt/2 (dS + dS')|.
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(512)
-k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, const double* __restrict__ noise,
-                  uint32_t step_off) {
+k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> E,
+                  const double* __restrict__ noise, uint32_t step_off) {
     int i, rep, local;
     if (!my_atom(s, i, rep, local)) return;
     const uint64_t step = *s.step_base + step_off;
     const int m = s.type[i];
     const MatConst<R>& mc = s.mat[m];
     R ex, ey, ez;
-    ext_field(s, rep, m, i, local, step, noise, ex, ey, ez);
+    if (E.x) {
+        ex = E.x[i]; ey = E.y[i]; ez = E.z[i];
+    } else {
+        ext_field(s, rep, m, i, local, step, noise, ex, ey, ez);
+    }
     const R sx = S.x[i], sy = S.y[i], sz = S.z[i];
     R fx, fy, fz;
     spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m, sx, sy, sz, fx, fy, fz);

@@ -20,10 +20,13 @@ template <typename R>
 struct Ops {
     // fused Heun step: predictor writes P from S; corrector writes O from S and P.  The Philox
     // step of a launch is *s.step_base + step_off.
+    // E: external-field arrays written by the predictor and read by the corrector, or all-null
+    // to regenerate the field in the corrector.
     static hipError_t fused_predictor(const Launch& L, int et, int jm, const SysDev<R>& s, Spin3<R> S,
-                                      Spin3<R> P, const double* noise, uint32_t step_off);
+                                      Spin3<R> P, Spin3<R> E, const double* noise, uint32_t step_off);
     static hipError_t fused_corrector(const Launch& L, int et, int jm, const SysDev<R>& s, Spin3<R> S,
-                                      Spin3<R> P, Spin3<R> O, const double* noise, uint32_t step_off);
+                                      Spin3<R> P, Spin3<R> O, Spin3<R> E, const double* noise,
+                                      uint32_t step_off);
     // plain Heun step
     static hipError_t spin_fields(const Launch& L, int et, int jm, const SysDev<R>& s, Spin3<R> S, Spin3<R> F);
     static hipError_t ext_fields(const Launch& L, const SysDev<R>& s, Spin3<R> E, const double* noise,
@@ -41,6 +44,20 @@ struct Ops {
     // Monte Carlo: one colour
     static hipError_t mc_colour(hipStream_t st, int et, int jm, const SysDev<R>& s, Spin3<R> S, const int* list,
                                 int count, int nrep, const double* sigma, uint64_t sweep, double* rejects);
+    // deterministic fitting step and its adjoint (isotropic uniform exchange); grids of 256
+    static hipError_t fit_predictor(hipStream_t st, int am, const SysDev<R>& s, const FitPrm<R>& p, Spin3<R> h,
+                                    Spin3<R> S, Spin3<R> P);
+    static hipError_t fit_corrector(hipStream_t st, int am, const SysDev<R>& s, const FitPrm<R>& p, Spin3<R> h,
+                                    Spin3<R> S, Spin3<R> P, Spin3<R> O);
+    static hipError_t adj_b1(hipStream_t st, int am, const SysDev<R>& s, const FitPrm<R>& p, Spin3<R> h,
+                             Spin3<R> S, Spin3<R> P, Spin3<R> lam, Spin3<R> lH1, Spin3<R> lPd, Spin3<R> lT,
+                             Spin3<R> H0, Spin3<R> sumS, R* nhat, double* acc);
+    static hipError_t adj_b2(hipStream_t st, int am, const SysDev<R>& s, const FitPrm<R>& p, Spin3<R> S,
+                             Spin3<R> P, Spin3<R> lH1, Spin3<R> lPd, Spin3<R> lT, Spin3<R> H0, Spin3<R> sumS,
+                             const R* nhat, Spin3<R> lH0, Spin3<R> lSp, Spin3<R> dh, double* acc);
+    static hipError_t adj_b3(hipStream_t st, int am, const SysDev<R>& s, const FitPrm<R>& p, Spin3<R> lH0,
+                             Spin3<R> lSp, Spin3<R> lam, R inj);
+    static hipError_t fit_mz(hipStream_t st, int n, Spin3<R> S, double* out4);
     // Philox normals of `step` as double [3][n]
     static hipError_t thermal_noise(const Launch& L, const SysDev<R>& s, uint64_t step, double* out);
     // conversions between the double host/staging layout and R device arrays
