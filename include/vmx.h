@@ -125,7 +125,15 @@ typedef struct vmx_options {
                              (thermal + applied) field for the corrector; 1 = the corrector
                              regenerates it from the same Philox counter (less memory traffic,
                              more arithmetic).  Results are identical.                         */
+    int32_t atom_order;   /* VMX_ATOM_ORDER_*: how the device stores atoms                     */
 } vmx_options;
+
+/* Atom storage order on the device.  Invisible to the caller: every array passed in or out is in
+ * the caller's order, and Philox counters use the caller's atom index. */
+enum {
+    VMX_ATOM_ORDER_AUTO = 0,   /* basis-major when the rows repeat with a lattice period        */
+    VMX_ATOM_ORDER_CALLER = 1  /* the caller's order                                            */
+};
 
 /* Device layout of the neighbour list.  Every layout keeps each row's neighbour order, so all
  * give identical results; they differ only in bytes moved per row. */
@@ -141,6 +149,9 @@ typedef struct vmx_ctx vmx_ctx;
 
 /* Layout chosen for a context (VMX_TOPOLOGY_CSR/ELL/STENCIL) and the number of stencils. */
 int vmx_get_topology(vmx_ctx* ctx, int32_t* topology, int32_t* num_stencils);
+
+/* Lattice period of the internal basis-major atom order (1: the caller's order). */
+int vmx_get_atom_period(vmx_ctx* ctx, int32_t* period);
 
 /* Text of the last error on this thread ("" if none). */
 const char* vmx_last_error(void);

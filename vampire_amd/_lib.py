@@ -90,6 +90,7 @@ class vmx_options(C.Structure):
         ("seed", C.c_uint64),
         ("topology", C.c_int32),
         ("recompute_external_field", C.c_int32),
+        ("atom_order", C.c_int32),
     ]
 
 
@@ -113,6 +114,7 @@ SIGNATURES = [
     ("vmx_create", C.c_int, [_P(vmx_system), _P(vmx_options), _P(_VP)]),
     ("vmx_destroy", C.c_int, [_VP]),
     ("vmx_get_topology", C.c_int, [_VP, _P(C.c_int32), _P(C.c_int32)]),
+    ("vmx_get_atom_period", C.c_int, [_VP, _P(C.c_int32)]),
     ("vmx_set_spins", C.c_int, [_VP, _DP, _DP, _DP]),
     ("vmx_get_spins", C.c_int, [_VP, _DP, _DP, _DP]),
     ("vmx_set_time_step", C.c_int, [_VP, C.c_double, C.c_double]),

@@ -44,7 +44,8 @@ class Context:
     """A device context for one system (or a batch of replicas)."""
 
     def __init__(self, system: System, precision: str = FP64, kernel: str = FUSED, block_size: int = 0,
-                 seed: int = 0, device: int = -1, topology: str = "auto", recompute_external_field: bool = False):
+                 seed: int = 0, device: int = -1, topology: str = "auto", recompute_external_field: bool = False,
+                 atom_order: str = "auto"):
         self.lib = L.load()
         self.system = system
         self.n = system.n
@@ -68,7 +69,8 @@ class Context:
                             kernel=L.VMX_KERNEL_FUSED if kernel == FUSED else L.VMX_KERNEL_PLAIN,
                             block_size=block_size, device=device, seed=seed,
                             topology=TOPOLOGIES[topology],
-                            recompute_external_field=int(bool(recompute_external_field)))
+                            recompute_external_field=int(bool(recompute_external_field)),
+                            atom_order={"auto": 0, "caller": 1}[atom_order])
         h = C.c_void_p()
         L.check(self.lib.vmx_create(C.byref(sysc), C.byref(opt), C.byref(h)))
         self.h = h
@@ -122,6 +124,13 @@ class Context:
         t, p = C.c_int32(0), C.c_int32(0)
         L.check(self.lib.vmx_get_topology(self.h, C.byref(t), C.byref(p)))
         return {1: "csr", 2: "ell", 3: "stencil"}[t.value], int(p.value)
+
+    @property
+    def atom_period(self) -> int:
+        """Lattice period q of the device's basis-major atom order (1: the caller's order)."""
+        v = C.c_int32(0)
+        L.check(self.lib.vmx_get_atom_period(self.h, C.byref(v)))
+        return int(v.value)
 
     @property
     def step(self) -> int:

@@ -224,6 +224,19 @@ struct vmx_ctx {
     int64_t nnz = 0;
     int et = 0, jm = 0, K = 1;
     int tp = TP_CSR, zmax = 0, npat = 0, wsel = 0;   // zmax: row stride W for ELL / stencil
+    // internal atom order: with lattice period q > 1 atoms are stored basis-major inside each
+    // replica; pi maps a caller index to the internal one, inv back (both empty when q == 1)
+    int q = 1;
+    std::vector<int> pi, inv;
+    // k consecutive blocks of n values, caller order -> internal order and back
+    void to_internal(const double* in, double* out, int k) const {
+        for (int b = 0; b < k; ++b)
+            for (int i = 0; i < n; ++i) out[(size_t)b * n + (q > 1 ? pi[i] : i)] = in[(size_t)b * n + i];
+    }
+    void to_caller(const double* in, double* out, int k) const {
+        for (int b = 0; b < k; ++b)
+            for (int i = 0; i < n; ++i) out[(size_t)b * n + i] = in[(size_t)b * n + (q > 1 ? pi[i] : i)];
+    }
     int am() const { return am_of(tp, jm, wsel); }
     std::vector<int> rowh_, colh_;   // host CSR (graph colouring for Monte Carlo)
     uint32_t ani = 0, fld = 0;
@@ -239,6 +252,7 @@ struct vmx_ctx {
     // device state
     DevBuf row, col, jidx, jtab, type, mat, cond, blk, rep_off, rep_key, step_base;
     DevBuf deg, pat, pat_off, pat_jid, pat_deg;
+    DevBuf rep_m;                 // [nrep] replica size / q
     DevBuf spin[3];               // three buffers of 3N R each (S, P, O roles rotate)
     int s_idx = 0, o_idx = 2;     // current spin buffer and the corrector's output buffer
     DevBuf plainF, plainE, plainD;
@@ -247,6 +261,14 @@ struct vmx_ctx {
     DevBuf mm, atom_rep;
     DevBuf staging;               // 3N doubles for host<->device spin copies and noise
     DevBuf noise_dev;             // 3N doubles
+    std::vector<double> noise_host;   // one step of caller noise in internal order
+    // a per-atom int array in internal order
+    std::vector<int> internal_ints(const int32_t* a) const {
+        std::vector<int> out(a, a + n);
+        if (q > 1)
+            for (int i = 0; i < n; ++i) out[pi[i]] = a[i];
+        return out;
+    }
     DevBuf partial;               // statistics partial sums
     int nblocks = 0;
     std::vector<MagStat*> mags;
@@ -307,6 +329,8 @@ struct vmx_ctx {
         s.n = n; s.nmat = nmat; s.nblocks = nblocks; s.xcd_remap = xcd_remap;
         s.zmax = zmax;
         s.am = am();
+        s.q = q;
+        s.rep_m = rep_m.as<int>();
         s.row = row.as<int>(); s.col = col.as<int>();
         s.jidx = jm == JM_UNIFORM ? nullptr : jidx.p;
         s.deg = deg.as<uint8_t>();
@@ -458,9 +482,13 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
     auto mark = [&](int e) -> hipError_t { return kernel_ms ? hipEventRecord(kev[e], stream) : hipSuccess; };
     VMX_HIP(hipEventRecord(ev0, stream));
     for (int k = 0; k < nsteps; ++k) {
-        if (noise)
-            VMX_HIP(hipMemcpyAsync(noise_dev.p, noise + (size_t)k * 3 * n, 3 * (size_t)n * sizeof(double),
-                                   hipMemcpyHostToDevice, stream));
+        if (noise) {
+            // the previous step still reads noise_dev: wait, then copy this step's block in
+            noise_host.resize(3 * (size_t)n);
+            to_internal(noise + (size_t)k * 3 * n, noise_host.data(), 3);
+            VMX_HIP(hipStreamSynchronize(stream));
+            VMX_HIP(hipMemcpy(noise_dev.p, noise_host.data(), 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+        }
         const int p_idx = 3 - s_idx - o_idx;
         int nk = 0;
         if (kernel == VMX_KERNEL_FUSED) {
@@ -588,20 +616,23 @@ int vmx_ctx::colour() {
         for (int i = 0; i < n; ++i)
             for (int e = rowh[i]; e < rowh[i + 1]; ++e) rcol[fill[colh[e]]++] = i;
     }
+    // atoms are visited in the caller's order (o), so the colouring does not depend on the
+    // internal storage order; c[] and the lists hold internal indices
     std::vector<int> c(n, -1);
-    std::vector<int> seen;  // seen[colour] == i marks colours used around atom i
+    std::vector<int> seen;  // seen[colour] == o marks colours used around the o-th atom
     int nc = 0;
-    for (int i = 0; i < n; ++i) {
+    for (int o = 0; o < n; ++o) {
+        const int i = q > 1 ? pi[o] : o;
         auto mark = [&](int j) {
             if (j != i && c[j] >= 0) {
                 if ((int)seen.size() <= c[j]) seen.resize(c[j] + 1, -1);
-                seen[c[j]] = i;
+                seen[c[j]] = o;
             }
         };
         for (int e = rowh[i]; e < rowh[i + 1]; ++e) mark(colh[e]);
         for (int e = rrow[i]; e < rrow[i + 1]; ++e) mark(rcol[e]);
         int k = 0;
-        while (k < (int)seen.size() && seen[k] == i) ++k;
+        while (k < (int)seen.size() && seen[k] == o) ++k;
         c[i] = k;
         nc = std::max(nc, k + 1);
     }
@@ -610,8 +641,10 @@ int vmx_ctx::colour() {
     colour_start.assign(nc + 1, 0);
     for (int k = 0; k < nc; ++k) {
         colour_start[k] = (int)list.size();
-        for (int i = 0; i < n; ++i)
+        for (int o = 0; o < n; ++o) {
+            const int i = q > 1 ? pi[o] : o;
             if (c[i] == k) list.push_back(i);
+        }
     }
     colour_start[nc] = (int)list.size();
     VMX_TRY(colour_list.alloc((size_t)n * sizeof(int)));
@@ -673,11 +706,17 @@ int vmx_ctx::fit_forward(const double* h, const double* spins0, double* mz_out) 
     VMX_HIP(hipStreamSynchronize(stream));
     fit.has_h = h != nullptr;
     if (h) {
-        VMX_HIP(hipMemcpy(st, h, N3 * sizeof(double), hipMemcpyHostToDevice));
+        std::vector<double> hi(N3);
+        to_internal(h, hi.data(), 3);
+        VMX_HIP(hipMemcpy(st, hi.data(), N3 * sizeof(double), hipMemcpyHostToDevice));
         VMX_HIP(Ops<R>::from_double(stream, (int)N3, st, fit.h.as<R>()));
         VMX_HIP(hipStreamSynchronize(stream));
     }
-    VMX_HIP(hipMemcpy(st, spins0, N3 * sizeof(double), hipMemcpyHostToDevice));
+    {
+        std::vector<double> si(N3);
+        to_internal(spins0, si.data(), 3);
+        VMX_HIP(hipMemcpy(st, si.data(), N3 * sizeof(double), hipMemcpyHostToDevice));
+    }
     R* ck = fit.ckpt.as<R>();
     VMX_HIP(Ops<R>::from_double(stream, (int)N3, st, ck));
     VMX_HIP(hipMemsetAsync(fit.mz.p, 0, (size_t)4 * std::max(fit.nrec, 1) * sizeof(double), stream));
@@ -768,7 +807,9 @@ int vmx_ctx::fit_backward(const double* dmz, double* grad, double* dh_out) {
         double* st = staging.as<double>();
         VMX_HIP(Ops<R>::to_double(stream, (int)N3, fit.dh.as<R>(), st));
         VMX_HIP(hipStreamSynchronize(stream));
-        VMX_HIP(hipMemcpy(dh_out, st, N3 * sizeof(double), hipMemcpyDeviceToHost));
+        std::vector<double> tmp(N3);
+        VMX_HIP(hipMemcpy(tmp.data(), st, N3 * sizeof(double), hipMemcpyDeviceToHost));
+        to_caller(tmp.data(), dh_out, 3);
     }
     return VMX_OK;
 }
@@ -884,6 +925,42 @@ int upload(DevBuf& b, const std::vector<T>& v) {
 }
 
 }  // namespace
+
+// Lattice period of the atom order.  VAMPIRE numbers atoms cell by cell with the unit-cell
+// atoms innermost (src/create/cs_create_crystal_structure2.cpp:100-107), so the rows of atoms
+// i and i + q (q = atoms per cell) carry the same j - i offsets away from the boundaries.
+// Stored as is, a wavefront's 64 consecutive rows mix q different stencils and each neighbour
+// slot reads q strided spans of memory; stored basis-major (all basis-0 atoms, then basis-1,
+// ...) the 64 rows of a wavefront read one contiguous span per slot.  Returns the q in 2..8
+// with the largest share of equal (i, i+q) rows when that share is at least 20% and larger than
+// for q = 1; else 1 (keep the order).  q must divide every replica's size.
+static int detect_period(int n, const std::vector<int>& rowh, const std::vector<int>& colh,
+                         const std::vector<int>& roff) {
+    const int stride = std::max(1, n / 200000);   // sample at most ~200k rows
+    auto share = [&](int q) {
+        int64_t same = 0, tot = 0;
+        for (int i = 0; i + q < n; i += stride) {
+            ++tot;
+            const int a = rowh[i], da = rowh[i + 1] - a, b = rowh[i + q], db = rowh[i + q + 1] - b;
+            if (da != db || da == 0) continue;
+            bool eq = true;
+            for (int k = 0; k < da && eq; ++k) eq = (colh[a + k] - i) == (colh[b + k] - (i + q));
+            same += eq;
+        }
+        return tot ? (double)same / (double)tot : 0.0;
+    };
+    const double s1 = share(1);
+    int best = 1;
+    double best_share = s1;
+    for (int q = 2; q <= 8; ++q) {
+        bool divides = true;
+        for (size_t r = 0; r + 1 < roff.size(); ++r) divides &= ((roff[r + 1] - roff[r]) % q) == 0;
+        if (!divides) continue;
+        const double sq = share(q);
+        if (sq > best_share + 1e-12) { best = q; best_share = sq; }
+    }
+    return (best > 1 && best_share >= 0.2) ? best : 1;
+}
 
 // Chooses and uploads the device layout of the neighbour list (VMX_TOPOLOGY_*).
 static int build_topology(vmx_ctx* c, int request, const std::vector<int>& rowh, const std::vector<int>& colh,
@@ -1044,7 +1121,46 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
         }
     }
 
+    // Internal atom order: basis-major when the rows repeat with a lattice period q (section
+    // comment at detect_period).  Row contents and their order are unchanged.
+    int q = 1;
+    if (opt->atom_order == VMX_ATOM_ORDER_AUTO) q = detect_period(n, rowh, colh, roff);
+    else if (opt->atom_order != VMX_ATOM_ORDER_CALLER) return fail(VMX_ERR_ARG, "unknown atom_order");
+    std::vector<int> pi, inv;
+    std::vector<int> type_int(sys->type, sys->type + n);
+    std::vector<double> mom_int(sys->moment, sys->moment + n);
+    if (q > 1) {
+        pi.resize(n);
+        inv.resize(n);
+        for (int r = 0; r < nrep; ++r) {
+            const int base = roff[r], nr = roff[r + 1] - roff[r], m = nr / q;
+            for (int l = 0; l < nr; ++l) {
+                const int p = (l % q) * m + l / q;
+                pi[base + l] = base + p;
+                inv[base + p] = base + l;
+            }
+        }
+        std::vector<int> rowp(n + 1), colp((size_t)total), iidp((size_t)total);
+        rowp[0] = 0;
+        for (int p = 0; p < n; ++p) {
+            const int i = inv[p], d = rowh[i + 1] - rowh[i];
+            rowp[p + 1] = rowp[p] + d;
+            for (int k = 0; k < d; ++k) {
+                colp[rowp[p] + k] = pi[colh[rowh[i] + k]];
+                iidp[rowp[p] + k] = iidh[rowh[i] + k];
+            }
+            type_int[p] = sys->type[i];
+            mom_int[p] = sys->moment[i];
+        }
+        rowh.swap(rowp);
+        colh.swap(colp);
+        iidh.swap(iidp);
+    }
+
     std::unique_ptr<vmx_ctx> c(new vmx_ctx());
+    c->q = q;
+    c->pi.swap(pi);
+    c->inv.swap(inv);
     c->device = opt->device;
     VMX_TRY(c->use_device());
     c->precision = opt->precision;
@@ -1083,13 +1199,18 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
         VMX_HIP(hipMemcpy(c->jtab.p, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     std::vector<uint8_t> t8(n);
-    for (int i = 0; i < n; ++i) t8[i] = (uint8_t)sys->type[i];
+    for (int i = 0; i < n; ++i) t8[i] = (uint8_t)type_int[i];
     VMX_TRY(c->type.alloc(n));
     VMX_HIP(hipMemcpy(c->type.p, t8.data(), n, hipMemcpyHostToDevice));
     VMX_TRY(c->mm.alloc((size_t)n * sizeof(double)));
-    VMX_HIP(hipMemcpy(c->mm.p, sys->moment, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    VMX_HIP(hipMemcpy(c->mm.p, mom_int.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice));
     VMX_TRY(c->rep_off.alloc((nrep + 1) * sizeof(int)));
     VMX_HIP(hipMemcpy(c->rep_off.p, roff.data(), (nrep + 1) * sizeof(int), hipMemcpyHostToDevice));
+    {
+        std::vector<int> rep_m(nrep);
+        for (int r = 0; r < nrep; ++r) rep_m[r] = (roff[r + 1] - roff[r]) / q;
+        VMX_TRY(upload(c->rep_m, rep_m));
+    }
     std::vector<uint2> keys(nrep);
     for (int r = 0; r < nrep; ++r) {
         const uint64_t k = opt->seed + (uint64_t)r;
@@ -1136,6 +1257,12 @@ int vmx_get_topology(vmx_ctx* c, int32_t* topology, int32_t* num_stencils) {
     return VMX_OK;
 }
 
+int vmx_get_atom_period(vmx_ctx* c, int32_t* period) {
+    if (!c || !period) return fail(VMX_ERR_ARG, "null argument");
+    *period = c->q;
+    return VMX_OK;
+}
+
 int vmx_destroy(vmx_ctx* ctx) {
     if (!ctx) return VMX_OK;
     (void)ctx->use_device();
@@ -1150,9 +1277,12 @@ int vmx_set_spins(vmx_ctx* c, const double* sx, const double* sy, const double* 
     const size_t nb = (size_t)c->n * sizeof(double);
     double* st = c->staging.as<double>();
     VMX_HIP(hipStreamSynchronize(c->stream));
-    VMX_HIP(hipMemcpy(st, sx, nb, hipMemcpyHostToDevice));
-    VMX_HIP(hipMemcpy(st + c->n, sy, nb, hipMemcpyHostToDevice));
-    VMX_HIP(hipMemcpy(st + 2 * (size_t)c->n, sz, nb, hipMemcpyHostToDevice));
+    std::vector<double> host(3 * (size_t)c->n), perm(3 * (size_t)c->n);
+    std::memcpy(host.data(), sx, nb);
+    std::memcpy(host.data() + c->n, sy, nb);
+    std::memcpy(host.data() + 2 * (size_t)c->n, sz, nb);
+    c->to_internal(host.data(), perm.data(), 3);
+    VMX_HIP(hipMemcpy(st, perm.data(), 3 * nb, hipMemcpyHostToDevice));
     if (c->precision == VMX_FP64) {
         VMX_HIP(hipMemcpyAsync(c->spin[c->s_idx].p, st, 3 * nb, hipMemcpyDeviceToDevice, c->stream));
     } else {
@@ -1173,9 +1303,12 @@ int vmx_get_spins(vmx_ctx* c, double* sx, double* sy, double* sz) {
         VMX_HIP(Ops<float>::to_double(c->stream, 3 * c->n, c->spin[c->s_idx].as<float>(), st));
     }
     VMX_HIP(hipStreamSynchronize(c->stream));
-    VMX_HIP(hipMemcpy(sx, st, nb, hipMemcpyDeviceToHost));
-    VMX_HIP(hipMemcpy(sy, st + c->n, nb, hipMemcpyDeviceToHost));
-    VMX_HIP(hipMemcpy(sz, st + 2 * (size_t)c->n, nb, hipMemcpyDeviceToHost));
+    std::vector<double> perm(3 * (size_t)c->n), host(3 * (size_t)c->n);
+    VMX_HIP(hipMemcpy(perm.data(), st, 3 * nb, hipMemcpyDeviceToHost));
+    c->to_caller(perm.data(), host.data(), 3);
+    std::memcpy(sx, host.data(), nb);
+    std::memcpy(sy, host.data() + c->n, nb);
+    std::memcpy(sz, host.data() + 2 * (size_t)c->n, nb);
     return VMX_OK;
 }
 
@@ -1303,7 +1436,7 @@ int vmx_stats_add_magnetization(vmx_ctx* c, int32_t mask_size, const int32_t* ma
     VMX_TRY(check_mask(c, mask_size, mask, st->empty));
     st->nbins = mask_size;
     VMX_TRY(st->mask.alloc((size_t)c->n * sizeof(int)));
-    VMX_HIP(hipMemcpy(st->mask.p, mask, (size_t)c->n * sizeof(int), hipMemcpyHostToDevice));
+    VMX_HIP(hipMemcpy(st->mask.p, c->internal_ints(mask).data(), (size_t)c->n * sizeof(int), hipMemcpyHostToDevice));
     VMX_TRY(st->cur.alloc(4 * (size_t)mask_size * sizeof(double)));
     VMX_TRY(st->mean.alloc(4 * (size_t)mask_size * sizeof(double)));
     VMX_HIP(hipMemset(st->cur.p, 0, 4 * (size_t)mask_size * sizeof(double)));
@@ -1325,7 +1458,7 @@ int vmx_stats_add_energy(vmx_ctx* c, int32_t mask_size, const int32_t* mask, int
     VMX_TRY(check_mask(c, mask_size, mask, st->empty));
     st->nbins = mask_size;
     VMX_TRY(st->mask.alloc((size_t)c->n * sizeof(int)));
-    VMX_HIP(hipMemcpy(st->mask.p, mask, (size_t)c->n * sizeof(int), hipMemcpyHostToDevice));
+    VMX_HIP(hipMemcpy(st->mask.p, c->internal_ints(mask).data(), (size_t)c->n * sizeof(int), hipMemcpyHostToDevice));
     VMX_TRY(st->cur.alloc(5 * (size_t)mask_size * sizeof(double)));
     VMX_TRY(st->mean.alloc(5 * (size_t)mask_size * sizeof(double)));
     VMX_HIP(hipMemset(st->cur.p, 0, 5 * (size_t)mask_size * sizeof(double)));
@@ -1406,8 +1539,11 @@ static int compute_fields_impl(vmx_ctx* c, const double* noise, double* h_spin, 
     const double* nd = nullptr;
     if (noise) {
         VMX_TRY(c->noise_dev.alloc(3 * (size_t)c->n * sizeof(double)));
-        VMX_HIP(hipMemcpyAsync(c->noise_dev.p, noise, 3 * (size_t)c->n * sizeof(double), hipMemcpyHostToDevice,
-                               c->stream));
+        c->noise_host.resize(3 * (size_t)c->n);
+        c->to_internal(noise, c->noise_host.data(), 3);
+        VMX_HIP(hipStreamSynchronize(c->stream));
+        VMX_HIP(hipMemcpy(c->noise_dev.p, c->noise_host.data(), 3 * (size_t)c->n * sizeof(double),
+                          hipMemcpyHostToDevice));
         nd = c->noise_dev.as<double>();
     }
     VMX_HIP(Ops<R>::spin_fields(L, c->et, c->am(), s, c->spins<R>(c->s_idx), c->arr3<R>(c->plainF)));
@@ -1420,7 +1556,9 @@ static int compute_fields_impl(vmx_ctx* c, const double* noise, double* h_spin, 
         const DevBuf& src = which == 0 ? c->plainF : c->plainE;
         VMX_HIP(Ops<R>::to_double(c->stream, 3 * c->n, src.as<R>(), st));
         VMX_HIP(hipStreamSynchronize(c->stream));
-        VMX_HIP(hipMemcpy(dst, st, nb, hipMemcpyDeviceToHost));
+        std::vector<double> tmp(3 * (size_t)c->n);
+        VMX_HIP(hipMemcpy(tmp.data(), st, nb, hipMemcpyDeviceToHost));
+        c->to_caller(tmp.data(), dst, 3);
     }
     VMX_HIP(hipStreamSynchronize(c->stream));
     return VMX_OK;
@@ -1439,7 +1577,9 @@ static int thermal_noise_impl(vmx_ctx* c, int64_t step, double* noise) {
     double* st = c->staging.as<double>();
     VMX_HIP(Ops<R>::thermal_noise(c->launch(), s, (uint64_t)step, st));
     VMX_HIP(hipStreamSynchronize(c->stream));
-    VMX_HIP(hipMemcpy(noise, st, 3 * (size_t)c->n * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> tmp(3 * (size_t)c->n);
+    VMX_HIP(hipMemcpy(tmp.data(), st, 3 * (size_t)c->n * sizeof(double), hipMemcpyDeviceToHost));
+    c->to_caller(tmp.data(), noise, 3);
     return VMX_OK;
 }
 

@@ -76,6 +76,8 @@ struct SysDev {
     int xcd_remap;                           // 1: contiguous logical blocks per XCD
     int zmax;                                // widest row (ELL slots, stencil width)
     int am;                                  // access mode the arrays below were built for
+    int q;                                   // basis-major atom order with period q (1: caller's order)
+    const int* __restrict__ rep_m;           // [nrep] replica size / q
     const int* __restrict__ row;             // CSR: [n+1]
     const int* __restrict__ col;             // CSR: [nnz]; ELL: [zmax][n]
     const void* __restrict__ jidx;           // per-edge table index (uint16 / int32), CSR or ELL layout
@@ -187,6 +189,17 @@ __device__ __forceinline__ int logical_block(int xcd_remap) {
 }
 
 template <typename R>
+__device__ __forceinline__ int caller_local(const SysDev<R>& s, int rep, int p) {
+    // internal replica-local index p = b * m + c (basis b, cell c) -> caller's c * q + b
+    if (s.q <= 1) return p;
+    const int m = s.rep_m[rep];
+    int b = 0;
+    while (p >= m) { p -= m; ++b; }   // at most q - 1 iterations
+    return p * s.q + b;
+}
+
+// i: internal atom index; local: the caller's replica-local index (keys the Philox counter)
+template <typename R>
 __device__ __forceinline__ bool my_atom(const SysDev<R>& s, int& i, int& rep, int& local) {
     const int lb = logical_block(s.xcd_remap);
     if (lb >= s.nblocks) return false;
@@ -195,7 +208,7 @@ __device__ __forceinline__ bool my_atom(const SysDev<R>& s, int& i, int& rep, in
     i = b.x + (int)threadIdx.x;
     const int r0 = s.rep_off[rep];
     if (i >= s.rep_off[rep + 1]) return false;
-    local = i - r0;
+    local = caller_local(s, rep, i - r0);
     return true;
 }
 
@@ -643,7 +656,7 @@ k_mc_colour(SysDev<R> s, Spin3<R> S, const int* __restrict__ list, int count, in
         const int i = list[t];
         const int rep = replica_of(s, nrep, i);
         myrep = rep;
-        const int local = i - s.rep_off[rep];
+        const int local = caller_local(s, rep, i - s.rep_off[rep]);
         const int m = s.type[i];
         const Cond<R>& c = s.cond[rep * s.nmat + m];
         const uint2 key = s.rep_key[rep];
