@@ -67,6 +67,40 @@ def cpu_baseline(system, nsteps, threads):
     return system.n * nsteps / (time.perf_counter() - t0)
 
 
+def copy_calibration(nbytes: int = 256 << 20, reps: int = 3) -> None:
+    """Device-to-device copies of a known size on the HIP runtime libvmx uses: under
+    rocprofv3 --pmc their FETCH_SIZE / WRITE_SIZE give the counters' scale in bytes."""
+    import ctypes as C
+    from vampire_amd import _lib
+    _lib.load()
+    hip = C.CDLL("libamdhip64.so.7")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    for _ in range(reps):
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(f"# copy calibration: {reps} x {nbytes >> 20} MiB device-to-device", file=sys.stderr, flush=True)
+
+
+def measured_traffic(n: int):
+    """HBM bytes per Heun step from the committed rocprofv3 --pmc summary of this workload
+    (FETCH_SIZE + WRITE_SIZE of both kernels, scaled by the copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "fcc100_fp32_fused_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    if d.get("atoms") != n:
+        return None
+    return d.get("traffic_bytes_per_step")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +111,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay the fused step as a HIP graph")
     ap.add_argument("--drift-steps", type=int, default=1000)
+    ap.add_argument("--legs", default="fp32_fused,fp64_fused,fp64_plain",
+                    help="comma list of fp32_fused, fp64_fused, fp64_plain (profiling runs use one)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="end with three 256 MiB device-to-device copies (counter scale for traffic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (replicas and spatial shards "
@@ -93,14 +131,19 @@ def main():
     z = int(system.nbr_end[0] - system.nbr_start[0] + 1)
     print(f"# system: FCC {args.L}^3, {n} atoms, z={z}, built in {t_build:.1f} s", file=sys.stderr, flush=True)
 
+    legs = [x for x in args.legs.split(",") if x]
+    table = {"fp32_fused": (FP32, FUSED), "fp64_fused": (FP64, FUSED), "fp64_plain": (FP64, PLAIN)}
     res = {}
-    for name, prec, kern in (("fp32_fused", FP32, FUSED), ("fp64_fused", FP64, FUSED), ("fp64_plain", FP64, PLAIN)):
+    for name in legs:
+        prec, kern = table[name]
         c, wall, kms = run_gpu(system, prec, kern, args.steps, args.warmup, use_graph=args.graph and kern == FUSED)
         res[name] = dict(wall=wall, kernel_ms=kms.tolist(), sps=n * args.steps / wall,
                          ms_per_step=1e3 * wall / args.steps)
         c.close()
         print(f"# {name}: {res[name]['ms_per_step']:.4f} ms/step, {res[name]['sps']:.3e} spin-steps/s, "
               f"kernels {['%.4f' % v for v in kms]} ms", file=sys.stderr, flush=True)
+    if args.copy_calibration:
+        copy_calibration()
 
     # FP32 vs FP64 drift after drift-steps from the same state and Philox stream
     drift = {}
@@ -123,14 +166,16 @@ def main():
                      dM_over_Ms=float(np.linalg.norm(m32 - m64)), mz_fp64=float(m64[2]), mz_fp32=float(m32[2]))
         print(f"# drift: {drift}", file=sys.stderr, flush=True)
 
-    head = res["fp32_fused"]
-    p32 = b_alg(4, z)
-    step_kernel_ms = float(sum(head["kernel_ms"]))
-    achieved = p32 * n / (step_kernel_ms * 1e-3) / 1e9
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="fused predictor + corrector (one Heun step)",
-                    bytes_per_spin_step=p32, kernel_ms=head["kernel_ms"])
+    head = res.get("fp32_fused")
+    roofline = None
+    if head is not None:
+        p32 = b_alg(4, z)
+        step_kernel_ms = float(sum(head["kernel_ms"]))
+        achieved = p32 * n / (step_kernel_ms * 1e-3) / 1e9
+        roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(n),
+                        kernel="fused predictor + corrector (one Heun step)",
+                        bytes_per_spin_step=p32, kernel_ms=head["kernel_ms"])
     cpu = None
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
@@ -142,14 +187,21 @@ def main():
                    value_1thread=round(v1, 1))
         print(f"# cpu: 1 thread {v1:.3e}, {nthreads} threads {vN:.3e} spin-steps/s", file=sys.stderr, flush=True)
 
+    def leg(name, p):
+        if name not in res:
+            return None
+        r = res[name]
+        return {"value": round(r["sps"], 1), "ms_per_step": round(r["ms_per_step"], 5), "kernel_ms": r["kernel_ms"],
+                "hbm_frac_b_alg": round(b_alg(p, z) * n / (sum(r["kernel_ms"]) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
     out = {
         "metric": METRIC,
-        "value": round(head["sps"], 1),
+        "value": None if head is None else round(head["sps"], 1),
         "unit": "spin-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(head["ms_per_step"], 5),
+        "ms_per_step": None if head is None else round(head["ms_per_step"], 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -160,13 +212,10 @@ def main():
                    "atoms": n, "neighbours": z, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "fp64_fused": {"value": round(res["fp64_fused"]["sps"], 1), "ms_per_step": round(res["fp64_fused"]["ms_per_step"], 5),
-                       "kernel_ms": res["fp64_fused"]["kernel_ms"],
-                       "hbm_frac_b_alg": round(b_alg(8, z) * n / (sum(res["fp64_fused"]["kernel_ms"]) * 1e-3) / 1e9
-                                               / HBM_PEAK_GBS, 4)},
-        "fp64_plain": {"value": round(res["fp64_plain"]["sps"], 1), "ms_per_step": round(res["fp64_plain"]["ms_per_step"], 5),
-                       "kernel_ms": res["fp64_plain"]["kernel_ms"]},
-        "fp32_over_fp64_plain": round(head["sps"] / res["fp64_plain"]["sps"], 3),
+        "fp64_fused": leg("fp64_fused", 8),
+        "fp64_plain": leg("fp64_plain", 8),
+        "fp32_over_fp64_plain": (round(head["sps"] / res["fp64_plain"]["sps"], 3)
+                                 if head is not None and "fp64_plain" in res else None),
         "fp32_vs_fp64_drift": drift,
     }
     print(json.dumps(out), flush=True)

@@ -17,8 +17,14 @@ GOLDEN_CASES = ["sc6_uniaxial", "fcc3_cubic", "bcc4_rotated", "film_dmi"]
 TOPOLOGIES = ["csr", "ell", "stencil"]
 
 
+# "fused_recompute": the fused step with the corrector recomputing dS instead of reading it back
+KERNELS = [PLAIN, FUSED, "fused_recompute"]
+
+
 def _ctx(system, precision=FP64, kernel=FUSED, seed=0, block=0, topology="auto"):
-    return Context(system, precision=precision, kernel=kernel, seed=seed, block_size=block, topology=topology)
+    recompute = kernel == "fused_recompute"
+    return Context(system, precision=precision, kernel=FUSED if recompute else kernel, seed=seed,
+                   block_size=block, topology=topology, recompute_euler=recompute)
 
 
 @pytest.mark.parametrize("topology", TOPOLOGIES)
@@ -66,7 +72,7 @@ def test_fields_match_reference(gpu, name):
 
 
 @pytest.mark.parametrize("topology", TOPOLOGIES)
-@pytest.mark.parametrize("kernel", [PLAIN, FUSED])
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("name", GOLDEN_CASES)
 def test_trajectory_matches_reference(gpu, name, kernel, topology):
     """Device FP64 fed the reference's noise stream reproduces sim::LLG_Heun: <= 1e-10 per spin,
@@ -87,7 +93,7 @@ def test_trajectory_matches_reference(gpu, name, kernel, topology):
             assert np.abs(c.get_spins() - f["snaps"][k]).max() <= 1e-10
 
 
-@pytest.mark.parametrize("kernel", [PLAIN, FUSED])
+@pytest.mark.parametrize("kernel", KERNELS)
 def test_c0_1000_steps_philox(gpu, kernel):
     """C0 (SC 20^3, 300 K): device Philox noise vs the oracle's Philox stream, 1000 steps,
     per-spin agreement <= 1e-10 (FP64)."""

@@ -89,7 +89,7 @@ class vmx_options(C.Structure):
         ("device", C.c_int32),
         ("seed", C.c_uint64),
         ("topology", C.c_int32),
-        ("recompute_external_field", C.c_int32),
+        ("recompute_euler", C.c_int32),
         ("atom_order", C.c_int32),
     ]
 

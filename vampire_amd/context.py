@@ -44,7 +44,7 @@ class Context:
     """A device context for one system (or a batch of replicas)."""
 
     def __init__(self, system: System, precision: str = FP64, kernel: str = FUSED, block_size: int = 0,
-                 seed: int = 0, device: int = -1, topology: str = "auto", recompute_external_field: bool = False,
+                 seed: int = 0, device: int = -1, topology: str = "auto", recompute_euler: bool = False,
                  atom_order: str = "auto"):
         self.lib = L.load()
         self.system = system
@@ -69,7 +69,7 @@ class Context:
                             kernel=L.VMX_KERNEL_FUSED if kernel == FUSED else L.VMX_KERNEL_PLAIN,
                             block_size=block_size, device=device, seed=seed,
                             topology=TOPOLOGIES[topology],
-                            recompute_external_field=int(bool(recompute_external_field)),
+                            recompute_euler=int(bool(recompute_euler)),
                             atom_order={"auto": 0, "caller": 1}[atom_order])
         h = C.c_void_p()
         L.check(self.lib.vmx_create(C.byref(sysc), C.byref(opt), C.byref(h)))

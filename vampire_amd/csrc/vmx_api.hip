@@ -224,6 +224,7 @@ struct vmx_ctx {
     int64_t nnz = 0;
     int et = 0, jm = 0, K = 1;
     int tp = TP_CSR, zmax = 0, npat = 0, wsel = 0;   // zmax: row stride W for ELL / stencil
+    int full_rows = 0;                               // ELL / stencil: every row has W neighbours
     // internal atom order: with lattice period q > 1 atoms are stored basis-major inside each
     // replica; pi maps a caller index to the internal one, inv back (both empty when q == 1)
     int q = 1;
@@ -256,8 +257,8 @@ struct vmx_ctx {
     DevBuf spin[3];               // three buffers of 3N R each (S, P, O roles rotate)
     int s_idx = 0, o_idx = 2;     // current spin buffer and the corrector's output buffer
     DevBuf plainF, plainE, plainD;
-    int store_hext = 1;           // fused step: predictor stores H_ext for the corrector
-    DevBuf hext_buf;              // 3N R
+    int store_euler = 1;          // fused step: predictor keeps dS for the corrector
+    DevBuf euler_buf;             // 3N R
     DevBuf mm, atom_rep;
     DevBuf staging;               // 3N doubles for host<->device spin copies and noise
     DevBuf noise_dev;             // 3N doubles
@@ -311,11 +312,11 @@ struct vmx_ctx {
         R* base = spin[idx].as<R>();
         return Spin3<R>{base, base + n, base + 2 * (size_t)n};
     }
-    // external-field arrays of the fused step, or null arrays when the corrector regenerates it
+    // Euler-derivative arrays of the fused step, or null arrays when the corrector recomputes dS
     template <typename R>
-    Spin3<R> hext() const {
-        if (!store_hext) return Spin3<R>{nullptr, nullptr, nullptr};
-        return arr3<R>(hext_buf);
+    Spin3<R> euler() const {
+        if (!store_euler) return Spin3<R>{nullptr, nullptr, nullptr};
+        return arr3<R>(euler_buf);
     }
     template <typename R>
     Spin3<R> arr3(const DevBuf& b) const {
@@ -328,6 +329,7 @@ struct vmx_ctx {
         SysDev<R> s;
         s.n = n; s.nmat = nmat; s.nblocks = nblocks; s.xcd_remap = xcd_remap;
         s.zmax = zmax;
+        s.full_rows = full_rows;
         s.am = am();
         s.q = q;
         s.rep_m = rep_m.as<int>();
@@ -342,7 +344,7 @@ struct vmx_ctx {
         s.type = type.as<uint8_t>();
         s.mat = mat.as<MatConst<R>>();
         s.cond = cond.as<Cond<R>>();
-        s.blk = blk.as<int2>();
+        s.blk = blk.as<int4>();
         s.rep_off = rep_off.as<int>();
         s.rep_key = rep_key.as<uint2>();
         s.step_base = step_base.as<unsigned long long>();
@@ -361,14 +363,34 @@ struct vmx_ctx {
         return Launch{g, block, stream};
     }
 
+    // Logical workgroups {first atom, replica, end atom, replica start}.  With a basis-major
+    // order (q > 1) each basis block of a replica is cut into chunks of `block` cells and the
+    // table runs chunk-major (chunk 0 of every basis, then chunk 1, ...): a lattice cell's q
+    // atoms, which are each other's neighbours, land on the same XCD's stretch of the table and
+    // share its L2 instead of being fetched into q different L2s.
     int build_blocks() {
-        std::vector<int2> tab;
-        for (int r = 0; r < nrep; ++r)
-            for (int a = rep_off_h[r]; a < rep_off_h[r + 1]; a += block) tab.push_back(make_int2(a, r));
-        if (tab.empty()) tab.push_back(make_int2(0, 0));
+        std::vector<int4> tab;
+        for (int r = 0; r < nrep; ++r) {
+            const int r0 = rep_off_h[r], r1 = rep_off_h[r + 1];
+            if (q > 1) {
+                const int m = (r1 - r0) / q;
+                for (int c0 = 0; c0 < m; c0 += block)
+                    for (int b = 0; b < q; ++b) {
+                        const int first = r0 + b * m + c0;
+                        tab.push_back(make_int4(first, r, first + std::min(block, m - c0), r0));
+                    }
+            } else {
+                for (int a = r0; a < r1; a += block) tab.push_back(make_int4(a, r, std::min(a + block, r1), r0));
+            }
+        }
+        if (tab.empty()) tab.push_back(make_int4(0, 0, 0, 0));
         nblocks = (int)tab.size();
-        VMX_TRY(blk.alloc(tab.size() * sizeof(int2)));
-        VMX_HIP(hipMemcpy(blk.p, tab.data(), tab.size() * sizeof(int2), hipMemcpyHostToDevice));
+        // a captured graph holds the table's address: never replay one across a rebuild
+        if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+        graph_key = -1;
+        VMX_TRY(blk.alloc(tab.size() * sizeof(int4)));
+        VMX_HIP(hipMemcpy(blk.p, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice));
         return VMX_OK;
     }
 
@@ -493,11 +515,11 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
         int nk = 0;
         if (kernel == VMX_KERNEL_FUSED) {
             VMX_HIP(mark(0));
-            VMX_HIP(Ops<R>::fused_predictor(L, et, am(), s, spins<R>(s_idx), spins<R>(p_idx), hext<R>(), nd,
+            VMX_HIP(Ops<R>::fused_predictor(L, et, am(), s, spins<R>(s_idx), spins<R>(p_idx), euler<R>(), nd,
                                             (uint32_t)k));
             VMX_HIP(mark(1));
             VMX_HIP(Ops<R>::fused_corrector(L, et, am(), s, spins<R>(s_idx), spins<R>(p_idx), spins<R>(o_idx),
-                                            hext<R>(), nd, (uint32_t)k));
+                                            euler<R>(), nd, (uint32_t)k));
             VMX_HIP(mark(2));
             nk = 2;
             std::swap(s_idx, o_idx);
@@ -546,10 +568,10 @@ int vmx_ctx::capture_graph(int nsteps) {
     int si = s_idx, oi = o_idx;
     for (int k = 0; k < nsteps; ++k) {
         const int pi = 3 - si - oi;
-        hipError_t e = Ops<R>::fused_predictor(L, et, am(), s, spins<R>(si), spins<R>(pi), hext<R>(), nullptr,
+        hipError_t e = Ops<R>::fused_predictor(L, et, am(), s, spins<R>(si), spins<R>(pi), euler<R>(), nullptr,
                                                (uint32_t)k);
         if (e == hipSuccess)
-            e = Ops<R>::fused_corrector(L, et, am(), s, spins<R>(si), spins<R>(pi), spins<R>(oi), hext<R>(), nullptr,
+            e = Ops<R>::fused_corrector(L, et, am(), s, spins<R>(si), spins<R>(pi), spins<R>(oi), euler<R>(), nullptr,
                                         (uint32_t)k);
         if (e != hipSuccess) {
             hipGraph_t g;
@@ -987,6 +1009,7 @@ static int build_topology(vmx_ctx* c, int request, const std::vector<int>& rowh,
             c->wsel = wsel;
             c->zmax = W;
             c->npat = (int)pdeg.size();
+            c->full_rows = std::all_of(pdeg.begin(), pdeg.end(), [W](uint8_t d) { return d == W; }) ? 1 : 0;
             VMX_TRY(upload(c->pat, pat));
             VMX_TRY(upload(c->pat_off, off));
             VMX_TRY(upload(c->pat_jid, pjid));
@@ -1022,6 +1045,7 @@ static int build_topology(vmx_ctx* c, int request, const std::vector<int>& rowh,
         c->tp = TP_ELL;
         c->wsel = wsel;
         c->zmax = W;
+        c->full_rows = std::all_of(deg.begin(), deg.end(), [W](uint8_t d) { return d == W; }) ? 1 : 0;
         VMX_TRY(upload(c->col, col));
         VMX_TRY(upload(c->deg, deg));
         if (c->jm == JM_IDX16) VMX_TRY(upload(c->jidx, j16));
@@ -1058,6 +1082,11 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
     *out = nullptr;
     const int n = sys->num_atoms;
     if (n <= 0) return fail(VMX_ERR_ARG, "num_atoms must be positive (got %d)", n);
+    // spin gathers use 32-bit byte offsets into each component array
+    const int64_t max_atoms = (opt->precision == VMX_FP32 ? (int64_t)1 << 30 : (int64_t)1 << 29) - 1;
+    if (n > max_atoms)
+        return fail(VMX_ERR_ARG, "num_atoms %d exceeds %lld for this precision (shard the system)", n,
+                    (long long)max_atoms);
     if (sys->num_materials < 1 || sys->num_materials > 255)
         return fail(VMX_ERR_ARG, "num_materials must be in [1,255] (got %d)", sys->num_materials);
     if (!sys->type || !sys->nbr_start || !sys->nbr_end || !sys->materials || !sys->moment)
@@ -1228,8 +1257,8 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
         VMX_HIP(hipMemset(c->spin[b].p, 0, 3 * (size_t)n * rs));
     }
     VMX_TRY(c->staging.alloc(3 * (size_t)n * sizeof(double)));
-    c->store_hext = opt->recompute_external_field ? 0 : 1;
-    if (c->kernel == VMX_KERNEL_FUSED) VMX_TRY(c->hext_buf.alloc(3 * (size_t)n * rs));
+    c->store_euler = opt->recompute_euler ? 0 : 1;
+    if (c->kernel == VMX_KERNEL_FUSED) VMX_TRY(c->euler_buf.alloc(3 * (size_t)n * rs));
     const size_t cond_bytes = (size_t)nrep * c->nmat * sizeof(Cond<double>) + 64;
     VMX_TRY(c->ring.init(std::max<size_t>(cond_bytes, 8 * (size_t)nrep + 64)));
     VMX_TRY(c->upload_conditions_any());

@@ -75,6 +75,7 @@ struct SysDev {
     int nblocks;                             // logical workgroups covering all replicas
     int xcd_remap;                           // 1: contiguous logical blocks per XCD
     int zmax;                                // widest row (ELL slots, stencil width)
+    int full_rows;                           // ELL / stencil: every row has exactly W neighbours
     int am;                                  // access mode the arrays below were built for
     int q;                                   // basis-major atom order with period q (1: caller's order)
     const int* __restrict__ rep_m;           // [nrep] replica size / q
@@ -90,7 +91,7 @@ struct SysDev {
     const uint8_t* __restrict__ type;        // [n]
     const MatConst<R>* __restrict__ mat;     // [nmat]
     const Cond<R>* __restrict__ cond;        // [nrep*nmat]
-    const int2* __restrict__ blk;            // [nblocks] {first atom, replica}
+    const int4* __restrict__ blk;            // [nblocks] {first atom, replica, end atom, replica start}
     const int* __restrict__ rep_off;         // [nrep+1]
     const uint2* __restrict__ rep_key;       // [nrep] Philox key
     const unsigned long long* __restrict__ step_base;  // Heun steps completed before this launch batch
@@ -125,10 +126,10 @@ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c
     const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint32_t hi0 = __umulhi(M0, c0), lo0 = M0 * c0;
-        const uint32_t hi1 = __umulhi(M1, c2), lo1 = M1 * c2;
-        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
-        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        // one 32x32->64 product gives both halves (v_mad_u64_u32 instead of mul_hi + mul_lo)
+        const uint64_t p0 = (uint64_t)M0 * c0, p1 = (uint64_t)M1 * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0; c1 = (uint32_t)p1; c2 = n2; c3 = (uint32_t)p0;
         k0 += W0; k1 += W1;
     }
     U4 o; o.v[0] = c0; o.v[1] = c1; o.v[2] = c2; o.v[3] = c3;
@@ -180,7 +181,8 @@ __device__ __forceinline__ void thermal_normals(const SysDev<R>& s, int rep, int
 // ---- workgroup -> atoms ------------------------------------------------------------------
 // Workgroups are dispatched round-robin over the 8 XCDs.  With xcd_remap the grid is padded to
 // a multiple of 8 and hardware block b takes logical block (b%8)*(grid/8) + b/8, so each XCD
-// walks one contiguous stretch of atoms and the neighbour rows it gathers stay in its own L2.
+// walks one contiguous stretch of the block table (one range of lattice cells, all basis atoms)
+// and the neighbour rows it gathers stay in its own L2.
 __device__ __forceinline__ int logical_block(int xcd_remap) {
     const int b = (int)blockIdx.x;
     if (!xcd_remap) return b;
@@ -203,26 +205,43 @@ template <typename R>
 __device__ __forceinline__ bool my_atom(const SysDev<R>& s, int& i, int& rep, int& local) {
     const int lb = logical_block(s.xcd_remap);
     if (lb >= s.nblocks) return false;
-    const int2 b = s.blk[lb];
+    const int4 b = s.blk[lb];
     rep = b.y;
     i = b.x + (int)threadIdx.x;
-    const int r0 = s.rep_off[rep];
-    if (i >= s.rep_off[rep + 1]) return false;
-    local = caller_local(s, rep, i - r0);
+    if (i >= b.z) return false;
+    local = caller_local(s, rep, i - b.w);
     return true;
+}
+
+// ---- raw buffer access -------------------------------------------------------------------
+// A load through a buffer resource takes the array base in SGPRs and a 32-bit byte offset in one
+// VGPR: a neighbour costs one shift of its index, shared by its x, y and z loads, instead of a
+// 64-bit address per component.  Component arrays stay below 4 GiB (checked at creation).
+constexpr int kRawBufferConfig = 0x00020000;   // gfx9 raw buffer word 3: 32-bit data format
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, kRawBufferConfig);
+}
+__device__ __forceinline__ void buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, float& v) {
+    v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, double& v) {
+    v = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
 }
 
 // ---- neighbours of atom i, in row order ------------------------------------------------------
 // Calls f(Sx, Sy, Sz, J) for every neighbour with its spin and a pointer to its 1/3/9 exchange
 // reals.  ELL and stencil rows first issue every index load of the row, then every spin load,
 // and only then accumulate (in row order, slots k < degree): the latency of a row is paid once
-// instead of once per neighbour.  Padding slots load in-bounds entries and are not accumulated.
+// instead of once per neighbour.  Padding slots load in-bounds entries and are not accumulated;
+// when every row is full (bulk lattices) a uniform branch skips the per-slot degree test.
 template <typename R, int ET, int AM, typename F>
 __device__ __forceinline__ void for_neighbours(const SysDev<R>& s, const R* __restrict__ X,
                                                const R* __restrict__ Y, const R* __restrict__ Z,
                                                int i, F&& f) {
     constexpr int K = (ET == ET_ISO) ? 1 : (ET == ET_VEC ? 3 : 9);
     constexpr int TP = Access<AM>::TP, JM = Access<AM>::JM, W = Access<AM>::W;
+    const uint32_t bytes = (uint32_t)s.n * (uint32_t)sizeof(R);
+    const __amdgpu_buffer_rsrc_t rx = make_rsrc(X, bytes), ry = make_rsrc(Y, bytes), rz = make_rsrc(Z, bytes);
     R ju[K];
     if (JM == JM_UNIFORM) {
 #pragma unroll
@@ -234,15 +253,17 @@ __device__ __forceinline__ void for_neighbours(const SysDev<R>& s, const R* __re
             const R* J = ju;
             if (JM == JM_IDX16) J = s.jtab + (size_t)K * ((const uint16_t*)s.jidx)[nn];
             if (JM == JM_IDX32) J = s.jtab + (size_t)K * ((const int*)s.jidx)[nn];
-            const int j = s.col[nn];
-            f(X[j], Y[j], Z[j], J);
+            const uint32_t o = (uint32_t)s.col[nn] * (uint32_t)sizeof(R);
+            R sx, sy, sz;
+            buf_ld(rx, o, sx); buf_ld(ry, o, sy); buf_ld(rz, o, sz);
+            f(sx, sy, sz, J);
         }
     } else {
         int jj[W];
         int jd[W];
-        int deg;
+        int deg = W;
         if constexpr (TP == TP_ELL) {
-            deg = s.deg[i];
+            if (!s.full_rows) deg = s.deg[i];
 #pragma unroll
             for (int k = 0; k < W; ++k) {
                 const size_t e = (size_t)k * (size_t)s.n + (size_t)i;
@@ -252,7 +273,7 @@ __device__ __forceinline__ void for_neighbours(const SysDev<R>& s, const R* __re
             }
         } else {
             const int p = s.pat[i];
-            deg = s.pat_deg[p];
+            if (!s.full_rows) deg = s.pat_deg[p];
             const int* __restrict__ off = s.pat_off + p * W;
             if constexpr (W % 4 == 0) {
 #pragma unroll
@@ -276,15 +297,16 @@ __device__ __forceinline__ void for_neighbours(const SysDev<R>& s, const R* __re
         R xs[W], ys[W], zs[W];
 #pragma unroll
         for (int k = 0; k < W; ++k) {
-            xs[k] = X[jj[k]];
-            ys[k] = Y[jj[k]];
-            zs[k] = Z[jj[k]];
+            const uint32_t o = (uint32_t)jj[k] * (uint32_t)sizeof(R);
+            buf_ld(rx, o, xs[k]); buf_ld(ry, o, ys[k]); buf_ld(rz, o, zs[k]);
         }
+        if (s.full_rows) {
 #pragma unroll
-        for (int k = 0; k < W; ++k) {
-            if (k < deg) {
-                const R* J = JM == JM_UNIFORM ? ju : s.jtab + (size_t)K * jd[k];
-                f(xs[k], ys[k], zs[k], J);
+            for (int k = 0; k < W; ++k) f(xs[k], ys[k], zs[k], JM == JM_UNIFORM ? ju : s.jtab + (size_t)K * jd[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                if (k < deg) f(xs[k], ys[k], zs[k], JM == JM_UNIFORM ? ju : s.jtab + (size_t)K * jd[k]);
             }
         }
     }
@@ -388,10 +410,18 @@ __device__ __forceinline__ float inv_norm(float x, float y, float z) {
 }
 
 // ======================= fused kernels: two launches per Heun step ========================
-// Predictor: S' = |S + dt f(S, H_spin(S) + H_ext)|, written to P.
+// FP32 keeps A = S + dt/2 dS instead of dS, so its corrector reads neither S nor dS: one array
+// less per atom.  FP64 keeps dS and forms S + dt/2 (dS + dS') in the reference's order, which is
+// what makes its results equal to the reference bit for bit.
+template <typename R> struct FoldEuler { static constexpr bool value = false; };
+template <> struct FoldEuler<float> { static constexpr bool value = true; };
+
+// Predictor: S' = |S + dt f(S, H_spin(S) + H_ext)|, written to P.  With D.x set the Euler
+// derivative f(S, H) (FP64) or A (FP32) is kept for the corrector, as the reference keeps its
+// euler arrays (src/simulate/LLGHeun.cpp:201-211).
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(512)
-k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> E, const double* __restrict__ noise,
+k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, const double* __restrict__ noise,
                   uint32_t step_off) {
     int i, rep, local;
     if (!my_atom(s, i, rep, local)) return;
@@ -401,22 +431,27 @@ k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> E, const double*
     R fx, fy, fz, ex, ey, ez;
     spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m, sx, sy, sz, fx, fy, fz);
     ext_field(s, rep, m, i, local, step, noise, ex, ey, ez);
-    if (E.x) {   // keep H_ext for the corrector instead of regenerating the noise there
-        E.x[i] = ex; E.y[i] = ey; E.z[i] = ez;
-    }
     const MatConst<R>& mc = s.mat[m];
     R d0, d1, d2;
     llg_rhs(mc.one_opa2, mc.alpha_opa2, sx, sy, sz, fx + ex, fy + ey, fz + ez, d0, d1, d2);
+    if (D.x) {
+        if constexpr (FoldEuler<R>::value) {
+            D.x[i] = sx + s.half_dt * d0; D.y[i] = sy + s.half_dt * d1; D.z[i] = sz + s.half_dt * d2;
+        } else {
+            D.x[i] = d0; D.y[i] = d1; D.z[i] = d2;
+        }
+    }
     R n0 = sx + d0 * s.dt, n1 = sy + d1 * s.dt, n2 = sz + d2 * s.dt;
     const R r = inv_norm(n0, n1, n2);
     P.x[i] = n0 * r; P.y[i] = n1 * r; P.z[i] = n2 * r;
 }
 
-// Corrector: recomputes dS at S with the same H_ext (read back when the predictor stored it,
-// else regenerated from the same Philox counter), dS' at S', S_next = |S + dt/2 (dS + dS')|.
+// Corrector: dS' at S' with the same H_ext (regenerated from the same Philox counter), then
+// S_next = |S + dt/2 (dS + dS')| (LLGHeun.cpp:257-269) with dS read back when the predictor
+// kept it, else recomputed; FP32 with a kept A forms |A + dt/2 dS'|.
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(512)
-k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> E,
+k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D,
                   const double* __restrict__ noise, uint32_t step_off) {
     int i, rep, local;
     if (!my_atom(s, i, rep, local)) return;
@@ -424,23 +459,31 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> E,
     const int m = s.type[i];
     const MatConst<R>& mc = s.mat[m];
     R ex, ey, ez;
-    if (E.x) {
-        ex = E.x[i]; ey = E.y[i]; ez = E.z[i];
-    } else {
-        ext_field(s, rep, m, i, local, step, noise, ex, ey, ez);
-    }
-    const R sx = S.x[i], sy = S.y[i], sz = S.z[i];
-    R fx, fy, fz;
-    spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m, sx, sy, sz, fx, fy, fz);
-    R e0, e1, e2;
-    llg_rhs(mc.one_opa2, mc.alpha_opa2, sx, sy, sz, fx + ex, fy + ey, fz + ez, e0, e1, e2);
+    ext_field(s, rep, m, i, local, step, noise, ex, ey, ez);
     const R px = P.x[i], py = P.y[i], pz = P.z[i];
+    R fx, fy, fz;
     spin_field<R, ET, AM>(s, P.x, P.y, P.z, i, m, px, py, pz, fx, fy, fz);
     R h0, h1, h2;
     llg_rhs(mc.one_opa2, mc.alpha_opa2, px, py, pz, fx + ex, fy + ey, fz + ez, h0, h1, h2);
-    R n0 = sx + s.half_dt * (e0 + h0);
-    R n1 = sy + s.half_dt * (e1 + h1);
-    R n2 = sz + s.half_dt * (e2 + h2);
+    R n0, n1, n2;
+    if (FoldEuler<R>::value && D.x) {
+        n0 = D.x[i] + s.half_dt * h0;
+        n1 = D.y[i] + s.half_dt * h1;
+        n2 = D.z[i] + s.half_dt * h2;
+    } else {
+        const R sx = S.x[i], sy = S.y[i], sz = S.z[i];
+        R e0, e1, e2;
+        if (D.x) {
+            e0 = D.x[i]; e1 = D.y[i]; e2 = D.z[i];
+        } else {
+            R gx, gy, gz;
+            spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m, sx, sy, sz, gx, gy, gz);
+            llg_rhs(mc.one_opa2, mc.alpha_opa2, sx, sy, sz, gx + ex, gy + ey, gz + ez, e0, e1, e2);
+        }
+        n0 = sx + s.half_dt * (e0 + h0);
+        n1 = sy + s.half_dt * (e1 + h1);
+        n2 = sz + s.half_dt * (e2 + h2);
+    }
     const R r = inv_norm(n0, n1, n2);
     O.x[i] = n0 * r; O.y[i] = n1 * r; O.z[i] = n2 * r;
 }

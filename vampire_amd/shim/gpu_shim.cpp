@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "atoms.hpp"
+#include "dipole.hpp"
 #include "errors.hpp"
 #include "exchange.hpp"
 #include "gpu.hpp"
@@ -136,6 +137,10 @@ void initialize() {
         ai::enable_neel_anisotropy || ai::enable_lattice_anisotropy || exchange::biquadratic) {
         std::cerr << "Error: the GPU backend supports second-order uniaxial and fourth-order cubic anisotropy "
                      "and bilinear exchange only" << std::endl;
+        err::vexit();
+    }
+    if (dipole::activated) {
+        std::cerr << "Error: the GPU backend does not compute the dipole field" << std::endl;
         err::vexit();
     }
 
