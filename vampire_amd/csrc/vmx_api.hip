@@ -251,7 +251,7 @@ struct vmx_ctx {
     std::vector<double> H_h;      // per replica [3]
 
     // device state
-    DevBuf row, col, jidx, jtab, type, mat, cond, blk, rep_off, rep_key, step_base;
+    DevBuf row, col, jidx, jtab, type, mat, cond, blk, blk_fused, rep_off, rep_key, step_base;
     DevBuf deg, pat, pat_off, pat_jid, pat_deg;
     DevBuf rep_m;                 // [nrep] replica size / q
     DevBuf spin[3];               // three buffers of 3N R each (S, P, O roles rotate)
@@ -272,6 +272,7 @@ struct vmx_ctx {
     }
     DevBuf partial;               // statistics partial sums
     int nblocks = 0;
+    int nblocks_fused = 0;        // entries of blk_fused (each FusedAtoms * block atoms)
     std::vector<MagStat*> mags;
     std::vector<EnergyStat*> energies;
 
@@ -330,6 +331,11 @@ struct vmx_ctx {
         s.n = n; s.nmat = nmat; s.nblocks = nblocks; s.xcd_remap = xcd_remap;
         s.zmax = zmax;
         s.full_rows = full_rows;
+        s.npat = npat;
+        // the fused kernels stage the stencil offsets in LDS when they take at most 16 KiB and
+        // the whole staging area stays within 64 KiB
+        s.pat_lds = (tp == TP_PAT && (size_t)npat * zmax * sizeof(int) <= 16384 &&
+                     fused_lds_bytes<R>(nmat, npat, zmax, 1) <= 65536) ? 1 : 0;
         s.am = am();
         s.q = q;
         s.rep_m = rep_m.as<int>();
@@ -362,35 +368,59 @@ struct vmx_ctx {
         if (xcd_remap) g = (g + 7) / 8 * 8;
         return Launch{g, block, stream};
     }
+    // the fused kernels take FusedAtoms<R> atoms per thread from their own block table
+    int fused_atoms() const {
+        return precision == VMX_FP32 ? FusedAtoms<float>::value : FusedAtoms<double>::value;
+    }
+    Launch launch_fused() const {
+        int g = nblocks_fused;
+        if (xcd_remap) g = (g + 7) / 8 * 8;
+        return Launch{g, block, stream};
+    }
+    template <typename R>
+    SysDev<R> sys_fused() const {
+        SysDev<R> s = sys<R>();
+        s.blk = blk_fused.as<int4>();
+        s.nblocks = nblocks_fused;
+        return s;
+    }
 
     // Logical workgroups {first atom, replica, end atom, replica start}.  With a basis-major
-    // order (q > 1) each basis block of a replica is cut into chunks of `block` cells and the
+    // order (q > 1) each basis block of a replica is cut into chunks of `span` cells and the
     // table runs chunk-major (chunk 0 of every basis, then chunk 1, ...): a lattice cell's q
     // atoms, which are each other's neighbours, land on the same XCD's stretch of the table and
     // share its L2 instead of being fetched into q different L2s.
-    int build_blocks() {
+    static std::vector<int4> block_table(int nrep, const std::vector<int>& off, int q, int span) {
         std::vector<int4> tab;
         for (int r = 0; r < nrep; ++r) {
-            const int r0 = rep_off_h[r], r1 = rep_off_h[r + 1];
+            const int r0 = off[r], r1 = off[r + 1];
             if (q > 1) {
                 const int m = (r1 - r0) / q;
-                for (int c0 = 0; c0 < m; c0 += block)
+                for (int c0 = 0; c0 < m; c0 += span)
                     for (int b = 0; b < q; ++b) {
                         const int first = r0 + b * m + c0;
-                        tab.push_back(make_int4(first, r, first + std::min(block, m - c0), r0));
+                        tab.push_back(make_int4(first, r, first + std::min(span, m - c0), r0));
                     }
             } else {
-                for (int a = r0; a < r1; a += block) tab.push_back(make_int4(a, r, std::min(a + block, r1), r0));
+                for (int a = r0; a < r1; a += span) tab.push_back(make_int4(a, r, std::min(a + span, r1), r0));
             }
         }
         if (tab.empty()) tab.push_back(make_int4(0, 0, 0, 0));
+        return tab;
+    }
+    int build_blocks() {
+        const std::vector<int4> tab = block_table(nrep, rep_off_h, q, block);
+        const std::vector<int4> tabf = block_table(nrep, rep_off_h, q, block * fused_atoms());
         nblocks = (int)tab.size();
-        // a captured graph holds the table's address: never replay one across a rebuild
+        nblocks_fused = (int)tabf.size();
+        // a captured graph holds the tables' addresses: never replay one across a rebuild
         if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
         if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
         graph_key = -1;
         VMX_TRY(blk.alloc(tab.size() * sizeof(int4)));
         VMX_HIP(hipMemcpy(blk.p, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice));
+        VMX_TRY(blk_fused.alloc(tabf.size() * sizeof(int4)));
+        VMX_HIP(hipMemcpy(blk_fused.p, tabf.data(), tabf.size() * sizeof(int4), hipMemcpyHostToDevice));
         return VMX_OK;
     }
 
@@ -483,8 +513,8 @@ struct vmx_ctx {
 // --------------------------------------------------------------------------------------------
 template <typename R>
 int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
-    const SysDev<R> s = sys<R>();
-    const Launch L = launch();
+    const SysDev<R> s = sys<R>(), sf = sys_fused<R>();
+    const Launch L = launch(), Lf = launch_fused();
     const double* nd = nullptr;
     if (noise) {
         VMX_TRY(noise_dev.alloc(3 * (size_t)n * sizeof(double)));
@@ -515,10 +545,10 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
         int nk = 0;
         if (kernel == VMX_KERNEL_FUSED) {
             VMX_HIP(mark(0));
-            VMX_HIP(Ops<R>::fused_predictor(L, et, am(), s, spins<R>(s_idx), spins<R>(p_idx), euler<R>(), nd,
+            VMX_HIP(Ops<R>::fused_predictor(Lf, et, am(), sf, spins<R>(s_idx), spins<R>(p_idx), euler<R>(), nd,
                                             (uint32_t)k));
             VMX_HIP(mark(1));
-            VMX_HIP(Ops<R>::fused_corrector(L, et, am(), s, spins<R>(s_idx), spins<R>(p_idx), spins<R>(o_idx),
+            VMX_HIP(Ops<R>::fused_corrector(Lf, et, am(), sf, spins<R>(s_idx), spins<R>(p_idx), spins<R>(o_idx),
                                             euler<R>(), nd, (uint32_t)k));
             VMX_HIP(mark(2));
             nk = 2;
@@ -562,8 +592,8 @@ template <typename R>
 int vmx_ctx::capture_graph(int nsteps) {
     if (graph_exec) { VMX_HIP(hipGraphExecDestroy(graph_exec)); graph_exec = nullptr; }
     if (graph) { VMX_HIP(hipGraphDestroy(graph)); graph = nullptr; }
-    const SysDev<R> s = sys<R>();
-    const Launch L = launch();
+    const SysDev<R> s = sys_fused<R>();
+    const Launch L = launch_fused();
     VMX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     int si = s_idx, oi = o_idx;
     for (int k = 0; k < nsteps; ++k) {

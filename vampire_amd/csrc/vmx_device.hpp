@@ -76,6 +76,8 @@ struct SysDev {
     int xcd_remap;                           // 1: contiguous logical blocks per XCD
     int zmax;                                // widest row (ELL slots, stencil width)
     int full_rows;                           // ELL / stencil: every row has exactly W neighbours
+    int npat;                                // stencils in the table
+    int pat_lds;                             // 1: fused kernels stage the stencil offsets in LDS
     int am;                                  // access mode the arrays below were built for
     int q;                                   // basis-major atom order with period q (1: caller's order)
     const int* __restrict__ rep_m;           // [nrep] replica size / q
@@ -229,19 +231,118 @@ __device__ __forceinline__ void buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, d
 }
 
 // ---- neighbours of atom i, in row order ------------------------------------------------------
-// Calls f(Sx, Sy, Sz, J) for every neighbour with its spin and a pointer to its 1/3/9 exchange
-// reals.  ELL and stencil rows first issue every index load of the row, then every spin load,
-// and only then accumulate (in row order, slots k < degree): the latency of a row is paid once
-// instead of once per neighbour.  Padding slots load in-bounds entries and are not accumulated;
-// when every row is full (bulk lattices) a uniform branch skips the per-slot degree test.
+// ELL and stencil rows are read in three phases: the row's indices (row_index), then every
+// neighbour spin (row_load), then the sum in row order over slots k < degree (row_sum).  Kernels
+// that handle several atoms per thread run each phase for all of them before the next, so the
+// memory latency of a phase is paid once per thread.  Padding slots load in-bounds entries and
+// are not summed; when every row is full (bulk lattices) a uniform branch skips the per-slot
+// degree test.
+template <int AM>
+struct RowIdx {
+    static constexpr int W = Access<AM>::W > 0 ? Access<AM>::W : 1;
+    int j[W];       // neighbour atom
+    int jd[W];      // J table index (16/32-bit J modes)
+    int deg;        // row length
+};
+template <typename R, int AM>
+struct RowSpins {
+    static constexpr int W = RowIdx<AM>::W;
+    R x[W], y[W], z[W];
+};
+
+// First load of a row: the stencil id (stencil layout; nothing for ELL).
+template <typename R, int AM>
+__device__ __forceinline__ int row_head(const SysDev<R>& s, int i) {
+    if constexpr (Access<AM>::TP == TP_PAT) return s.pat[i];
+    return 0;
+}
+
+// off_tab: the stencil offset table, in global memory or staged in LDS.
+template <typename R, int AM>
+__device__ __forceinline__ void row_index(const SysDev<R>& s, int i, int head, const int* __restrict__ off_tab,
+                                          RowIdx<AM>& r) {
+    constexpr int TP = Access<AM>::TP, JM = Access<AM>::JM, W = Access<AM>::W;
+    static_assert(TP != TP_CSR, "CSR rows are walked, not staged");
+    r.deg = W;
+    if constexpr (TP == TP_ELL) {
+        if (!s.full_rows) r.deg = s.deg[i];
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            const size_t e = (size_t)k * (size_t)s.n + (size_t)i;
+            r.j[k] = s.col[e];
+            if (JM == JM_IDX16) r.jd[k] = ((const uint16_t*)s.jidx)[e];
+            if (JM == JM_IDX32) r.jd[k] = ((const int*)s.jidx)[e];
+        }
+    } else {
+        const int p = head;
+        if (!s.full_rows) r.deg = s.pat_deg[p];
+        const int* __restrict__ off = off_tab + p * W;
+        if constexpr (W % 4 == 0) {
+#pragma unroll
+            for (int q = 0; q < W / 4; ++q) {
+                const int4 o = reinterpret_cast<const int4*>(off)[q];
+                r.j[4 * q + 0] = i + o.x; r.j[4 * q + 1] = i + o.y;
+                r.j[4 * q + 2] = i + o.z; r.j[4 * q + 3] = i + o.w;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < W / 2; ++q) {
+                const int2 o = reinterpret_cast<const int2*>(off)[q];
+                r.j[2 * q + 0] = i + o.x; r.j[2 * q + 1] = i + o.y;
+            }
+        }
+        if (JM != JM_UNIFORM) {
+#pragma unroll
+            for (int k = 0; k < W; ++k) r.jd[k] = s.pat_jid[p * W + k];
+        }
+    }
+}
+
+// Buffer resources of one SoA spin buffer (n reals per component).
+template <typename R>
+struct SpinRsrc {
+    __amdgpu_buffer_rsrc_t x, y, z;
+    __device__ __forceinline__ SpinRsrc(const SysDev<R>& s, const R* X, const R* Y, const R* Z) {
+        const uint32_t bytes = (uint32_t)s.n * (uint32_t)sizeof(R);
+        x = make_rsrc(X, bytes); y = make_rsrc(Y, bytes); z = make_rsrc(Z, bytes);
+    }
+    __device__ __forceinline__ void load(int j, R& vx, R& vy, R& vz) const {
+        const uint32_t o = (uint32_t)j * (uint32_t)sizeof(R);
+        buf_ld(x, o, vx); buf_ld(y, o, vy); buf_ld(z, o, vz);
+    }
+};
+
+template <typename R, int AM>
+__device__ __forceinline__ void row_load(const SpinRsrc<R>& rs, const RowIdx<AM>& r, RowSpins<R, AM>& v) {
+#pragma unroll
+    for (int k = 0; k < Access<AM>::W; ++k) rs.load(r.j[k], v.x[k], v.y[k], v.z[k]);
+}
+
+template <typename R, int ET, int AM, typename F>
+__device__ __forceinline__ void row_sum(const SysDev<R>& s, const RowIdx<AM>& r, const RowSpins<R, AM>& v,
+                                        const R* ju, F&& f) {
+    constexpr int K = (ET == ET_ISO) ? 1 : (ET == ET_VEC ? 3 : 9);
+    constexpr int JM = Access<AM>::JM, W = Access<AM>::W;
+    if (s.full_rows) {
+#pragma unroll
+        for (int k = 0; k < W; ++k) f(v.x[k], v.y[k], v.z[k], JM == JM_UNIFORM ? ju : s.jtab + (size_t)K * r.jd[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            if (k < r.deg) f(v.x[k], v.y[k], v.z[k], JM == JM_UNIFORM ? ju : s.jtab + (size_t)K * r.jd[k]);
+        }
+    }
+}
+
+// Calls f(Sx, Sy, Sz, J) for every neighbour of atom i in row order, with its spin and a pointer
+// to its 1/3/9 exchange reals.
 template <typename R, int ET, int AM, typename F>
 __device__ __forceinline__ void for_neighbours(const SysDev<R>& s, const R* __restrict__ X,
                                                const R* __restrict__ Y, const R* __restrict__ Z,
                                                int i, F&& f) {
     constexpr int K = (ET == ET_ISO) ? 1 : (ET == ET_VEC ? 3 : 9);
-    constexpr int TP = Access<AM>::TP, JM = Access<AM>::JM, W = Access<AM>::W;
-    const uint32_t bytes = (uint32_t)s.n * (uint32_t)sizeof(R);
-    const __amdgpu_buffer_rsrc_t rx = make_rsrc(X, bytes), ry = make_rsrc(Y, bytes), rz = make_rsrc(Z, bytes);
+    constexpr int TP = Access<AM>::TP, JM = Access<AM>::JM;
+    const SpinRsrc<R> rs(s, X, Y, Z);
     R ju[K];
     if (JM == JM_UNIFORM) {
 #pragma unroll
@@ -253,62 +354,16 @@ __device__ __forceinline__ void for_neighbours(const SysDev<R>& s, const R* __re
             const R* J = ju;
             if (JM == JM_IDX16) J = s.jtab + (size_t)K * ((const uint16_t*)s.jidx)[nn];
             if (JM == JM_IDX32) J = s.jtab + (size_t)K * ((const int*)s.jidx)[nn];
-            const uint32_t o = (uint32_t)s.col[nn] * (uint32_t)sizeof(R);
             R sx, sy, sz;
-            buf_ld(rx, o, sx); buf_ld(ry, o, sy); buf_ld(rz, o, sz);
+            rs.load(s.col[nn], sx, sy, sz);
             f(sx, sy, sz, J);
         }
     } else {
-        int jj[W];
-        int jd[W];
-        int deg = W;
-        if constexpr (TP == TP_ELL) {
-            if (!s.full_rows) deg = s.deg[i];
-#pragma unroll
-            for (int k = 0; k < W; ++k) {
-                const size_t e = (size_t)k * (size_t)s.n + (size_t)i;
-                jj[k] = s.col[e];
-                if (JM == JM_IDX16) jd[k] = ((const uint16_t*)s.jidx)[e];
-                if (JM == JM_IDX32) jd[k] = ((const int*)s.jidx)[e];
-            }
-        } else {
-            const int p = s.pat[i];
-            if (!s.full_rows) deg = s.pat_deg[p];
-            const int* __restrict__ off = s.pat_off + p * W;
-            if constexpr (W % 4 == 0) {
-#pragma unroll
-                for (int q = 0; q < W / 4; ++q) {
-                    const int4 o = reinterpret_cast<const int4*>(off)[q];
-                    jj[4 * q + 0] = i + o.x; jj[4 * q + 1] = i + o.y;
-                    jj[4 * q + 2] = i + o.z; jj[4 * q + 3] = i + o.w;
-                }
-            } else {
-#pragma unroll
-                for (int q = 0; q < W / 2; ++q) {
-                    const int2 o = reinterpret_cast<const int2*>(off)[q];
-                    jj[2 * q + 0] = i + o.x; jj[2 * q + 1] = i + o.y;
-                }
-            }
-            if (JM != JM_UNIFORM) {
-#pragma unroll
-                for (int k = 0; k < W; ++k) jd[k] = s.pat_jid[p * W + k];
-            }
-        }
-        R xs[W], ys[W], zs[W];
-#pragma unroll
-        for (int k = 0; k < W; ++k) {
-            const uint32_t o = (uint32_t)jj[k] * (uint32_t)sizeof(R);
-            buf_ld(rx, o, xs[k]); buf_ld(ry, o, ys[k]); buf_ld(rz, o, zs[k]);
-        }
-        if (s.full_rows) {
-#pragma unroll
-            for (int k = 0; k < W; ++k) f(xs[k], ys[k], zs[k], JM == JM_UNIFORM ? ju : s.jtab + (size_t)K * jd[k]);
-        } else {
-#pragma unroll
-            for (int k = 0; k < W; ++k) {
-                if (k < deg) f(xs[k], ys[k], zs[k], JM == JM_UNIFORM ? ju : s.jtab + (size_t)K * jd[k]);
-            }
-        }
+        RowIdx<AM> r;
+        row_index(s, i, row_head<R, AM>(s, i), s.pat_off, r);
+        RowSpins<R, AM> v;
+        row_load(rs, r, v);
+        row_sum<R, ET, AM>(s, r, v, ju, f);
     }
 }
 
@@ -316,46 +371,44 @@ __device__ __forceinline__ void for_neighbours(const SysDev<R>& s, const R* __re
 // Exchange: src/exchange/exchange_fields.cpp:52-78 (isotropic), :85-116 (vectorial),
 // :122-162 (tensorial); the sum starts at 0 and adds one neighbour at a time in row order.
 // Anisotropy terms are then added in the order of src/anisotropy/fields.cpp:44-59.
-template <typename R, int ET, int AM>
-__device__ __forceinline__ void spin_field(const SysDev<R>& s, const R* __restrict__ X,
-                                           const R* __restrict__ Y, const R* __restrict__ Z,
-                                           int i, int m, R sx, R sy, R sz,
-                                           R& fx, R& fy, R& fz) {
-    R hx = R(0), hy = R(0), hz = R(0);
-    for_neighbours<R, ET, AM>(s, X, Y, Z, i, [&](const R Sx, const R Sy, const R Sz, const R* J) {
-        if (ET == ET_ISO) {
-            const R Jij = J[0];
-            hx += Jij * Sx;
-            hy += Jij * Sy;
-            hz += Jij * Sz;
-        } else if (ET == ET_VEC) {
-            hx += J[0] * Sx;
-            hy += J[1] * Sy;
-            hz += J[2] * Sz;
-        } else {
-            hx += (J[0] * Sx + J[1] * Sy + J[2] * Sz);
-            hy += (J[3] * Sx + J[4] * Sy + J[5] * Sz);
-            hz += (J[6] * Sx + J[7] * Sy + J[8] * Sz);
-        }
-    });
-    // field array starts at zero (fields.cpp:79-81), exchange is added to it
+template <typename R, int ET>
+__device__ __forceinline__ void exchange_term(const R* J, R Sx, R Sy, R Sz, R& hx, R& hy, R& hz) {
+    if (ET == ET_ISO) {
+        const R Jij = J[0];
+        hx += Jij * Sx;
+        hy += Jij * Sy;
+        hz += Jij * Sz;
+    } else if (ET == ET_VEC) {
+        hx += J[0] * Sx;
+        hy += J[1] * Sy;
+        hz += J[2] * Sz;
+    } else {
+        hx += (J[0] * Sx + J[1] * Sy + J[2] * Sz);
+        hy += (J[3] * Sx + J[4] * Sy + J[5] * Sz);
+        hz += (J[6] * Sx + J[7] * Sy + J[8] * Sz);
+    }
+}
+
+// fx..fz = 0 + exchange (the field array starts at zero, fields.cpp:79-81), then anisotropy.
+template <typename R>
+__device__ __forceinline__ void add_anisotropy(uint32_t ani, const MatConst<R>& mc, R sx, R sy, R sz, R hx, R hy,
+                                               R hz, R& fx, R& fy, R& fz) {
     fx = R(0); fy = R(0); fz = R(0);
     fx += hx; fy += hy; fz += hz;
-    if (s.ani) {
-        const MatConst<R>& mc = s.mat[m];
-        if (s.ani & ANI_UNI2) {  // uniaxial_second_order.cpp:68-94, scale 2
+    if (ani) {
+        if (ani & ANI_UNI2) {  // uniaxial_second_order.cpp:68-94, scale 2
             const R ex = mc.ku[0], ey = mc.ku[1], ez = mc.ku[2];
             const R sdote = (sx * ex + sy * ey + sz * ez);
             const R k2 = R(2.0) * mc.ku2 * sdote;
             fx += ex * k2; fy += ey * k2; fz += ez * k2;
         }
-        if (s.ani & ANI_CUB4) {  // cubic_fourth_order.cpp:48-68, scale 0.5*4
+        if (ani & ANI_CUB4) {  // cubic_fourth_order.cpp:48-68, scale 0.5*4
             const R k4 = R(2.0) * mc.kc4;
             fx += sx * sx * sx * k4;
             fy += sy * sy * sy * k4;
             fz += sz * sz * sz * k4;
         }
-        if (s.ani & ANI_CUB4R) {  // cubic_fourth_order_rotation.cpp:48-95
+        if (ani & ANI_CUB4R) {  // cubic_fourth_order_rotation.cpp:48-95
             const R k4 = R(2.0) * mc.kc4;
             const R d1 = sx * mc.e1[0] + sy * mc.e1[1] + sz * mc.e1[2];
             const R d2 = sx * mc.e2[0] + sy * mc.e2[1] + sz * mc.e2[2];
@@ -368,14 +421,37 @@ __device__ __forceinline__ void spin_field(const SysDev<R>& s, const R* __restri
     }
 }
 
+template <typename R, int ET, int AM>
+__device__ __forceinline__ void spin_field(const SysDev<R>& s, const R* __restrict__ X,
+                                           const R* __restrict__ Y, const R* __restrict__ Z,
+                                           int i, int m, R sx, R sy, R sz,
+                                           R& fx, R& fy, R& fz) {
+    R hx = R(0), hy = R(0), hz = R(0);
+    for_neighbours<R, ET, AM>(s, X, Y, Z, i, [&](const R Sx, const R Sy, const R Sz, const R* J) {
+        exchange_term<R, ET>(J, Sx, Sy, Sz, hx, hy, hz);
+    });
+    add_anisotropy(s.ani, s.mat[m], sx, sy, sz, hx, hy, hz, fx, fy, fz);
+}
+
+// The same field from a staged row, with the atom's material record mc.
+template <typename R, int ET, int AM>
+__device__ __forceinline__ void row_field(const SysDev<R>& s, const RowIdx<AM>& r, const RowSpins<R, AM>& v,
+                                          const R* ju, const MatConst<R>& mc, R sx, R sy, R sz, R& fx, R& fy,
+                                          R& fz) {
+    R hx = R(0), hy = R(0), hz = R(0);
+    row_sum<R, ET, AM>(s, r, v, ju, [&](const R Sx, const R Sy, const R Sz, const R* J) {
+        exchange_term<R, ET>(J, Sx, Sy, Sz, hx, hy, hz);
+    });
+    add_anisotropy(s.ani, mc, sx, sy, sz, hx, hy, hz, fx, fy, fz);
+}
+
 // ---- external field at atom i (thermal + applied), src/simulate/fields.cpp:120-289 ----------
 // The array starts at 0; the thermal term overwrites it with N(0,1)*sigma (generate, then *=),
 // then the applied field is added.
 template <typename R>
-__device__ __forceinline__ void ext_field(const SysDev<R>& s, int rep, int m, int i, int local,
+__device__ __forceinline__ void ext_field(const SysDev<R>& s, const Cond<R>& c, int rep, int i, int local,
                                           uint64_t step, const double* __restrict__ noise,
                                           R& ex, R& ey, R& ez) {
-    const Cond<R>& c = s.cond[rep * s.nmat + m];
     ex = R(0); ey = R(0); ez = R(0);
     if (s.fld & FLD_THERMAL) {
         R gx, gy, gz;
@@ -391,6 +467,12 @@ __device__ __forceinline__ void ext_field(const SysDev<R>& s, int rep, int m, in
     if (s.fld & FLD_APPLIED) {
         ex += c.happ[0]; ey += c.happ[1]; ez += c.happ[2];
     }
+}
+template <typename R>
+__device__ __forceinline__ void ext_field(const SysDev<R>& s, int rep, int m, int i, int local,
+                                          uint64_t step, const double* __restrict__ noise,
+                                          R& ex, R& ey, R& ez) {
+    ext_field(s, s.cond[rep * s.nmat + m], rep, i, local, step, noise, ex, ey, ez);
 }
 
 // ---- LLG right-hand side, src/simulate/LLGHeun.cpp:201-203 (term order kept) --------------
@@ -416,6 +498,86 @@ __device__ __forceinline__ float inv_norm(float x, float y, float z) {
 template <typename R> struct FoldEuler { static constexpr bool value = false; };
 template <> struct FoldEuler<float> { static constexpr bool value = true; };
 
+// Atoms per thread of the fused kernels: a workgroup of B threads takes kFusedAtoms * B
+// consecutive atoms of its block-table entry (thread t: t, t + B, ...), and each memory phase
+// (own data and stencil id, row indices, neighbour spins) is issued for all of a thread's atoms
+// before any of them is waited for.
+template <typename R> struct FusedAtoms { static constexpr int value = 1; };
+template <> struct FusedAtoms<float> { static constexpr int value = 1; };
+
+template <typename R, int ET>
+struct UniformJ {            // the exchange reals when every edge carries the same J
+    static constexpr int K = (ET == ET_ISO) ? 1 : (ET == ET_VEC ? 3 : 9);
+    R v[K];
+    __device__ __forceinline__ UniformJ(const SysDev<R>& s, int jm) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] = jm == JM_UNIFORM ? s.jtab[k] : R(0);
+    }
+};
+
+// Resolves a thread's atoms; false when it has none.  Atoms past the entry's end are replaced by
+// its first atom (their loads are harmless) and flagged, so nothing is stored for them.
+template <int U>
+__device__ __forceinline__ bool fused_atoms(const int4& b, int (&idx)[U], bool (&ok)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        idx[u] = b.x + (int)threadIdx.x + u * (int)blockDim.x;
+        ok[u] = idx[u] < b.z;
+        if (!ok[u]) idx[u] = b.x;
+    }
+    return ok[0];
+}
+
+// LDS of the fused kernels, filled once per workgroup by every thread before any returns:
+//   [material records [nmat]] [condition records of the workgroup's replica [nmat]]
+//   [stencil offsets [npat][W], when the table is small (s.pat_lds)]
+// Lanes then read their material, condition and row offsets from LDS (same-address reads are
+// broadcast) instead of through the vector memory path, which the neighbour gathers keep busy.
+extern __shared__ __align__(16) int vmx_lds[];
+template <typename R>
+struct FusedLds {
+    static constexpr int kMatInts = (int)(sizeof(MatConst<R>) / sizeof(int));
+    static constexpr int kCondInts = (int)(sizeof(Cond<R>) / sizeof(int));
+    static __device__ __forceinline__ int cond_off(const SysDev<R>& s) { return s.nmat * kMatInts; }
+    static __device__ __forceinline__ int pat_off(const SysDev<R>& s) {
+        return ((s.nmat * (kMatInts + kCondInts)) + 3) & ~3;   // 16-byte aligned for int4 reads
+    }
+    static __device__ __forceinline__ const MatConst<R>* mat() { return reinterpret_cast<const MatConst<R>*>(vmx_lds); }
+    static __device__ __forceinline__ const Cond<R>* cond(const SysDev<R>& s) {
+        return reinterpret_cast<const Cond<R>*>(vmx_lds + cond_off(s));
+    }
+};
+// bytes of dynamic LDS the fused kernels need (host and device agree on this layout)
+template <typename R>
+__host__ __device__ inline size_t fused_lds_bytes(int nmat, int npat, int zmax, int pat_lds) {
+    const size_t rec = (size_t)nmat * ((sizeof(MatConst<R>) + sizeof(Cond<R>)) / sizeof(int));
+    const size_t pat = pat_lds ? (size_t)npat * (size_t)zmax : 0;
+    return (((rec + 3) & ~(size_t)3) + pat) * sizeof(int);
+}
+template <typename R, int AM>
+__device__ __forceinline__ void stage_fused(const SysDev<R>& s, int rep) {
+    using L = FusedLds<R>;
+    const int* __restrict__ mat = reinterpret_cast<const int*>(s.mat);
+    const int* __restrict__ cond = reinterpret_cast<const int*>(s.cond + rep * s.nmat);
+    const int nm = s.nmat * L::kMatInts, nc = s.nmat * L::kCondInts;
+    for (int k = (int)threadIdx.x; k < nm; k += (int)blockDim.x) vmx_lds[k] = mat[k];
+    for (int k = (int)threadIdx.x; k < nc; k += (int)blockDim.x) vmx_lds[nm + k] = cond[k];
+    if constexpr (Access<AM>::TP == TP_PAT) {
+        if (s.pat_lds) {
+            const int base = L::pat_off(s), total = s.npat * Access<AM>::W;
+            for (int k = (int)threadIdx.x; k < total; k += (int)blockDim.x) vmx_lds[base + k] = s.pat_off[k];
+        }
+    }
+    __syncthreads();
+}
+template <typename R, int AM>
+__device__ __forceinline__ void fused_row_index(const SysDev<R>& s, int i, int head, RowIdx<AM>& r) {
+    if (Access<AM>::TP == TP_PAT && s.pat_lds)
+        row_index(s, i, head, vmx_lds + FusedLds<R>::pat_off(s), r);
+    else
+        row_index(s, i, head, s.pat_off, r);
+}
+
 // Predictor: S' = |S + dt f(S, H_spin(S) + H_ext)|, written to P.  With D.x set the Euler
 // derivative f(S, H) (FP64) or A (FP32) is kept for the corrector, as the reference keeps its
 // euler arrays (src/simulate/LLGHeun.cpp:201-211).
@@ -423,27 +585,69 @@ template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(512)
 k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, const double* __restrict__ noise,
                   uint32_t step_off) {
-    int i, rep, local;
-    if (!my_atom(s, i, rep, local)) return;
+    constexpr int U = FusedAtoms<R>::value;
+    const int lb = logical_block(s.xcd_remap);
+    if (lb >= s.nblocks) return;                 // whole workgroup
+    const int4 b = s.blk[lb];
+    stage_fused<R, AM>(s, b.y);
+    int idx[U];
+    bool ok[U];
+    if (!fused_atoms<U>(b, idx, ok)) return;
+    const int rep = b.y;
     const uint64_t step = *s.step_base + step_off;
-    const int m = s.type[i];
-    const R sx = S.x[i], sy = S.y[i], sz = S.z[i];
-    R fx, fy, fz, ex, ey, ez;
-    spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m, sx, sy, sz, fx, fy, fz);
-    ext_field(s, rep, m, i, local, step, noise, ex, ey, ez);
-    const MatConst<R>& mc = s.mat[m];
-    R d0, d1, d2;
-    llg_rhs(mc.one_opa2, mc.alpha_opa2, sx, sy, sz, fx + ex, fy + ey, fz + ez, d0, d1, d2);
-    if (D.x) {
-        if constexpr (FoldEuler<R>::value) {
-            D.x[i] = sx + s.half_dt * d0; D.y[i] = sy + s.half_dt * d1; D.z[i] = sz + s.half_dt * d2;
-        } else {
-            D.x[i] = d0; D.y[i] = d1; D.z[i] = d2;
+    const SpinRsrc<R> rs(s, S.x, S.y, S.z);
+    int m[U];
+    R sx[U], sy[U], sz[U], fx[U], fy[U], fz[U];
+    if constexpr (Access<AM>::TP == TP_CSR) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            m[u] = s.type[idx[u]];
+            rs.load(idx[u], sx[u], sy[u], sz[u]);
+            spin_field<R, ET, AM>(s, S.x, S.y, S.z, idx[u], m[u], sx[u], sy[u], sz[u], fx[u], fy[u], fz[u]);
+        }
+    } else {
+        int head[U];
+        RowIdx<AM> ri[U];
+        RowSpins<R, AM> rv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            m[u] = s.type[idx[u]];
+            rs.load(idx[u], sx[u], sy[u], sz[u]);
+            head[u] = row_head<R, AM>(s, idx[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) fused_row_index(s, idx[u], head[u], ri[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) row_load(rs, ri[u], rv[u]);
+        const UniformJ<R, ET> ju(s, Access<AM>::JM);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const MatConst<R>& mc = FusedLds<R>::mat()[m[u]];
+            row_field<R, ET, AM>(s, ri[u], rv[u], ju.v, mc, sx[u], sy[u], sz[u], fx[u], fy[u], fz[u]);
         }
     }
-    R n0 = sx + d0 * s.dt, n1 = sy + d1 * s.dt, n2 = sz + d2 * s.dt;
-    const R r = inv_norm(n0, n1, n2);
-    P.x[i] = n0 * r; P.y[i] = n1 * r; P.z[i] = n2 * r;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int i = idx[u];
+        const MatConst<R>& mc = FusedLds<R>::mat()[m[u]];
+        const Cond<R>& c = FusedLds<R>::cond(s)[m[u]];
+        R ex, ey, ez;
+        ext_field(s, c, rep, i, caller_local(s, rep, i - b.w), step, noise, ex, ey, ez);
+        R d0, d1, d2;
+        llg_rhs(mc.one_opa2, mc.alpha_opa2, sx[u], sy[u], sz[u], fx[u] + ex, fy[u] + ey, fz[u] + ez, d0, d1, d2);
+        R n0 = sx[u] + d0 * s.dt, n1 = sy[u] + d1 * s.dt, n2 = sz[u] + d2 * s.dt;
+        const R r = inv_norm(n0, n1, n2);
+        if (ok[u]) {
+            if (D.x) {
+                if constexpr (FoldEuler<R>::value) {
+                    D.x[i] = sx[u] + s.half_dt * d0; D.y[i] = sy[u] + s.half_dt * d1; D.z[i] = sz[u] + s.half_dt * d2;
+                } else {
+                    D.x[i] = d0; D.y[i] = d1; D.z[i] = d2;
+                }
+            }
+            P.x[i] = n0 * r; P.y[i] = n1 * r; P.z[i] = n2 * r;
+        }
+    }
 }
 
 // Corrector: dS' at S' with the same H_ext (regenerated from the same Philox counter), then
@@ -453,39 +657,95 @@ template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(512)
 k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D,
                   const double* __restrict__ noise, uint32_t step_off) {
-    int i, rep, local;
-    if (!my_atom(s, i, rep, local)) return;
+    constexpr int U = FusedAtoms<R>::value;
+    constexpr bool kStaged = Access<AM>::TP != TP_CSR;
+    const int lb = logical_block(s.xcd_remap);
+    if (lb >= s.nblocks) return;                 // whole workgroup
+    const int4 b = s.blk[lb];
+    stage_fused<R, AM>(s, b.y);
+    int idx[U];
+    bool ok[U];
+    if (!fused_atoms<U>(b, idx, ok)) return;
+    const int rep = b.y;
     const uint64_t step = *s.step_base + step_off;
-    const int m = s.type[i];
-    const MatConst<R>& mc = s.mat[m];
-    R ex, ey, ez;
-    ext_field(s, rep, m, i, local, step, noise, ex, ey, ez);
-    const R px = P.x[i], py = P.y[i], pz = P.z[i];
-    R fx, fy, fz;
-    spin_field<R, ET, AM>(s, P.x, P.y, P.z, i, m, px, py, pz, fx, fy, fz);
-    R h0, h1, h2;
-    llg_rhs(mc.one_opa2, mc.alpha_opa2, px, py, pz, fx + ex, fy + ey, fz + ez, h0, h1, h2);
-    R n0, n1, n2;
-    if (FoldEuler<R>::value && D.x) {
-        n0 = D.x[i] + s.half_dt * h0;
-        n1 = D.y[i] + s.half_dt * h1;
-        n2 = D.z[i] + s.half_dt * h2;
-    } else {
-        const R sx = S.x[i], sy = S.y[i], sz = S.z[i];
-        R e0, e1, e2;
-        if (D.x) {
-            e0 = D.x[i]; e1 = D.y[i]; e2 = D.z[i];
+    const bool keep = D.x != nullptr;
+    const bool fold = FoldEuler<R>::value && keep;
+    const SpinRsrc<R> rp(s, P.x, P.y, P.z);
+    int m[U];
+    R px[U], py[U], pz[U], fx[U], fy[U], fz[U];
+    R ax[U], ay[U], az[U];          // A (folded) or S
+    R dx[U], dy[U], dz[U];          // kept dS (FP64)
+    int head[U];
+    RowIdx<AM> ri[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int i = idx[u];
+        m[u] = s.type[i];
+        rp.load(i, px[u], py[u], pz[u]);
+        if (kStaged) head[u] = row_head<R, AM>(s, i);
+        if (fold) {
+            ax[u] = D.x[i]; ay[u] = D.y[i]; az[u] = D.z[i];
         } else {
-            R gx, gy, gz;
-            spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m, sx, sy, sz, gx, gy, gz);
-            llg_rhs(mc.one_opa2, mc.alpha_opa2, sx, sy, sz, gx + ex, gy + ey, gz + ez, e0, e1, e2);
+            ax[u] = S.x[i]; ay[u] = S.y[i]; az[u] = S.z[i];
+            if (keep) { dx[u] = D.x[i]; dy[u] = D.y[i]; dz[u] = D.z[i]; }
         }
-        n0 = sx + s.half_dt * (e0 + h0);
-        n1 = sy + s.half_dt * (e1 + h1);
-        n2 = sz + s.half_dt * (e2 + h2);
     }
-    const R r = inv_norm(n0, n1, n2);
-    O.x[i] = n0 * r; O.y[i] = n1 * r; O.z[i] = n2 * r;
+    const UniformJ<R, ET> ju(s, Access<AM>::JM);
+    if constexpr (kStaged) {
+        RowSpins<R, AM> rv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) fused_row_index(s, idx[u], head[u], ri[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) row_load(rp, ri[u], rv[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const MatConst<R>& mc = FusedLds<R>::mat()[m[u]];
+            row_field<R, ET, AM>(s, ri[u], rv[u], ju.v, mc, px[u], py[u], pz[u], fx[u], fy[u], fz[u]);
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            spin_field<R, ET, AM>(s, P.x, P.y, P.z, idx[u], m[u], px[u], py[u], pz[u], fx[u], fy[u], fz[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int i = idx[u];
+        const MatConst<R>& mc = FusedLds<R>::mat()[m[u]];
+        const Cond<R>& c = FusedLds<R>::cond(s)[m[u]];
+        R ex, ey, ez;
+        ext_field(s, c, rep, i, caller_local(s, rep, i - b.w), step, noise, ex, ey, ez);
+        R h0, h1, h2;
+        llg_rhs(mc.one_opa2, mc.alpha_opa2, px[u], py[u], pz[u], fx[u] + ex, fy[u] + ey, fz[u] + ez, h0, h1, h2);
+        R n0, n1, n2;
+        if (fold) {
+            n0 = ax[u] + s.half_dt * h0;
+            n1 = ay[u] + s.half_dt * h1;
+            n2 = az[u] + s.half_dt * h2;
+        } else {
+            R e0, e1, e2;
+            if (keep) {
+                e0 = dx[u]; e1 = dy[u]; e2 = dz[u];
+            } else {
+                R gx, gy, gz;
+                if constexpr (kStaged) {
+                    const SpinRsrc<R> rs(s, S.x, S.y, S.z);
+                    RowSpins<R, AM> sv;
+                    row_load(rs, ri[u], sv);
+                    row_field<R, ET, AM>(s, ri[u], sv, ju.v, mc, ax[u], ay[u], az[u], gx, gy, gz);
+                } else {
+                    spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m[u], ax[u], ay[u], az[u], gx, gy, gz);
+                }
+                llg_rhs(mc.one_opa2, mc.alpha_opa2, ax[u], ay[u], az[u], gx + ex, gy + ey, gz + ez, e0, e1, e2);
+            }
+            n0 = ax[u] + s.half_dt * (e0 + h0);
+            n1 = ay[u] + s.half_dt * (e1 + h1);
+            n2 = az[u] + s.half_dt * (e2 + h2);
+        }
+        const R r = inv_norm(n0, n1, n2);
+        if (ok[u]) {
+            O.x[i] = n0 * r; O.y[i] = n1 * r; O.z[i] = n2 * r;
+        }
+    }
 }
 
 // ======================= plain kernels: the reference dataflow ==============================
