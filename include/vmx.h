@@ -60,8 +60,10 @@ enum {
     VMX_KERNEL_PLAIN = 0,  /* five kernels per Heun step with field, noise and dS arrays in
                               HBM, the dataflow of the reference step (src/simulate/LLGHeun.cpp
                               :160-282, src/cuda/llg_heun.cu:121-208)                        */
-    VMX_KERNEL_FUSED = 1   /* two kernels per step: predictor, corrector; fields, noise and dS
-                              are recomputed in registers instead of stored                  */
+    VMX_KERNEL_FUSED = 1   /* two kernels per step: predictor, corrector; spin fields live in
+                              registers only; dS and the external field pass from predictor to
+                              corrector through HBM as options choose (recompute_euler,
+                              keep_external_field); the corrector recomputes the rest        */
 };
 
 /* ---- anisotropy terms (flags of anisotropy::internal, src/anisotropy/fields.cpp:31-68) ----- */
@@ -126,6 +128,10 @@ typedef struct vmx_options {
                              corrector recomputes it (one neighbour gather more, 6 fewer reals
                              of memory traffic per atom).  Results are identical.              */
     int32_t atom_order;   /* VMX_ATOM_ORDER_*: how the device stores atoms                     */
+    int32_t keep_external_field; /* fused kernels: 0 = the corrector regenerates the external
+                             (thermal + applied) field from the same Philox counter; 1 = the
+                             predictor keeps it for the corrector, as the reference keeps its
+                             field arrays (6 more reals of traffic per atom).  Same results.   */
 } vmx_options;
 
 /* Atom storage order on the device.  Invisible to the caller: every array passed in or out is in

@@ -17,14 +17,15 @@ GOLDEN_CASES = ["sc6_uniaxial", "fcc3_cubic", "bcc4_rotated", "film_dmi"]
 TOPOLOGIES = ["csr", "ell", "stencil"]
 
 
-# "fused_recompute": the fused step with the corrector recomputing dS instead of reading it back
-KERNELS = [PLAIN, FUSED, "fused_recompute"]
+# Fused-step variants: "fused" keeps dS for the corrector and regenerates H_ext there (the
+# default); "fused_keep" also keeps H_ext; "fused_recompute" keeps nothing (dS recomputed too).
+KERNELS = [PLAIN, FUSED, "fused_keep", "fused_recompute"]
 
 
 def _ctx(system, precision=FP64, kernel=FUSED, seed=0, block=0, topology="auto"):
-    recompute = kernel == "fused_recompute"
-    return Context(system, precision=precision, kernel=FUSED if recompute else kernel, seed=seed,
-                   block_size=block, topology=topology, recompute_euler=recompute)
+    return Context(system, precision=precision, kernel=PLAIN if kernel == PLAIN else FUSED, seed=seed,
+                   block_size=block, topology=topology, recompute_euler=kernel == "fused_recompute",
+                   keep_external_field=kernel == "fused_keep")
 
 
 @pytest.mark.parametrize("topology", TOPOLOGIES)

@@ -91,6 +91,7 @@ class vmx_options(C.Structure):
         ("topology", C.c_int32),
         ("recompute_euler", C.c_int32),
         ("atom_order", C.c_int32),
+        ("keep_external_field", C.c_int32),
     ]
 
 

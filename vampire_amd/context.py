@@ -45,7 +45,7 @@ class Context:
 
     def __init__(self, system: System, precision: str = FP64, kernel: str = FUSED, block_size: int = 0,
                  seed: int = 0, device: int = -1, topology: str = "auto", recompute_euler: bool = False,
-                 atom_order: str = "auto"):
+                 atom_order: str = "auto", keep_external_field: bool = False):
         self.lib = L.load()
         self.system = system
         self.n = system.n
@@ -70,7 +70,8 @@ class Context:
                             block_size=block_size, device=device, seed=seed,
                             topology=TOPOLOGIES[topology],
                             recompute_euler=int(bool(recompute_euler)),
-                            atom_order={"auto": 0, "caller": 1}[atom_order])
+                            atom_order={"auto": 0, "caller": 1}[atom_order],
+                            keep_external_field=int(bool(keep_external_field)))
         h = C.c_void_p()
         L.check(self.lib.vmx_create(C.byref(sysc), C.byref(opt), C.byref(h)))
         self.h = h

@@ -257,8 +257,10 @@ struct vmx_ctx {
     DevBuf spin[3];               // three buffers of 3N R each (S, P, O roles rotate)
     int s_idx = 0, o_idx = 2;     // current spin buffer and the corrector's output buffer
     DevBuf plainF, plainE, plainD;
-    int store_euler = 1;          // fused step: predictor keeps dS for the corrector
+    int store_euler = 1;          // fused step: predictor keeps dS (FP64) or S + dt/2 dS (FP32)
     DevBuf euler_buf;             // 3N R
+    int store_field = 0;          // fused step: predictor keeps H_ext for the corrector
+    DevBuf field_buf;             // 3N R
     DevBuf mm, atom_rep;
     DevBuf staging;               // 3N doubles for host<->device spin copies and noise
     DevBuf noise_dev;             // 3N doubles
@@ -318,6 +320,12 @@ struct vmx_ctx {
     Spin3<R> euler() const {
         if (!store_euler) return Spin3<R>{nullptr, nullptr, nullptr};
         return arr3<R>(euler_buf);
+    }
+    // kept external-field arrays, or null arrays when the corrector regenerates the field
+    template <typename R>
+    Spin3<R> field() const {
+        if (!store_field) return Spin3<R>{nullptr, nullptr, nullptr};
+        return arr3<R>(field_buf);
     }
     template <typename R>
     Spin3<R> arr3(const DevBuf& b) const {
@@ -545,11 +553,11 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
         int nk = 0;
         if (kernel == VMX_KERNEL_FUSED) {
             VMX_HIP(mark(0));
-            VMX_HIP(Ops<R>::fused_predictor(Lf, et, am(), sf, spins<R>(s_idx), spins<R>(p_idx), euler<R>(), nd,
-                                            (uint32_t)k));
+            VMX_HIP(Ops<R>::fused_predictor(Lf, et, am(), sf, spins<R>(s_idx), spins<R>(p_idx), euler<R>(),
+                                            field<R>(), nd, (uint32_t)k));
             VMX_HIP(mark(1));
             VMX_HIP(Ops<R>::fused_corrector(Lf, et, am(), sf, spins<R>(s_idx), spins<R>(p_idx), spins<R>(o_idx),
-                                            euler<R>(), nd, (uint32_t)k));
+                                            euler<R>(), field<R>(), nd, (uint32_t)k));
             VMX_HIP(mark(2));
             nk = 2;
             std::swap(s_idx, o_idx);
@@ -598,11 +606,11 @@ int vmx_ctx::capture_graph(int nsteps) {
     int si = s_idx, oi = o_idx;
     for (int k = 0; k < nsteps; ++k) {
         const int pi = 3 - si - oi;
-        hipError_t e = Ops<R>::fused_predictor(L, et, am(), s, spins<R>(si), spins<R>(pi), euler<R>(), nullptr,
-                                               (uint32_t)k);
+        hipError_t e = Ops<R>::fused_predictor(L, et, am(), s, spins<R>(si), spins<R>(pi), euler<R>(), field<R>(),
+                                               nullptr, (uint32_t)k);
         if (e == hipSuccess)
-            e = Ops<R>::fused_corrector(L, et, am(), s, spins<R>(si), spins<R>(pi), spins<R>(oi), euler<R>(), nullptr,
-                                        (uint32_t)k);
+            e = Ops<R>::fused_corrector(L, et, am(), s, spins<R>(si), spins<R>(pi), spins<R>(oi), euler<R>(),
+                                        field<R>(), nullptr, (uint32_t)k);
         if (e != hipSuccess) {
             hipGraph_t g;
             (void)hipStreamEndCapture(stream, &g);
@@ -1288,7 +1296,9 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
     }
     VMX_TRY(c->staging.alloc(3 * (size_t)n * sizeof(double)));
     c->store_euler = opt->recompute_euler ? 0 : 1;
-    if (c->kernel == VMX_KERNEL_FUSED) VMX_TRY(c->euler_buf.alloc(3 * (size_t)n * rs));
+    c->store_field = opt->keep_external_field ? 1 : 0;
+    if (c->kernel == VMX_KERNEL_FUSED && c->store_euler) VMX_TRY(c->euler_buf.alloc(3 * (size_t)n * rs));
+    if (c->kernel == VMX_KERNEL_FUSED && c->store_field) VMX_TRY(c->field_buf.alloc(3 * (size_t)n * rs));
     const size_t cond_bytes = (size_t)nrep * c->nmat * sizeof(Cond<double>) + 64;
     VMX_TRY(c->ring.init(std::max<size_t>(cond_bytes, 8 * (size_t)nrep + 64)));
     VMX_TRY(c->upload_conditions_any());

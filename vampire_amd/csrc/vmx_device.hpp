@@ -580,10 +580,11 @@ __device__ __forceinline__ void fused_row_index(const SysDev<R>& s, int i, int h
 
 // Predictor: S' = |S + dt f(S, H_spin(S) + H_ext)|, written to P.  With D.x set the Euler
 // derivative f(S, H) (FP64) or A (FP32) is kept for the corrector, as the reference keeps its
-// euler arrays (src/simulate/LLGHeun.cpp:201-211).
+// euler arrays (src/simulate/LLGHeun.cpp:201-211); with E.x set H_ext is kept too, as the
+// reference keeps its external-field arrays for both stages (LLGHeun.cpp:187-197, 228-237).
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(512)
-k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, const double* __restrict__ noise,
+k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, Spin3<R> E, const double* __restrict__ noise,
                   uint32_t step_off) {
     constexpr int U = FusedAtoms<R>::value;
     const int lb = logical_block(s.xcd_remap);
@@ -645,17 +646,21 @@ k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, const double*
                     D.x[i] = d0; D.y[i] = d1; D.z[i] = d2;
                 }
             }
+            if (E.x) {
+                E.x[i] = ex; E.y[i] = ey; E.z[i] = ez;
+            }
             P.x[i] = n0 * r; P.y[i] = n1 * r; P.z[i] = n2 * r;
         }
     }
 }
 
-// Corrector: dS' at S' with the same H_ext (regenerated from the same Philox counter), then
-// S_next = |S + dt/2 (dS + dS')| (LLGHeun.cpp:257-269) with dS read back when the predictor
-// kept it, else recomputed; FP32 with a kept A forms |A + dt/2 dS'|.
+// Corrector: dS' at S' with the same H_ext (read back when the predictor kept it, else
+// regenerated from the same Philox counter), then S_next = |S + dt/2 (dS + dS')|
+// (LLGHeun.cpp:257-269) with dS read back when the predictor kept it, else recomputed; FP32
+// with a kept A forms |A + dt/2 dS'|.
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(512)
-k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D,
+k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D, Spin3<R> E,
                   const double* __restrict__ noise, uint32_t step_off) {
     constexpr int U = FusedAtoms<R>::value;
     constexpr bool kStaged = Access<AM>::TP != TP_CSR;
@@ -675,6 +680,7 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D,
     R px[U], py[U], pz[U], fx[U], fy[U], fz[U];
     R ax[U], ay[U], az[U];          // A (folded) or S
     R dx[U], dy[U], dz[U];          // kept dS (FP64)
+    R hx[U], hy[U], hz[U];          // kept H_ext
     int head[U];
     RowIdx<AM> ri[U];
 #pragma unroll
@@ -689,6 +695,7 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D,
             ax[u] = S.x[i]; ay[u] = S.y[i]; az[u] = S.z[i];
             if (keep) { dx[u] = D.x[i]; dy[u] = D.y[i]; dz[u] = D.z[i]; }
         }
+        if (E.x) { hx[u] = E.x[i]; hy[u] = E.y[i]; hz[u] = E.z[i]; }
     }
     const UniformJ<R, ET> ju(s, Access<AM>::JM);
     if constexpr (kStaged) {
@@ -711,9 +718,13 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D,
     for (int u = 0; u < U; ++u) {
         const int i = idx[u];
         const MatConst<R>& mc = FusedLds<R>::mat()[m[u]];
-        const Cond<R>& c = FusedLds<R>::cond(s)[m[u]];
         R ex, ey, ez;
-        ext_field(s, c, rep, i, caller_local(s, rep, i - b.w), step, noise, ex, ey, ez);
+        if (E.x) {
+            ex = hx[u]; ey = hy[u]; ez = hz[u];
+        } else {
+            const Cond<R>& c = FusedLds<R>::cond(s)[m[u]];
+            ext_field(s, c, rep, i, caller_local(s, rep, i - b.w), step, noise, ex, ey, ez);
+        }
         R h0, h1, h2;
         llg_rhs(mc.one_opa2, mc.alpha_opa2, px[u], py[u], pz[u], fx[u] + ex, fy[u] + ey, fz[u] + ez, h0, h1, h2);
         R n0, n1, n2;

@@ -20,12 +20,12 @@ template <typename R>
 struct Ops {
     // fused Heun step: predictor writes P from S; corrector writes O from S and P.  The Philox
     // step of a launch is *s.step_base + step_off.
-    // E: external-field arrays written by the predictor and read by the corrector, or all-null
-    // to regenerate the field in the corrector.
+    // D: Euler derivative (FP64) or S + dt/2 dS (FP32) kept by the predictor for the corrector;
+    // E: external field kept the same way.  All-null arrays make the corrector recompute them.
     static hipError_t fused_predictor(const Launch& L, int et, int jm, const SysDev<R>& s, Spin3<R> S,
-                                      Spin3<R> P, Spin3<R> E, const double* noise, uint32_t step_off);
+                                      Spin3<R> P, Spin3<R> D, Spin3<R> E, const double* noise, uint32_t step_off);
     static hipError_t fused_corrector(const Launch& L, int et, int jm, const SysDev<R>& s, Spin3<R> S,
-                                      Spin3<R> P, Spin3<R> O, Spin3<R> E, const double* noise,
+                                      Spin3<R> P, Spin3<R> O, Spin3<R> D, Spin3<R> E, const double* noise,
                                       uint32_t step_off);
     // plain Heun step
     static hipError_t spin_fields(const Launch& L, int et, int jm, const SysDev<R>& s, Spin3<R> S, Spin3<R> F);
