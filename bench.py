@@ -2,10 +2,13 @@
 FP32 fused kernels (the headline value) beside FP64 fused and the plain FP64 kernels, with the
 HBM roofline of the Heun step and the CPU restatement of the reference timed on the host.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--L 100] [--cpu-steps 3] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--L 100] [--cpu-seconds 10] [--no-cpu]
+                    [--batching-steps 200] [--legs fp32_fused,fp64_fused,fp64_plain]
 
 One "step" is one Heun step (predictor + corrector) over all 4,000,000 spins, inputs resident
-in HBM.  Prints one JSON line.
+in HBM.  Also reported: the FP32-vs-FP64 drift of the magnetisation, the replica-batching leg
+(256 SC replicas in one context against one context per replica) and the oracle timed on the
+host cores.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -53,6 +56,84 @@ def run_gpu(system, precision, kernel, steps, warmup, seed=7, T=600.0, use_graph
     wall = time.perf_counter() - t0
     kernel_ms = c.time_kernels(20)
     return c, wall, kernel_ms
+
+
+def batching_leg(steps: int, graph_chunk: int = 50):
+    """Config C2: 256 SC replicas (8-48 cells per edge, 8.67M atoms, each at its own temperature in
+    [0, 1.5 Tc], alpha = 1, FP32).  One context integrating every replica in the same launches
+    (eager and HIP-graph replay) against one context per replica run one after the other, and the
+    checkerboard Metropolis sweep the same two ways.  Spin-steps/s over all replicas."""
+    from vampire_amd import system as S
+    from vampire_amd.context import FP32, FUSED, Context
+    reps, temps = S.config_c2()
+    total = sum(r.n for r in reps)
+    batched = S.concat_systems(reps)
+    out = {"workload": f"{len(reps)} SC replicas, L in [8,48], {total} atoms, T uniform in [0, 1.5 Tc], "
+                       "alpha=1, FP32 fused LLG-Heun; checkerboard Metropolis",
+           "replicas": len(reps), "atoms": total, "steps": steps}
+
+    def up(n):
+        sp = np.zeros((3, n))
+        sp[2] = 1.0
+        return sp
+
+    c = Context(batched, precision=FP32, kernel=FUSED, seed=7, device=0)
+    for r, T in enumerate(temps):
+        c.set_conditions(T, replica=r)
+    c.set_spins(up(total))
+    c.llg_heun(20)
+    c.synchronize()
+    t0 = time.perf_counter()
+    c.llg_heun(steps)
+    c.synchronize()
+    out["batched_sps"] = total * steps / (time.perf_counter() - t0)
+    c.llg_heun_graph(graph_chunk, 1)
+    c.synchronize()
+    t0 = time.perf_counter()
+    c.llg_heun_graph(graph_chunk, steps // graph_chunk)
+    c.synchronize()
+    out["batched_graph_sps"] = total * graph_chunk * (steps // graph_chunk) / (time.perf_counter() - t0)
+    c.mc_sweep(2)
+    c.synchronize()
+    t0 = time.perf_counter()
+    c.mc_sweep(steps)
+    c.synchronize()
+    out["mc_batched_sps"] = total * steps / (time.perf_counter() - t0)
+    c.close()
+
+    ctxs = []
+    for r, (s_r, T) in enumerate(zip(reps, temps)):
+        cr = Context(s_r, precision=FP32, kernel=FUSED, seed=7 + r, device=0)
+        cr.set_conditions(T)
+        cr.set_spins(up(s_r.n))
+        cr.llg_heun(2)
+        cr.mc_sweep(1)
+        cr.synchronize()
+        ctxs.append(cr)
+    t0 = time.perf_counter()
+    for cr in ctxs:                 # one replica at a time, its launches back to back
+        cr.llg_heun(steps)
+        cr.synchronize()
+    out["sequential_sps"] = total * steps / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for cr in ctxs:
+        cr.mc_sweep(steps)
+        cr.synchronize()
+    out["mc_sequential_sps"] = total * steps / (time.perf_counter() - t0)
+    for cr in ctxs:
+        cr.close()
+    out["speedup"] = round(max(out["batched_sps"], out["batched_graph_sps"]) / out["sequential_sps"], 2)
+    out["mc_speedup"] = round(out["mc_batched_sps"] / out["mc_sequential_sps"], 2)
+    for k in ("batched_sps", "batched_graph_sps", "sequential_sps", "mc_batched_sps", "mc_sequential_sps"):
+        out[k] = round(out[k], 1)
+    return out
+
+
+def cpu_sample(system, threads, seconds):
+    """Oracle Heun steps on the host until the sample lasts `seconds`: (spin-steps/s, steps)."""
+    one = cpu_baseline(system, 1, threads)
+    steps = max(1, int(round(seconds * one / system.n)))
+    return cpu_baseline(system, steps, threads), steps
 
 
 def cpu_baseline(system, nsteps, threads):
@@ -107,10 +188,12 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--L", type=int, default=100, help="FCC unit cells per edge (100 -> 4M atoms)")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU work per thread count: steps are added until the sample lasts this long")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay the fused step as a HIP graph")
     ap.add_argument("--drift-steps", type=int, default=1000)
+    ap.add_argument("--batching-steps", type=int, default=200, help="0 skips the replica-batching leg")
     ap.add_argument("--legs", default="fp32_fused,fp64_fused,fp64_plain",
                     help="comma list of fp32_fused, fp64_fused, fp64_plain (profiling runs use one)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -176,14 +259,19 @@ def main():
                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(n),
                         kernel="fused predictor + corrector (one Heun step)",
                         bytes_per_spin_step=p32, kernel_ms=head["kernel_ms"])
+    batching = None
+    if args.batching_steps:
+        batching = batching_leg(args.batching_steps)
+        print(f"# batching: {batching}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
-        v1 = cpu_baseline(system, args.cpu_steps, 1)
-        vN = cpu_baseline(system, args.cpu_steps, nthreads)
+        (v1, k1), (vN, kN) = (cpu_sample(system, 1, args.cpu_seconds),
+                              cpu_sample(system, nthreads, args.cpu_seconds))
         cpu = dict(value=round(vN, 1), unit="spin-steps/s", cores=nthreads, kind="port",
-                   sample=f"{args.cpu_steps} Heun steps of the same 4M-atom FCC system, FP64, OpenMP over "
-                          f"{nthreads} threads (1 thread: {v1:.4g} spin-steps/s)",
+                   sample=f"{kN} Heun steps of the same 4M-atom FCC system (FP64 oracle, OpenMP over {nthreads} "
+                          f"threads, >= {args.cpu_seconds:.0f} s); 1 thread: {k1} steps, {v1:.4g} spin-steps/s",
                    value_1thread=round(v1, 1))
         print(f"# cpu: 1 thread {v1:.3e}, {nthreads} threads {vN:.3e} spin-steps/s", file=sys.stderr, flush=True)
 
@@ -217,6 +305,7 @@ def main():
         "fp32_over_fp64_plain": (round(head["sps"] / res["fp64_plain"]["sps"], 3)
                                  if head is not None and "fp64_plain" in res else None),
         "fp32_vs_fp64_drift": drift,
+        "batching": batching,
     }
     print(json.dumps(out), flush=True)
 

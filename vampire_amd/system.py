@@ -439,6 +439,22 @@ def config_c1(L: int = 100) -> System:
     return ferromagnet("fcc", L, alpha=0.05)
 
 
+# Curie temperature of the classical Heisenberg simple-cubic ferromagnet with the Co-like C0
+# parameters: k_B Tc = 1.443 J (high-temperature series), J = 6.064e-21 J -> Tc = 634 K.
+TC_SC_CO = 1.443 * J_CO / K_B
+
+
+def config_c2(n_replicas: int = 256, seed: int = 11, L_min: int = 8, L_max: int = 48):
+    """Batch of periodic SC ferromagnets with the C0 parameters and alpha = 1: edge lengths
+    uniform in [L_min, L_max] cells, temperatures uniform over [0, 1.5 Tc].  Returns the list of
+    replica systems and their temperatures (K); concat_systems() gives the batched system."""
+    rng = np.random.default_rng(seed)
+    Ls = rng.integers(L_min, L_max + 1, size=n_replicas)
+    temps = rng.uniform(0.0, 1.5 * TC_SC_CO, size=n_replicas)
+    systems = [ferromagnet("sc", int(L), alpha=1.0) for L in Ls]
+    return systems, [float(t) for t in temps]
+
+
 def config_c3(L: int = 32) -> System:
     """BCC 32^3 unit cells (65,536 atoms), deterministic (T=0) fitting workload."""
     return ferromagnet("bcc", L, alpha=0.1)
