@@ -498,12 +498,12 @@ __device__ __forceinline__ float inv_norm(float x, float y, float z) {
 template <typename R> struct FoldEuler { static constexpr bool value = false; };
 template <> struct FoldEuler<float> { static constexpr bool value = true; };
 
-// Atoms per thread of the fused kernels: a workgroup of B threads takes kFusedAtoms * B
-// consecutive atoms of its block-table entry (thread t: t, t + B, ...), and each memory phase
-// (own data and stencil id, row indices, neighbour spins) is issued for all of a thread's atoms
-// before any of them is waited for.
+// Atoms per thread of the fused kernels: a workgroup of B threads takes FusedAtoms * B
+// consecutive atoms of its block-table entry (thread t: t, t + B, ...).  The loads of every
+// atom's own data are issued before the first atom is processed, so each wavefront keeps
+// FusedAtoms times more HBM reads in flight.
 template <typename R> struct FusedAtoms { static constexpr int value = 1; };
-template <> struct FusedAtoms<float> { static constexpr int value = 1; };
+template <> struct FusedAtoms<float> { static constexpr int value = 2; };
 
 template <typename R, int ET>
 struct UniformJ {            // the exchange reals when every edge carries the same J
@@ -516,7 +516,8 @@ struct UniformJ {            // the exchange reals when every edge carries the s
 };
 
 // Resolves a thread's atoms; false when it has none.  Atoms past the entry's end are replaced by
-// its first atom (their loads are harmless) and flagged, so nothing is stored for them.
+// its first atom (their loads are harmless) and flagged, so nothing is stored for them.  Every
+// thread, with atoms or not, goes on to the workgroup's LDS staging barrier.
 template <int U>
 __device__ __forceinline__ bool fused_atoms(const int4& b, int (&idx)[U], bool (&ok)[U]) {
 #pragma unroll
@@ -582,60 +583,54 @@ __device__ __forceinline__ void fused_row_index(const SysDev<R>& s, int i, int h
 // derivative f(S, H) (FP64) or A (FP32) is kept for the corrector, as the reference keeps its
 // euler arrays (src/simulate/LLGHeun.cpp:201-211); with E.x set H_ext is kept too, as the
 // reference keeps its external-field arrays for both stages (LLGHeun.cpp:187-197, 228-237).
+// A thread first issues the loads of its atoms' own data (spin, material, stencil id), which
+// come from HBM, then takes its atoms one at a time through row indices, neighbour gathers (mostly
+// L2 hits) and the step.
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(512)
 k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, Spin3<R> E, const double* __restrict__ noise,
                   uint32_t step_off) {
     constexpr int U = FusedAtoms<R>::value;
+    constexpr bool kStaged = Access<AM>::TP != TP_CSR;
     const int lb = logical_block(s.xcd_remap);
     if (lb >= s.nblocks) return;                 // whole workgroup
     const int4 b = s.blk[lb];
-    stage_fused<R, AM>(s, b.y);
     int idx[U];
     bool ok[U];
-    if (!fused_atoms<U>(b, idx, ok)) return;
+    fused_atoms<U>(b, idx, ok);
     const int rep = b.y;
-    const uint64_t step = *s.step_base + step_off;
     const SpinRsrc<R> rs(s, S.x, S.y, S.z);
-    int m[U];
-    R sx[U], sy[U], sz[U], fx[U], fy[U], fz[U];
-    if constexpr (Access<AM>::TP == TP_CSR) {
+    int m[U], head[U];
+    R sx[U], sy[U], sz[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            m[u] = s.type[idx[u]];
-            rs.load(idx[u], sx[u], sy[u], sz[u]);
-            spin_field<R, ET, AM>(s, S.x, S.y, S.z, idx[u], m[u], sx[u], sy[u], sz[u], fx[u], fy[u], fz[u]);
-        }
-    } else {
-        int head[U];
-        RowIdx<AM> ri[U];
-        RowSpins<R, AM> rv[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            m[u] = s.type[idx[u]];
-            rs.load(idx[u], sx[u], sy[u], sz[u]);
-            head[u] = row_head<R, AM>(s, idx[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) fused_row_index(s, idx[u], head[u], ri[u]);
-#pragma unroll
-        for (int u = 0; u < U; ++u) row_load(rs, ri[u], rv[u]);
-        const UniformJ<R, ET> ju(s, Access<AM>::JM);
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const MatConst<R>& mc = FusedLds<R>::mat()[m[u]];
-            row_field<R, ET, AM>(s, ri[u], rv[u], ju.v, mc, sx[u], sy[u], sz[u], fx[u], fy[u], fz[u]);
-        }
+    for (int u = 0; u < U; ++u) {   // own data from HBM, in flight while the LDS tables are staged
+        m[u] = s.type[idx[u]];
+        rs.load(idx[u], sx[u], sy[u], sz[u]);
+        if (kStaged) head[u] = row_head<R, AM>(s, idx[u]);
     }
+    stage_fused<R, AM>(s, rep);
+    if (!ok[0]) return;
+    const uint64_t step = *s.step_base + step_off;
+    const UniformJ<R, ET> ju(s, Access<AM>::JM);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int i = idx[u];
         const MatConst<R>& mc = FusedLds<R>::mat()[m[u]];
+        R fx, fy, fz;
+        if constexpr (kStaged) {
+            RowIdx<AM> ri;
+            RowSpins<R, AM> rv;
+            fused_row_index(s, i, head[u], ri);
+            row_load(rs, ri, rv);
+            row_field<R, ET, AM>(s, ri, rv, ju.v, mc, sx[u], sy[u], sz[u], fx, fy, fz);
+        } else {
+            spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m[u], sx[u], sy[u], sz[u], fx, fy, fz);
+        }
         const Cond<R>& c = FusedLds<R>::cond(s)[m[u]];
         R ex, ey, ez;
         ext_field(s, c, rep, i, caller_local(s, rep, i - b.w), step, noise, ex, ey, ez);
         R d0, d1, d2;
-        llg_rhs(mc.one_opa2, mc.alpha_opa2, sx[u], sy[u], sz[u], fx[u] + ex, fy[u] + ey, fz[u] + ez, d0, d1, d2);
+        llg_rhs(mc.one_opa2, mc.alpha_opa2, sx[u], sy[u], sz[u], fx + ex, fy + ey, fz + ez, d0, d1, d2);
         R n0 = sx[u] + d0 * s.dt, n1 = sy[u] + d1 * s.dt, n2 = sz[u] + d2 * s.dt;
         const R r = inv_norm(n0, n1, n2);
         if (ok[u]) {
@@ -657,7 +652,7 @@ k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, Spin3<R> E, c
 // Corrector: dS' at S' with the same H_ext (read back when the predictor kept it, else
 // regenerated from the same Philox counter), then S_next = |S + dt/2 (dS + dS')|
 // (LLGHeun.cpp:257-269) with dS read back when the predictor kept it, else recomputed; FP32
-// with a kept A forms |A + dt/2 dS'|.
+// with a kept A forms |A + dt/2 dS'|.  Own data of all the thread's atoms first, as above.
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(512)
 k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D, Spin3<R> E,
@@ -667,22 +662,18 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D, S
     const int lb = logical_block(s.xcd_remap);
     if (lb >= s.nblocks) return;                 // whole workgroup
     const int4 b = s.blk[lb];
-    stage_fused<R, AM>(s, b.y);
     int idx[U];
     bool ok[U];
-    if (!fused_atoms<U>(b, idx, ok)) return;
+    fused_atoms<U>(b, idx, ok);
     const int rep = b.y;
-    const uint64_t step = *s.step_base + step_off;
     const bool keep = D.x != nullptr;
     const bool fold = FoldEuler<R>::value && keep;
     const SpinRsrc<R> rp(s, P.x, P.y, P.z);
-    int m[U];
-    R px[U], py[U], pz[U], fx[U], fy[U], fz[U];
+    int m[U], head[U];
+    R px[U], py[U], pz[U];
     R ax[U], ay[U], az[U];          // A (folded) or S
     R dx[U], dy[U], dz[U];          // kept dS (FP64)
     R hx[U], hy[U], hz[U];          // kept H_ext
-    int head[U];
-    RowIdx<AM> ri[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int i = idx[u];
@@ -697,27 +688,24 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D, S
         }
         if (E.x) { hx[u] = E.x[i]; hy[u] = E.y[i]; hz[u] = E.z[i]; }
     }
+    stage_fused<R, AM>(s, rep);      // own data above is in flight meanwhile
+    if (!ok[0]) return;
+    const uint64_t step = *s.step_base + step_off;
     const UniformJ<R, ET> ju(s, Access<AM>::JM);
-    if constexpr (kStaged) {
-        RowSpins<R, AM> rv[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) fused_row_index(s, idx[u], head[u], ri[u]);
-#pragma unroll
-        for (int u = 0; u < U; ++u) row_load(rp, ri[u], rv[u]);
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const MatConst<R>& mc = FusedLds<R>::mat()[m[u]];
-            row_field<R, ET, AM>(s, ri[u], rv[u], ju.v, mc, px[u], py[u], pz[u], fx[u], fy[u], fz[u]);
-        }
-    } else {
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            spin_field<R, ET, AM>(s, P.x, P.y, P.z, idx[u], m[u], px[u], py[u], pz[u], fx[u], fy[u], fz[u]);
-    }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int i = idx[u];
         const MatConst<R>& mc = FusedLds<R>::mat()[m[u]];
+        RowIdx<AM> ri;
+        R fx, fy, fz;
+        if constexpr (kStaged) {
+            RowSpins<R, AM> rv;
+            fused_row_index(s, i, head[u], ri);
+            row_load(rp, ri, rv);
+            row_field<R, ET, AM>(s, ri, rv, ju.v, mc, px[u], py[u], pz[u], fx, fy, fz);
+        } else {
+            spin_field<R, ET, AM>(s, P.x, P.y, P.z, i, m[u], px[u], py[u], pz[u], fx, fy, fz);
+        }
         R ex, ey, ez;
         if (E.x) {
             ex = hx[u]; ey = hy[u]; ez = hz[u];
@@ -726,7 +714,7 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D, S
             ext_field(s, c, rep, i, caller_local(s, rep, i - b.w), step, noise, ex, ey, ez);
         }
         R h0, h1, h2;
-        llg_rhs(mc.one_opa2, mc.alpha_opa2, px[u], py[u], pz[u], fx[u] + ex, fy[u] + ey, fz[u] + ez, h0, h1, h2);
+        llg_rhs(mc.one_opa2, mc.alpha_opa2, px[u], py[u], pz[u], fx + ex, fy + ey, fz + ez, h0, h1, h2);
         R n0, n1, n2;
         if (fold) {
             n0 = ax[u] + s.half_dt * h0;
@@ -741,8 +729,8 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D, S
                 if constexpr (kStaged) {
                     const SpinRsrc<R> rs(s, S.x, S.y, S.z);
                     RowSpins<R, AM> sv;
-                    row_load(rs, ri[u], sv);
-                    row_field<R, ET, AM>(s, ri[u], sv, ju.v, mc, ax[u], ay[u], az[u], gx, gy, gz);
+                    row_load(rs, ri, sv);
+                    row_field<R, ET, AM>(s, ri, sv, ju.v, mc, ax[u], ay[u], az[u], gx, gy, gz);
                 } else {
                     spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m[u], ax[u], ay[u], az[u], gx, gy, gz);
                 }
