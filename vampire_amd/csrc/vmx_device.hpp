@@ -561,13 +561,19 @@ __device__ __forceinline__ void stage_fused(const SysDev<R>& s, int rep) {
     const int* __restrict__ mat = reinterpret_cast<const int*>(s.mat);
     const int* __restrict__ cond = reinterpret_cast<const int*>(s.cond + rep * s.nmat);
     const int nm = s.nmat * L::kMatInts, nc = s.nmat * L::kCondInts;
-    for (int k = (int)threadIdx.x; k < nm; k += (int)blockDim.x) vmx_lds[k] = mat[k];
-    for (int k = (int)threadIdx.x; k < nc; k += (int)blockDim.x) vmx_lds[nm + k] = cond[k];
-    if constexpr (Access<AM>::TP == TP_PAT) {
-        if (s.pat_lds) {
-            const int base = L::pat_off(s), total = s.npat * Access<AM>::W;
-            for (int k = (int)threadIdx.x; k < total; k += (int)blockDim.x) vmx_lds[base + k] = s.pat_off[k];
-        }
+    const int np = (Access<AM>::TP == TP_PAT && s.pat_lds) ? s.npat * Access<AM>::W : 0;
+    const int rec = nm + nc, base = L::pat_off(s), total = rec + np;
+    // one pass over the three tables with two loads in flight per thread: every wait for these
+    // loads also waits for the own-data loads issued before, so fewer rounds mean fewer waits
+    auto src = [&](int k) { return k < nm ? mat[k] : (k < rec ? cond[k - nm] : s.pat_off[k - rec]); };
+    auto dst = [&](int k) { return k < rec ? k : base + (k - rec); };
+    const int bd = (int)blockDim.x;
+    for (int k0 = (int)threadIdx.x; k0 < total; k0 += 2 * bd) {
+        const int k1 = k0 + bd;
+        const int v0 = src(k0);
+        const int v1 = k1 < total ? src(k1) : 0;
+        vmx_lds[dst(k0)] = v0;
+        if (k1 < total) vmx_lds[dst(k1)] = v1;
     }
     __syncthreads();
 }
