@@ -37,8 +37,11 @@ def run_gpu(system, precision, kernel, steps, warmup, seed=7, T=600.0, use_graph
     from vampire_amd.context import Context
     spins = np.zeros((3, system.n))
     spins[2] = 1.0
-    c = Context(system, precision=precision, kernel=kernel, seed=seed, block_size=block, device=0)
+    c = Context(system, precision=precision, kernel=kernel, seed=seed, block_size=max(block, 0), device=0)
     c.set_conditions(T)
+    if block < 0:                 # autotune block size and XCD mapping (outside the timed region)
+        c.set_spins(spins)
+        c.autotune(20)
     c.set_spins(spins)
     if warmup:
         c.llg_heun(warmup)
@@ -55,6 +58,7 @@ def run_gpu(system, precision, kernel, steps, warmup, seed=7, T=600.0, use_graph
     c.synchronize()
     wall = time.perf_counter() - t0
     kernel_ms = c.time_kernels(20)
+    c.bench_launch = c.launch_config
     return c, wall, kernel_ms
 
 
@@ -193,6 +197,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay the fused step as a HIP graph")
     ap.add_argument("--drift-steps", type=int, default=1000)
+    ap.add_argument("--block", type=int, default=-1, help="threads per workgroup; 0: library default, -1: autotune")
     ap.add_argument("--batching-steps", type=int, default=200, help="0 skips the replica-batching leg")
     ap.add_argument("--legs", default="fp32_fused,fp64_fused,fp64_plain",
                     help="comma list of fp32_fused, fp64_fused, fp64_plain (profiling runs use one)")
@@ -219,12 +224,14 @@ def main():
     res = {}
     for name in legs:
         prec, kern = table[name]
-        c, wall, kms = run_gpu(system, prec, kern, args.steps, args.warmup, use_graph=args.graph and kern == FUSED)
+        c, wall, kms = run_gpu(system, prec, kern, args.steps, args.warmup, use_graph=args.graph and kern == FUSED,
+                               block=args.block)
         res[name] = dict(wall=wall, kernel_ms=kms.tolist(), sps=n * args.steps / wall,
-                         ms_per_step=1e3 * wall / args.steps)
+                         ms_per_step=1e3 * wall / args.steps,
+                         launch={"block": c.bench_launch[0], "xcd_remap": c.bench_launch[1]})
         c.close()
         print(f"# {name}: {res[name]['ms_per_step']:.4f} ms/step, {res[name]['sps']:.3e} spin-steps/s, "
-              f"kernels {['%.4f' % v for v in kms]} ms", file=sys.stderr, flush=True)
+              f"kernels {['%.4f' % v for v in kms]} ms, launch {res[name]['launch']}", file=sys.stderr, flush=True)
     if args.copy_calibration:
         copy_calibration()
 
@@ -258,7 +265,7 @@ def main():
         roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(n),
                         kernel="fused predictor + corrector (one Heun step)",
-                        bytes_per_spin_step=p32, kernel_ms=head["kernel_ms"])
+                        bytes_per_spin_step=p32, kernel_ms=head["kernel_ms"], launch=head["launch"])
     batching = None
     if args.batching_steps:
         batching = batching_leg(args.batching_steps)
@@ -280,6 +287,7 @@ def main():
             return None
         r = res[name]
         return {"value": round(r["sps"], 1), "ms_per_step": round(r["ms_per_step"], 5), "kernel_ms": r["kernel_ms"],
+                "launch": r["launch"],
                 "hbm_frac_b_alg": round(b_alg(p, z) * n / (sum(r["kernel_ms"]) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
     out = {

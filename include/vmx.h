@@ -295,10 +295,16 @@ int vmx_fit_forward(vmx_ctx* ctx, const vmx_fit_params* p, const double* h, cons
  * checkpoints are recomputed forward, then swept backward (hand-written adjoint kernels). */
 int vmx_fit_backward(vmx_ctx* ctx, const double* dL_dmz, double* grad, double* dL_dh);
 
-/* Chooses the workgroup size (and for FP32 the dS-storage variant) by timing a few steps of
- * each candidate on this system; the result applies to later calls.  The spins and step
- * counter are restored afterwards.  Writes the chosen block size. */
+/* Chooses the workgroup size (64, 128, 256, 512) and whether workgroups are remapped to give
+ * each XCD one contiguous stretch of the system, by timing a few Heun steps of each candidate
+ * on this system; the result applies to later calls.  The spins and step counter are restored
+ * afterwards.  Writes the chosen block size.  (Replaces the fixed gpu::num_threads launch
+ * width of the reference backend, hdr/gpu.hpp:22.) */
 int vmx_autotune(vmx_ctx* ctx, int32_t steps_per_trial, int32_t* chosen_block_size);
+
+/* The launch configuration in use: threads per workgroup and 1 when workgroups are remapped
+ * per XCD.  No counterpart in the reference; reported by benchmarks next to their timings. */
+int vmx_get_launch_config(vmx_ctx* ctx, int32_t* block_size, int32_t* xcd_remap);
 
 #ifdef __cplusplus
 }

@@ -171,6 +171,13 @@ class Context:
         L.check(self.lib.vmx_time_kernels(self.h, int(nsteps), L.dptr(ms), C.byref(nk)))
         return ms[:nk.value].copy()
 
+    @property
+    def launch_config(self):
+        """(threads per workgroup, XCD remapping on) in use."""
+        b, x = C.c_int32(0), C.c_int32(0)
+        L.check(self.lib.vmx_get_launch_config(self.h, C.byref(b), C.byref(x)))
+        return int(b.value), bool(x.value)
+
     def autotune(self, steps_per_trial: int = 20) -> int:
         v = C.c_int32(0)
         L.check(self.lib.vmx_autotune(self.h, int(steps_per_trial), C.byref(v)))

@@ -1733,6 +1733,14 @@ int vmx_fit_backward(vmx_ctx* c, const double* dL_dmz, double* grad, double* dL_
                                     : c->fit_backward<float>(dL_dmz, grad, dL_dh);
 }
 
+int vmx_get_launch_config(vmx_ctx* c, int32_t* block_size, int32_t* xcd_remap) {
+    g_last_error.clear();
+    if (!c || !block_size || !xcd_remap) return fail(VMX_ERR_ARG, "null argument");
+    *block_size = c->block;
+    *xcd_remap = c->xcd_remap;
+    return VMX_OK;
+}
+
 int vmx_autotune(vmx_ctx* c, int32_t steps_per_trial, int32_t* chosen) {
     if (!c) return fail(VMX_ERR_ARG, "null context");
     if (steps_per_trial <= 0) return fail(VMX_ERR_ARG, "steps_per_trial must be positive");
