@@ -133,6 +133,89 @@ def batching_leg(steps: int, graph_chunk: int = 50):
     return out
 
 
+def configs_leg(c0_steps: int = 10000, c3_steps: int = 2000, c4_steps: int = 1000):
+    """The other workloads of the path, timed on the device:
+    C0  SC 20^3 (8000 atoms), 300 K, FP64 fused, HIP-graph replay; the oracle on one host thread
+        for 1000 steps beside it (the reference's own CPU-runnable case);
+    C3  BCC 32^3 (65,536 atoms), deterministic Heun under Hz = 1 T from a 30-degree tilt, 2000 steps,
+        checkpoint every 100: trajectory forward and hand-written adjoint backward, FP32 and FP64;
+    C4  ferrimagnetic SC film 256x256x8 with interfacial DMI (3x3 tensor exchange), 10% dilution
+        (472,014 atoms), FP32 fused, temperature set every step along a Gaussian 100 fs pulse
+        300 -> 1200 K (each step: vmx_set_conditions + one Heun step, as gpu::llg_heun does)."""
+    from vampire_amd import system as S
+    from vampire_amd.context import FP32, FP64, FUSED, Context
+    from vampire_amd.fit import Trajectory
+    out = {}
+
+    s0 = S.config_c0()
+    sp0 = S.random_spins(s0.n, 1)
+    c = Context(s0, precision=FP64, kernel=FUSED, seed=1, device=0)
+    c.set_conditions(300.0)
+    c.set_spins(sp0)
+    c.llg_heun_graph(100, 1)
+    c.synchronize()
+    t0 = time.perf_counter()
+    c.llg_heun_graph(100, c0_steps // 100)
+    c.synchronize()
+    dt = time.perf_counter() - t0
+    c.close()
+    from oracle import oracle as O
+    o = O.Oracle(s0)
+    o.set_conditions(300.0)
+    t1 = time.perf_counter()
+    o.llg_heun(sp0, 1000, seed=1, threads=1)
+    cpu = s0.n * 1000 / (time.perf_counter() - t1)
+    out["c0"] = {"workload": "SC 20^3, 8000 atoms, 300 K, FP64 fused, graph replay", "steps": c0_steps,
+                 "seconds": round(dt, 4), "spin_steps_per_s": round(s0.n * c0_steps / dt, 1),
+                 "cpu_1thread_spin_steps_per_s": round(cpu, 1)}
+
+    s3 = S.config_c3()
+    th = np.deg2rad(30.0)
+    sp3 = np.zeros((3, s3.n))
+    sp3[0], sp3[2] = np.sin(th), np.cos(th)
+    rng = np.random.default_rng(3)
+    h = 0.01 * rng.standard_normal((3, s3.n))
+    c3 = {"workload": "BCC 32^3, 65536 atoms, T=0, Hz=1 T, 2000 steps, checkpoint 100, adjoint in J, Ku, alpha, h",
+          "steps": c3_steps}
+    for prec in (FP32, FP64):
+        t = Trajectory(s3, sp3, nsteps=c3_steps, record_every=10, checkpoint_every=100, precision=prec,
+                       H=(0.0, 0.0, 1.0))
+        mz = t.forward(S.J_CO, S.KU_CO, 0.1, h)
+        t.backward(np.ones_like(mz))
+        ta = time.perf_counter()
+        mz = t.forward(S.J_CO * 1.1, S.KU_CO * 0.9, 0.12, h)
+        tb = time.perf_counter()
+        t.backward(np.linspace(-1.0, 1.0, mz.size))
+        tc = time.perf_counter()
+        t.close()
+        c3[prec] = {"forward_s": round(tb - ta, 4), "backward_s": round(tc - tb, 4),
+                    "forward_spin_steps_per_s": round(s3.n * c3_steps / (tb - ta), 1)}
+    out["c3"] = c3
+
+    s4 = S.config_c4()
+    c = Context(s4, precision=FP32, kernel=FUSED, seed=3, device=0)
+    c.set_spins(S.random_spins(s4.n, 3))
+    name, npat = c.topology
+    tau, t_peak = 100e-15, 0.5 * c4_steps * 1e-16
+    temps = [300.0 + 900.0 * np.exp(-((k * 1e-16 - t_peak) / tau) ** 2) for k in range(c4_steps)]
+    c.set_conditions(300.0)
+    c.llg_heun(20)
+    c.synchronize()
+    t0 = time.perf_counter()
+    for T in temps:
+        c.set_conditions(T)
+        c.llg_heun(1)
+    c.synchronize()
+    dt = time.perf_counter() - t0
+    k4 = c.time_kernels(20)
+    c.close()
+    out["c4"] = {"workload": "SC film 256x256x8, DMI tensor exchange, 10% dilution, 472014 atoms, FP32 fused, "
+                             "per-step Gaussian T pulse 300->1200 K", "atoms": s4.n, "steps": c4_steps,
+                 "topology": name, "stencils": npat, "seconds": round(dt, 4),
+                 "spin_steps_per_s": round(s4.n * c4_steps / dt, 1), "kernel_ms": k4.tolist()}
+    return out
+
+
 def cpu_sample(system, threads, seconds):
     """Oracle Heun steps on the host until the sample lasts `seconds`: (spin-steps/s, steps)."""
     one = cpu_baseline(system, 1, threads)
@@ -199,6 +282,7 @@ def main():
     ap.add_argument("--drift-steps", type=int, default=1000)
     ap.add_argument("--block", type=int, default=-1, help="threads per workgroup; 0: library default, -1: autotune")
     ap.add_argument("--batching-steps", type=int, default=200, help="0 skips the replica-batching leg")
+    ap.add_argument("--no-configs", action="store_true", help="skip the C0 / C3 / C4 leg")
     ap.add_argument("--legs", default="fp32_fused,fp64_fused,fp64_plain",
                     help="comma list of fp32_fused, fp64_fused, fp64_plain (profiling runs use one)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -271,6 +355,11 @@ def main():
         batching = batching_leg(args.batching_steps)
         print(f"# batching: {batching}", file=sys.stderr, flush=True)
 
+    configs = None
+    if not args.no_configs:
+        configs = configs_leg()
+        print(f"# configs: {configs}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
@@ -314,6 +403,7 @@ def main():
                                  if head is not None and "fp64_plain" in res else None),
         "fp32_vs_fp64_drift": drift,
         "batching": batching,
+        "configs": configs,
     }
     print(json.dumps(out), flush=True)
 
