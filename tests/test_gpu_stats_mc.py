@@ -97,7 +97,8 @@ def test_mc_checkerboard_matches_oracle(gpu, topology):
 def test_mc_curve_matches_reference(gpu):
     """Statistical parity: <|m|>(T) of checkerboard Metropolis on SC 8^3 against the reference
     random-site montecarlo::mc_step (its own MT19937 stream) within combined error bars
-    (4 sigma + 0.01 for finite-run bias), temperatures 100-900 K."""
+    (4 sigma + 0.01 for finite-run bias), temperatures 100-900 K, and the pseudo-critical
+    temperature (the <|m|> = 1/2 crossing) within 4 combined sigma + 5 K."""
     f = RC.load("mc_sc8")
     rows = f["M_T"]
     temps = rows[:, 0]
@@ -125,3 +126,20 @@ def test_mc_curve_matches_reference(gpu):
     for r in range(len(temps)):
         tol = 4.0 * np.hypot(err[r], rows[r, 2]) + 0.01
         assert abs(mean[r] - rows[r, 1]) <= tol, (temps[r], mean[r], rows[r, 1], tol)
+    # pseudo-critical temperature of the L = 8 system: where <|m|> crosses 1/2 (the reference's
+    # curve gives ~581 K; the infinite-lattice Tc of these parameters is ~634 K)
+    tc_ref, dtc_ref = _crossing(temps, rows[:, 1], rows[:, 2])
+    tc_dev, dtc_dev = _crossing(temps, mean, err)
+    assert 500.0 < tc_ref < 700.0
+    assert abs(tc_dev - tc_ref) <= 4.0 * np.hypot(dtc_ref, dtc_dev) + 5.0, (tc_dev, tc_ref, dtc_dev, dtc_ref)
+
+
+def _crossing(temps, m, dm, level=0.5):
+    """Temperature where the linear interpolation of m(T) crosses `level`, and its error from the
+    errors of the two bracketing points."""
+    k = int(np.nonzero((m[:-1] >= level) & (m[1:] < level))[0][0])
+    t1, t2, m1, m2 = temps[k], temps[k + 1], m[k], m[k + 1]
+    tc = t1 + (m1 - level) * (t2 - t1) / (m1 - m2)
+    d1 = (t2 - t1) * (level - m2) / (m1 - m2) ** 2
+    d2 = (t2 - t1) * (m1 - level) / (m1 - m2) ** 2
+    return tc, float(np.hypot(d1 * dm[k], d2 * dm[k + 1]))
