@@ -302,6 +302,16 @@ int vmx_fit_backward(vmx_ctx* ctx, const double* dL_dmz, double* grad, double* d
  * width of the reference backend, hdr/gpu.hpp:22.) */
 int vmx_autotune(vmx_ctx* ctx, int32_t steps_per_trial, int32_t* chosen_block_size);
 
+/* ---- checkpoints: resume a run with the same spins, Philox counters and Metropolis state ------
+ * Replaces, for the device state, sim::save_checkpoint / load_checkpoint (src/utility/
+ * checkpoint.cpp:27-207), which write the host spin arrays and the MT19937 state; here the random
+ * state is the Heun step and Metropolis sweep counters with the seed (Philox keys), and the spins
+ * come from the device.  The buffer is self-describing; the caller stores it as it likes.
+ * vmx_checkpoint_load needs a context created for the same system, precision and seed. */
+int vmx_checkpoint_bytes(vmx_ctx* ctx, uint64_t* bytes);
+int vmx_checkpoint_save(vmx_ctx* ctx, void* buf, uint64_t bytes);
+int vmx_checkpoint_load(vmx_ctx* ctx, const void* buf, uint64_t bytes);
+
 /* The launch configuration in use: threads per workgroup and 1 when workgroups are remapped
  * per XCD.  No counterpart in the reference; reported by benchmarks next to their timings. */
 int vmx_get_launch_config(vmx_ctx* ctx, int32_t* block_size, int32_t* xcd_remap);

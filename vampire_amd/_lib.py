@@ -142,6 +142,9 @@ SIGNATURES = [
     ("vmx_fit_forward", C.c_int, [_VP, _P(vmx_fit_params), _DP, _DP, C.c_int32, C.c_int32, C.c_int32, _DP]),
     ("vmx_fit_backward", C.c_int, [_VP, _DP, _DP, _DP]),
     ("vmx_autotune", C.c_int, [_VP, C.c_int32, _P(C.c_int32)]),
+    ("vmx_checkpoint_bytes", C.c_int, [_VP, _P(C.c_uint64)]),
+    ("vmx_checkpoint_save", C.c_int, [_VP, C.c_void_p, C.c_uint64]),
+    ("vmx_checkpoint_load", C.c_int, [_VP, C.c_void_p, C.c_uint64]),
     ("vmx_get_launch_config", C.c_int, [_VP, _P(C.c_int32), _P(C.c_int32)]),
 ]
 

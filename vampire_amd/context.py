@@ -171,6 +171,29 @@ class Context:
         L.check(self.lib.vmx_time_kernels(self.h, int(nsteps), L.dptr(ms), C.byref(nk)))
         return ms[:nk.value].copy()
 
+    # --- checkpoints --------------------------------------------------------------------------
+    def checkpoint(self) -> bytes:
+        """The device state (spins, Heun step and Metropolis sweep counters, conditions,
+        Metropolis cone widths) as a self-describing byte string."""
+        nb = C.c_uint64(0)
+        L.check(self.lib.vmx_checkpoint_bytes(self.h, C.byref(nb)))
+        buf = C.create_string_buffer(nb.value)
+        L.check(self.lib.vmx_checkpoint_save(self.h, buf, nb.value))
+        return buf.raw
+
+    def restore(self, data: bytes) -> None:
+        """Loads a checkpoint written by a context of the same system, precision and seed."""
+        buf = C.create_string_buffer(data, len(data))
+        L.check(self.lib.vmx_checkpoint_load(self.h, buf, len(data)))
+
+    def save_checkpoint(self, path: str) -> None:
+        with open(path, "wb") as f:
+            f.write(self.checkpoint())
+
+    def load_checkpoint(self, path: str) -> None:
+        with open(path, "rb") as f:
+            self.restore(f.read())
+
     @property
     def launch_config(self):
         """(threads per workgroup, XCD remapping on) in use."""

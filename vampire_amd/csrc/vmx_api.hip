@@ -1447,6 +1447,91 @@ int vmx_set_step(vmx_ctx* c, int64_t step) {
     return VMX_OK;
 }
 
+// ---- checkpoints -------------------------------------------------------------------------
+// Layout (little-endian, 8-byte fields):
+//   "VMXCHK01" | n | nrep | precision | field flags | Heun step | Metropolis sweep | seed
+//   | T[nrep] | H[nrep][3] | sigma[nrep] | moves[nrep] | rejects[nrep] | spins x[n] y[n] z[n]
+// Spins are stored in the caller's order as doubles (an FP32 spin converts exactly both ways),
+// so a context loaded from a checkpoint continues with the same values, Philox counters and
+// Metropolis state as the one that saved it.
+namespace {
+constexpr char kChkMagic[8] = {'V', 'M', 'X', 'C', 'H', 'K', '0', '1'};
+size_t chk_bytes(const vmx_ctx* c) {
+    return 8 * 8 + 8 * (size_t)c->nrep * (1 + 3 + 3) + 24 * (size_t)c->n;
+}
+}  // namespace
+
+int vmx_checkpoint_bytes(vmx_ctx* c, uint64_t* bytes) {
+    g_last_error.clear();
+    if (!c || !bytes) return fail(VMX_ERR_ARG, "null argument");
+    *bytes = chk_bytes(c);
+    return VMX_OK;
+}
+
+int vmx_checkpoint_save(vmx_ctx* c, void* buf, uint64_t bytes) {
+    g_last_error.clear();
+    if (!c || !buf) return fail(VMX_ERR_ARG, "null argument");
+    if (bytes < chk_bytes(c)) return fail(VMX_ERR_ARG, "checkpoint buffer of %llu bytes, need %llu",
+                                          (unsigned long long)bytes, (unsigned long long)chk_bytes(c));
+    VMX_TRY(c->use_device());
+    char* p = static_cast<char*>(buf);
+    auto put = [&](const void* v, size_t b) { std::memcpy(p, v, b); p += b; };
+    auto put64 = [&](uint64_t v) { put(&v, 8); };
+    put(kChkMagic, 8);
+    put64((uint64_t)c->n);
+    put64((uint64_t)c->nrep);
+    put64((uint64_t)c->precision);
+    put64((uint64_t)c->fld);
+    put64(c->step);
+    put64(c->sweep);
+    put64(c->seed);
+    put(c->T_h.data(), 8 * (size_t)c->nrep);
+    put(c->H_h.data(), 24 * (size_t)c->nrep);
+    std::vector<double> tmp(3 * (size_t)c->n);
+    VMX_TRY(vmx_mc_get_state(c, tmp.data(), tmp.data() + c->nrep, tmp.data() + 2 * (size_t)c->nrep));
+    put(tmp.data(), 24 * (size_t)c->nrep);
+    VMX_TRY(vmx_get_spins(c, tmp.data(), tmp.data() + c->n, tmp.data() + 2 * (size_t)c->n));
+    put(tmp.data(), 24 * (size_t)c->n);
+    return VMX_OK;
+}
+
+int vmx_checkpoint_load(vmx_ctx* c, const void* buf, uint64_t bytes) {
+    g_last_error.clear();
+    if (!c || !buf) return fail(VMX_ERR_ARG, "null argument");
+    if (bytes < chk_bytes(c)) return fail(VMX_ERR_ARG, "checkpoint of %llu bytes, this context needs %llu",
+                                          (unsigned long long)bytes, (unsigned long long)chk_bytes(c));
+    const char* p = static_cast<const char*>(buf);
+    auto get = [&](void* v, size_t b) { std::memcpy(v, p, b); p += b; };
+    auto get64 = [&]() { uint64_t v; get(&v, 8); return v; };
+    char magic[8];
+    get(magic, 8);
+    if (std::memcmp(magic, kChkMagic, 8) != 0) return fail(VMX_ERR_ARG, "not a libvmx checkpoint");
+    const uint64_t n = get64(), nrep = get64(), prec = get64(), fld = get64();
+    if (n != (uint64_t)c->n || nrep != (uint64_t)c->nrep || prec != (uint64_t)c->precision)
+        return fail(VMX_ERR_ARG, "checkpoint of %llu atoms, %llu replicas, precision %llu does not match the context",
+                    (unsigned long long)n, (unsigned long long)nrep, (unsigned long long)prec);
+    const uint64_t step = get64(), sweep = get64(), seed = get64();
+    if (seed != c->seed) return fail(VMX_ERR_ARG, "checkpoint seed %llu differs from the context's %llu",
+                                     (unsigned long long)seed, (unsigned long long)c->seed);
+    VMX_TRY(c->use_device());
+    get(c->T_h.data(), 8 * (size_t)c->nrep);
+    get(c->H_h.data(), 24 * (size_t)c->nrep);
+    c->fld = (uint32_t)fld & 0x7u;
+    VMX_TRY(c->upload_conditions_any());
+    std::vector<double> tmp(3 * (size_t)c->n);
+    get(tmp.data(), 24 * (size_t)c->nrep);
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    const size_t rb = c->nrep * sizeof(double);
+    VMX_HIP(hipMemcpy(c->mc_sigma.p, tmp.data(), rb, hipMemcpyHostToDevice));
+    VMX_HIP(hipMemcpy(c->mc_tot_moves.p, tmp.data() + c->nrep, rb, hipMemcpyHostToDevice));
+    VMX_HIP(hipMemcpy(c->mc_tot_rej.p, tmp.data() + 2 * (size_t)c->nrep, rb, hipMemcpyHostToDevice));
+    get(tmp.data(), 24 * (size_t)c->n);
+    VMX_TRY(vmx_set_spins(c, tmp.data(), tmp.data() + c->n, tmp.data() + 2 * (size_t)c->n));
+    VMX_TRY(vmx_set_step(c, (int64_t)step));
+    c->sweep = sweep;
+    return VMX_OK;
+}
+
 int vmx_mc_sweep(vmx_ctx* c, int32_t nsweeps) {
     if (!c) return fail(VMX_ERR_ARG, "null context");
     if (nsweeps < 0) return fail(VMX_ERR_ARG, "nsweeps must be >= 0");
