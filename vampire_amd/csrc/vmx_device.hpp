@@ -610,7 +610,7 @@ k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, Spin3<R> E, c
     R sx[U], sy[U], sz[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {   // own data from HBM, in flight while the LDS tables are staged
-        m[u] = s.type[idx[u]];
+        m[u] = s.nmat == 1 ? 0 : (int)s.type[idx[u]];   // one material: no per-atom load
         rs.load(idx[u], sx[u], sy[u], sz[u]);
         if (kStaged) head[u] = row_head<R, AM>(s, idx[u]);
     }
@@ -683,7 +683,7 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D, S
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int i = idx[u];
-        m[u] = s.type[i];
+        m[u] = s.nmat == 1 ? 0 : (int)s.type[i];
         rp.load(i, px[u], py[u], pz[u]);
         if (kStaged) head[u] = row_head<R, AM>(s, i);
         if (fold) {
