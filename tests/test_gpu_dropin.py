@@ -96,3 +96,64 @@ def test_dropin_applied_field_physics(gpu, exe, tmp_path):
     exp = np.array([np.cos(a) / np.cosh(a), np.sin(a) / np.cosh(a), np.tanh(a)])
     assert n == 1
     assert np.abs(spins[:3] - exp).max() < 1e-6
+
+
+DECKS = os.path.join(ROOT, "tests", "golden", "decks")
+
+
+def run_reference_deck(tmp, name, deck, acc, edits=()):
+    """Runs one of the reference's own test decks (tests/golden/decks/<deck>: input and material
+    file as the reference holds them) with gpu:acceleration appended and the given line edits;
+    returns the rows of its `output` file."""
+    d = tmp / name
+    d.mkdir()
+    src = os.path.join(DECKS, deck)
+    text = open(os.path.join(src, "input")).read()
+    for old, new in edits:
+        assert old in text, old
+        text = text.replace(old, new)
+    (d / "input").write_text(text + f"\ngpu:acceleration={acc}\n")
+    for f in os.listdir(src):
+        if f != "input":
+            (d / f).write_text(open(os.path.join(src, f)).read())
+    r = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=280)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    rows = [[float(v) for v in line.split()] for line in (d / "output").read_text().splitlines()
+            if line.strip() and not line.startswith("#")]
+    return np.array(rows)
+
+
+@pytest.mark.timeout(300)
+def test_reference_curie_deck_on_device(gpu, exe, tmp_path):
+    """The reference's tests/curie-temperature deck, unmodified but for gpu:acceleration=on (SC
+    22^3 cells, J = 11.2e-21 J, 0-1800 K in 100 K steps, 10k + 30k Heun steps each), run by
+    sim::run through gpu:: on the device.  Its output (columns: time, T, mean |m| over the
+    temperature's loop, m direction and |m|, |m|, field alignment) must show the ferromagnet's M(T): |m| = 1 at 0 K,
+    falling with T, crossing 1/2 below the infinite-lattice Tc = 1.443 J/kB = 1171 K and near
+    zero above it."""
+    out = run_reference_deck(tmp_path, "gpu", "curie-temperature", "on")
+    T, m = out[:, 1], out[:, 6]
+    assert np.array_equal(T, np.arange(0.0, 1801.0, 100.0))
+    assert m[0] == pytest.approx(1.0, abs=1e-12)
+    below = T <= 1200.0
+    assert np.all(np.diff(m[below]) < 0.02)
+    assert np.all(m[T >= 1400.0] < 0.05)
+    k = int(np.nonzero((m[:-1] >= 0.5) & (m[1:] < 0.5))[0][0])
+    t_half = T[k] + (m[k] - 0.5) * 100.0 / (m[k] - m[k + 1])
+    assert 800.0 < t_half < 1171.0
+
+
+@pytest.mark.timeout(300)
+def test_reference_curie_deck_device_vs_cpu(gpu, exe, tmp_path):
+    """A shortened copy of the same deck (5 nm cube, 0-1800 K in 300 K steps, 2000 + 4000 steps)
+    run by the reference CPU integrator and by the device: their noise streams differ (MT19937 vs
+    Philox), so the mean |m| over each temperature's loop (reset after equilibration by
+    stats::mag_m_reset, src/program/curie_temperature.cpp:111) must agree statistically (0.03)."""
+    edits = [("system-size-x = 7.7", "system-size-x = 5.0"), ("system-size-y = 7.7", "system-size-y = 5.0"),
+             ("system-size-z = 7.7", "system-size-z = 5.0"), ("temperature-increment=100", "temperature-increment=300"),
+             ("equilibration-time-steps=10000", "equilibration-time-steps=2000"),
+             ("loop-time-steps=30000", "loop-time-steps=4000")]
+    cpu = run_reference_deck(tmp_path, "cpu", "curie-temperature", "off", edits)
+    dev = run_reference_deck(tmp_path, "gpu", "curie-temperature", "on", edits)
+    assert np.array_equal(cpu[:, 1], dev[:, 1]) and len(cpu) == 7
+    assert np.abs(cpu[:, 2] - dev[:, 2]).max() <= 0.03, (cpu[:, 2], dev[:, 2])

@@ -267,9 +267,22 @@ void get() {
     }
 }
 
+// With host statistics the averages live in the host classes, so they are the ones cleared
+// (src/cuda/statistics.cu:115-121). The host energy averages are cleared in both modes, as the CPU
+// path does (src/statistics/statistics.cpp:74-77), since this backend accumulates them on the host.
 void reset() {
-    if (!g_ctx || gpu::cpu_stats) return;
-    check(vmx_stats_reset(g_ctx), "vmx_stats_reset");
+    if (!g_ctx) return;
+    if (gpu::cpu_stats) {
+        if (::stats::calculate_system_magnetization) ::stats::system_magnetization.reset_magnetization_averages();
+        if (::stats::calculate_material_magnetization) ::stats::material_magnetization.reset_magnetization_averages();
+        if (::stats::calculate_height_magnetization) ::stats::height_magnetization.reset_magnetization_averages();
+        if (::stats::calculate_material_height_magnetization)
+            ::stats::material_height_magnetization.reset_magnetization_averages();
+    } else {
+        check(vmx_stats_reset(g_ctx), "vmx_stats_reset");
+    }
+    if (::stats::calculate_system_energy) ::stats::system_energy.reset_averages();
+    if (::stats::calculate_material_energy) ::stats::material_energy.reset_averages();
 }
 
 }  // namespace stats
