@@ -316,6 +316,50 @@ int vmx_checkpoint_load(vmx_ctx* ctx, const void* buf, uint64_t bytes);
  * per XCD.  No counterpart in the reference; reported by benchmarks next to their timings. */
 int vmx_get_launch_config(vmx_ctx* ctx, int32_t* block_size, int32_t* xcd_remap);
 
+/* ---- macrocell dipole field (tensor solver) ----------------------------------------------------
+ * The cells and the cell-cell tensor as the reference builds them at set-up (dipole::initialize,
+ * src/dipole/initialize.cpp:31-150, tensor.cpp, inter.cpp, intra.cpp); the device then forms the
+ * field itself: cell moments (cells::mag, src/cells/mag.cpp:35-77), cell fields
+ * (dipole::internal::update_field, src/dipole/update.cpp:37-105) and atom fields
+ * (src/dipole/field.cpp:106-126), added to H_ext after the thermal and applied fields
+ * (fields.cpp:166, 291-310).  The reference's device backend leaves this to the host and copies
+ * spins out and fields in at every update (field.cpp:61, 80). */
+typedef struct vmx_dipole {
+    int32_t num_cells;             /* cells::num_cells                                          */
+    int32_t num_local_cells;       /* cells with magnetic atoms: rows of the tensor              */
+    const int32_t* local_cell;     /* [num_local_cells] cells::cell_id_array                     */
+    const int32_t* atom_cell;      /* [num_atoms] cells::atom_cell_id_array (caller's order)      */
+    const int32_t* magnetic;       /* [num_materials] 0 for mp::material[m].non_magnetic; NULL:
+                                      every material magnetic                                    */
+    const double* volume;          /* [num_cells] cells::volume_array (A^3), self-demag term      */
+    const double* tensor;          /* [6][num_local_cells][num_cells]: rij_tensor_xx, _xy, _xz,
+                                      _yy, _yz, _zz [local cell][cell] (dipole/internal.hpp)      */
+    int32_t update_rate;           /* dipole::update_rate: Heun steps between refreshes          */
+} vmx_dipole;
+
+/* Sets the dipole solver and computes the field of the current spins (time 0, as
+ * dipole::initialize does, initialize.cpp:143); vmx_llg_heun then refreshes it after every step
+ * whose count since this call is a multiple of update_rate (dipole::calculate_field after
+ * sim::increment_time, field.cpp:43-71, sim.cpp:191-200).  NULL removes the field.  One replica
+ * only; graph replay, Metropolis sweeps, device energy statistics and checkpoints refuse while a
+ * dipole field is set. */
+int vmx_set_dipole(vmx_ctx* ctx, const vmx_dipole* dipole);
+
+/* Recomputes the field from the current spins now (e.g. after vmx_set_spins). */
+int vmx_update_dipole(vmx_ctx* ctx);
+
+/* Per-atom fields of the last refresh, caller's order (any pointer may be NULL): B
+ * (dipole::atom_dipolar_field_array_*, the field in H_ext) and mu0 Hd
+ * (dipole::atom_mu0demag_field_array_*, the field of the magnetostatic energy). */
+int vmx_get_dipole_field(vmx_ctx* ctx, double* bx, double* by, double* bz, double* hx, double* hy,
+                         double* hz);
+
+/* A per-atom field (tesla, caller's order) added to H_ext where the dipole field goes, for callers
+ * that compute the dipole field themselves: gpu::transfer_dipole_fields_from_cpu_to_gpu
+ * (src/gpu/transfer.cpp:39, called from dipole::calculate_field, field.cpp:80) uploads
+ * dipole::atom_dipolar_field_array_* with it.  Three NULLs remove it. */
+int vmx_set_atom_field(vmx_ctx* ctx, const double* hx, const double* hy, const double* hz);
+
 #ifdef __cplusplus
 }
 #endif

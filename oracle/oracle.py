@@ -31,11 +31,19 @@ _IP = _P(C.c_int)
 _DP = _P(C.c_double)
 
 
+class or_dipole(C.Structure):
+    _fields_ = [
+        ("ncells", C.c_int), ("nlocal", C.c_int), ("local_cell", _IP), ("atom_cell", _IP), ("magnetic", _IP),
+        ("volume", _DP), ("tensor", _DP), ("update_rate", C.c_int), ("time", C.c_longlong), ("atom_B", _DP),
+        ("atom_Hd", _DP), ("cell_mag", _DP), ("cell_B", _DP), ("cell_Hd", _DP),
+    ]
+
+
 class or_sys(C.Structure):
     _fields_ = [
         ("n", C.c_int), ("nmat", C.c_int), ("type", _IP), ("moment", _DP), ("nbr_start", _IP),
         ("nbr_end", _IP), ("nbr", _IP), ("iid", _IP), ("exchange_type", C.c_int), ("exchange", _DP),
-        ("mat", _P(or_material)), ("ani", C.c_uint),
+        ("mat", _P(or_material)), ("ani", C.c_uint), ("dip", _P(or_dipole)),
     ]
 
 
@@ -70,6 +78,7 @@ def load():
                                            C.c_uint64, C.c_uint64, _DP, _DP, _DP]
         lib.or_mc_random_site.argtypes = [_P(or_sys), _P(or_cond), _DP, _DP, _DP, C.c_int, C.c_uint64,
                                           C.c_uint64, _DP, _DP, _DP]
+        lib.or_dipole_update.argtypes = [_P(or_sys), _DP, _DP, _DP]
         lib.or_greedy_colour.argtypes = [_P(or_sys), _IP]
         lib.or_greedy_colour.restype = C.c_int
         _lib = lib
@@ -132,6 +141,39 @@ class Oracle:
                           exchange_type=s.exchange_type, exchange=_d(k["ex"]), mat=mats,
                           ani=s.anisotropy_flags)
         self.cond = or_cond(temperature=0.0, fld=0, dt=s.dt, half_dt=s.half_dt)
+
+    def set_dipole(self, spins: np.ndarray, num_cells: int, local_cell, atom_cell, volume, tensor,
+                   update_rate: int, magnetic=None):
+        """Macrocell dipole field, tensor solver: cells and tensor as the reference builds them
+        (tensor [6][len(local_cell)][num_cells]); the field of `spins` is computed now (time 0) and
+        refreshed every `update_rate` Heun steps or sweeps."""
+        nl = len(local_cell)
+        k = dict(local_cell=np.ascontiguousarray(local_cell, dtype=np.int32),
+                 atom_cell=np.ascontiguousarray(atom_cell, dtype=np.int32),
+                 magnetic=np.ascontiguousarray(np.ones(len(self.sys_.materials)) if magnetic is None else magnetic,
+                                               dtype=np.int32),
+                 volume=np.ascontiguousarray(volume, dtype=np.float64),
+                 tensor=np.ascontiguousarray(tensor, dtype=np.float64).reshape(-1),
+                 atom_B=np.zeros(3 * self.n), atom_Hd=np.zeros(3 * self.n), cell_mag=np.zeros(3 * num_cells),
+                 cell_B=np.zeros(3 * num_cells), cell_Hd=np.zeros(3 * num_cells))
+        assert k["tensor"].size == 6 * nl * num_cells and k["atom_cell"].size == self.n
+        self._dip_keep = k
+        self.dip = or_dipole(ncells=int(num_cells), nlocal=nl, local_cell=_i(k["local_cell"]),
+                             atom_cell=_i(k["atom_cell"]), magnetic=_i(k["magnetic"]), volume=_d(k["volume"]),
+                             tensor=_d(k["tensor"]), update_rate=int(update_rate), time=0, atom_B=_d(k["atom_B"]),
+                             atom_Hd=_d(k["atom_Hd"]), cell_mag=_d(k["cell_mag"]), cell_B=_d(k["cell_B"]),
+                             cell_Hd=_d(k["cell_Hd"]))
+        self.sys.dip = C.pointer(self.dip)
+        self.dipole_update(spins)
+
+    def dipole_update(self, spins: np.ndarray):
+        sp = [np.ascontiguousarray(spins[k], dtype=np.float64) for k in range(3)]
+        self.lib.or_dipole_update(C.byref(self.sys), *[_d(a) for a in sp])
+
+    def dipole_fields(self):
+        """(atom B [3][n], atom mu0 Hd [3][n], cell moments, cell B, cell mu0 Hd [3][num_cells])."""
+        k = self._dip_keep
+        return tuple(k[x].reshape(3, -1).copy() for x in ("atom_B", "atom_Hd", "cell_mag", "cell_B", "cell_Hd"))
 
     def set_conditions(self, temperature: float, H=(0.0, 0.0, 0.0), thermal=True, applied=True,
                        local_applied=False):

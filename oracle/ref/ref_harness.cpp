@@ -4,6 +4,7 @@
 //   ref_harness <rundir> dump                       system after cs::create + sim::run set-up
 //   ref_harness <rundir> llg <nsteps> <every>       sim::LLG_Heun with injected noise
 //   ref_harness <rundir> mc <nsweeps> <T> <every>   montecarlo::mc_step, reference RNG
+//   ref_harness <rundir> dipole                     macrocell dipole set-up and fields (dipole.bin)
 //
 // <rundir> holds `input` (and the material file it names).  Inputs read from <rundir>:
 //   spins_in.bin  optional: 3*N doubles (x block, y block, z block) replacing the initial spins
@@ -20,6 +21,9 @@
 
 #include "anisotropy.hpp"
 #include "atoms.hpp"
+#include "cells.hpp"
+#include "dipole.hpp"
+#include "dipole/internal.hpp"
 #include "create.hpp"
 #include "exchange.hpp"
 #include "material.hpp"
@@ -96,6 +100,48 @@ void setup() {
             atoms::z_spin_array[i] = sp[2 * n + i];
         }
     }
+    // the dipole set-up of sim::run (src/simulate/sim.cpp:270-292); returns at once unless the
+    // input activates the dipole field, and computes the field of the spins above at time 0
+    dipole::initialize(cells::num_atoms_in_unit_cell, cells::num_cells, cells::num_local_cells,
+                       cells::macro_cell_size, cells::local_cell_array, cells::num_atoms_in_cell,
+                       cells::num_atoms_in_cell_global, cells::index_atoms_array, cells::volume_array,
+                       cells::pos_and_mom_array, cells::atom_in_cell_coords_array_x,
+                       cells::atom_in_cell_coords_array_y, cells::atom_in_cell_coords_array_z,
+                       atoms::type_array, cells::atom_cell_id_array, atoms::x_coord_array,
+                       atoms::y_coord_array, atoms::z_coord_array, atoms::x_spin_array,
+                       atoms::y_spin_array, atoms::z_spin_array, atoms::m_spin_array, atoms::num_atoms);
+}
+
+// Macrocell dipole data after a field update: sizes in meta_dipole.txt; dipole.bin holds
+// local_cell[nl] (int32), atom_cell[n] (int32), num_atoms_in_cell[nc] (int32), volume[nc],
+// the six tensors [6][nl][nc], cells::mag[3][nc], the cell B and mu0Hd fields [3][nc] each and the
+// atom B and mu0Hd fields [3][n] each (doubles).
+void dump_dipole(std::ofstream& f) {
+    const int nc = dipole::internal::cells_num_cells, nl = dipole::internal::cells_num_local_cells;
+    std::ofstream meta("meta_dipole.txt");
+    meta.precision(17);
+    meta << "num_cells " << nc << "\nnum_local_cells " << nl << "\nupdate_rate " << dipole::update_rate
+         << "\nmacro_cell_size " << cells::macro_cell_size << "\n";
+    std::vector<int> lc(cells::cell_id_array.begin(), cells::cell_id_array.begin() + nl);
+    put(f, lc);
+    put(f, dipole::internal::atom_cell_id_array);
+    std::vector<int> cnt(dipole::internal::cells_num_atoms_in_cell.begin(),
+                         dipole::internal::cells_num_atoms_in_cell.begin() + nc);
+    put(f, cnt);
+    std::vector<double> vol(dipole::internal::cells_volume_array.begin(),
+                            dipole::internal::cells_volume_array.begin() + nc);
+    put(f, vol);
+    for (auto* t : {&dipole::internal::rij_tensor_xx, &dipole::internal::rij_tensor_xy, &dipole::internal::rij_tensor_xz,
+                    &dipole::internal::rij_tensor_yy, &dipole::internal::rij_tensor_yz, &dipole::internal::rij_tensor_zz})
+        for (int l = 0; l < nl; ++l) put(f, (*t)[l]);
+    put(f, cells::mag_array_x); put(f, cells::mag_array_y); put(f, cells::mag_array_z);
+    put(f, dipole::cells_field_array_x); put(f, dipole::cells_field_array_y); put(f, dipole::cells_field_array_z);
+    put(f, dipole::cells_mu0Hd_field_array_x); put(f, dipole::cells_mu0Hd_field_array_y);
+    put(f, dipole::cells_mu0Hd_field_array_z);
+    put(f, dipole::atom_dipolar_field_array_x); put(f, dipole::atom_dipolar_field_array_y);
+    put(f, dipole::atom_dipolar_field_array_z);
+    put(f, dipole::atom_mu0demag_field_array_x); put(f, dipole::atom_mu0demag_field_array_y);
+    put(f, dipole::atom_mu0demag_field_array_z);
 }
 
 void dump_system() {
@@ -208,6 +254,8 @@ int main(int argc, char** argv) {
         std::ofstream tf("traj.bin", std::ios::binary);
         for (int s = 1; s <= nsteps; ++s) {
             sim::LLG_Heun();
+            // sim::increment_time refreshes the dipole field after each step (sim.cpp:191-200)
+            dipole::calculate_field(s, atoms::x_spin_array, atoms::y_spin_array, atoms::z_spin_array);
             if (s % every == 0) {
                 put(tf, atoms::x_spin_array);
                 put(tf, atoms::y_spin_array);
@@ -238,6 +286,29 @@ int main(int argc, char** argv) {
                 put(mf, m);
             }
         }
+        return 0;
+    }
+    if (cmd == "dipole") {
+        if (!dipole::activated) {
+            std::cerr << "the input does not activate the dipole field" << std::endl;
+            return 2;
+        }
+        // the time-0 field of spins_in.bin, then the field of spins2.bin at the next update time
+        std::ofstream df("dipole.bin", std::ios::binary);
+        dump_dipole(df);
+        std::vector<double> sp2 = read_doubles("spins2.bin");
+        if (sp2.size() != 3 * (size_t)n) {
+            std::cerr << "spins2.bin must hold 3*N doubles" << std::endl;
+            return 2;
+        }
+        for (int i = 0; i < n; ++i) {
+            atoms::x_spin_array[i] = sp2[i];
+            atoms::y_spin_array[i] = sp2[n + i];
+            atoms::z_spin_array[i] = sp2[2 * n + i];
+        }
+        dipole::calculate_field(dipole::update_rate, atoms::x_spin_array, atoms::y_spin_array, atoms::z_spin_array);
+        std::ofstream df2("dipole2.bin", std::ios::binary);
+        dump_dipole(df2);
         return 0;
     }
     std::cerr << "unknown command " << cmd << std::endl;

@@ -29,6 +29,14 @@
  *   or_mc_random_site     montecarlo::mc_step              src/montecarlo/mc.cpp:29-127,
  *                                                          mc_moves.cpp:71-87,158-162
  *   or_mc_checkerboard    the colour-ordered variant of mc_step run by the device
+ *   or_dipole_update      cells::mag                       src/cells/mag.cpp:35-77
+ *                         dipole::internal::update_field   src/dipole/update.cpp:37-105 (tensor solver)
+ *                         atom fields from cell fields     src/dipole/field.cpp:106-126
+ *                         the refresh every update_rate steps, src/dipole/field.cpp:43-71 after
+ *                         sim::increment_time (src/simulate/sim.cpp:191-200); the field enters
+ *                         H_ext last (fields.cpp:166, 291-310), the demag field the single-spin
+ *                         and statistic energies (src/simulate/energy.cpp:123-125, 171;
+ *                         src/statistics/energy.cpp:198-208)
  *
  * Noise: the reference draws MT19937 + Ziggurat normals (src/random/random.cpp:219-244), x for all
  * atoms, then y, then z (fields.cpp:274-276).  The oracle takes either an explicit noise array in
@@ -43,6 +51,9 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
 
 typedef struct {
     double alpha, gamma_rel, mu_s_SI;
@@ -52,6 +63,24 @@ typedef struct {
     double kc4, kc_vector1[3], kc_vector2[3], kc_vector3[3];
     double applied_field_strength, applied_field_unit_vector[3];
 } or_material;
+
+/* Macrocell dipole field, tensor solver: inputs as the reference builds them at set-up, and the
+ * field state, refreshed by or_dipole_update. */
+typedef struct {
+    int ncells, nlocal;
+    const int* local_cell;       /* [nlocal] cells::cell_id_array: the cell of each tensor row */
+    const int* atom_cell;        /* [n] cells::atom_cell_id_array */
+    const int* magnetic;         /* [nmat] 1 unless mp::material[m].non_magnetic */
+    const double* volume;        /* [ncells] cells::volume_array */
+    const double* tensor;        /* [6][nlocal][ncells] rij_tensor_{xx,xy,xz,yy,yz,zz}[lc][j] */
+    int update_rate;             /* dipole::update_rate */
+    long long time;              /* Heun steps / sweeps since the field was set up */
+    double* atom_B;              /* [3][n] dipole::atom_dipolar_field_array_{x,y,z} */
+    double* atom_Hd;             /* [3][n] dipole::atom_mu0demag_field_array_{x,y,z} */
+    double* cell_mag;            /* [3][ncells] cells::mag_array_{x,y,z} */
+    double* cell_B;              /* [3][ncells] dipole::cells_field_array_{x,y,z} */
+    double* cell_Hd;             /* [3][ncells] dipole::cells_mu0Hd_field_array_{x,y,z} */
+} or_dipole;
 
 typedef struct {
     int n, nmat;
@@ -65,6 +94,7 @@ typedef struct {
     const double* exchange;      /* [num_interactions][1|3|9] */
     const or_material* mat;
     unsigned ani;                /* 1 uniaxial2, 2 cubic4, 4 cubic4 rotated */
+    or_dipole* dip;              /* NULL: no dipole field */
 } or_sys;
 
 typedef struct {
@@ -264,8 +294,107 @@ void or_external_fields(const or_sys* s, const or_cond* c, const double* noise, 
             ez[atom] += happ[3 * m + 2];
         }
     }
+    if (s->dip) {                                               /* fields.cpp:291-310 */
+        const double* B = s->dip->atom_B;
+        for (int atom = 0; atom < n; ++atom) {
+            ex[atom] += B[atom];
+            ey[atom] += B[(size_t)n + atom];
+            ez[atom] += B[2 * (size_t)n + atom];
+        }
+    }
     free(sigma);
     free(happ);
+}
+
+/* ---- macrocell dipole field (tensor solver) -------------------------------------------------- */
+void or_dipole_update(const or_sys* s, const double* sx, const double* sy, const double* sz) {
+    or_dipole* d = s->dip;
+    const int n = s->n, nc = d->ncells;
+    double* mx = d->cell_mag;
+    double* my = mx + nc;
+    double* mz = my + nc;
+    int* count = (int*)calloc((size_t)nc, sizeof(int));
+    for (int i = 0; i < nc; ++i) { mx[i] = 0.0; my[i] = 0.0; mz[i] = 0.0; }
+    for (int i = 0; i < n; ++i) {                               /* cells/mag.cpp:53-66 */
+        const int cell = d->atom_cell[i];
+        const int type = s->type[i];
+        const double mus = s->mat[type].mu_s_SI;
+        if (d->magnetic[type]) {
+            mx[cell] += sx[i] * mus;
+            my[cell] += sy[i] * mus;
+            mz[cell] += sz[i] * mus;
+            count[cell]++;                                      /* cells/initialize.cpp:209-224 */
+        }
+    }
+    const double imuB = 1.0 / 9.27400915e-24;
+    const size_t ts = (size_t)d->nlocal * nc;
+    double* bx = d->cell_B;
+    double* by = bx + nc;
+    double* bz = by + nc;
+    double* hx = d->cell_Hd;
+    double* hy = hx + nc;
+    double* hz = hy + nc;
+    for (int lc = 0; lc < d->nlocal; lc++) {                    /* dipole/update.cpp:52-93 */
+        const int i = d->local_cell[lc];
+        if (count[i] > 0) {
+            const double eightPI_three_cell_volume = 8.0 * M_PI / (3.0 * d->volume[i]);
+            const double self_demag = eightPI_three_cell_volume;
+            const double mx_i = mx[i] * imuB;
+            const double my_i = my[i] * imuB;
+            const double mz_i = mz[i] * imuB;
+            bx[i] = self_demag * mx_i * 0.0;
+            by[i] = self_demag * my_i * 0.0;
+            bz[i] = self_demag * mz_i * 0.0;
+            hx[i] = -0.5 * self_demag * mx_i;
+            hy[i] = -0.5 * self_demag * my_i;
+            hz[i] = -0.5 * self_demag * mz_i;
+            const double* txx = d->tensor + (size_t)lc * nc;
+            const double* txy = txx + ts;
+            const double* txz = txy + ts;
+            const double* tyy = txz + ts;
+            const double* tyz = tyy + ts;
+            const double* tzz = tyz + ts;
+            for (int j = 0; j < nc; j++) {
+                if (count[j] > 0) {
+                    const double mxj = mx[j] * imuB;
+                    const double myj = my[j] * imuB;
+                    const double mzj = mz[j] * imuB;
+                    bx[i] += (mxj * txx[j] + myj * txy[j] + mzj * txz[j]);
+                    by[i] += (mxj * txy[j] + myj * tyy[j] + mzj * tyz[j]);
+                    bz[i] += (mxj * txz[j] + myj * tyz[j] + mzj * tzz[j]);
+                    hx[i] += (mxj * txx[j] + myj * txy[j] + mzj * txz[j]);
+                    hy[i] += (mxj * txy[j] + myj * tyy[j] + mzj * tyz[j]);
+                    hz[i] += (mxj * txz[j] + myj * tyz[j] + mzj * tzz[j]);
+                }
+            }
+            bx[i] = bx[i] * 9.27400915e-01;
+            by[i] = by[i] * 9.27400915e-01;
+            bz[i] = bz[i] * 9.27400915e-01;
+            hx[i] = hx[i] * 9.27400915e-01;
+            hy[i] = hy[i] * 9.27400915e-01;
+            hz[i] = hz[i] * 9.27400915e-01;
+        }
+    }
+    for (int atom = 0; atom < n; atom++) {                      /* dipole/field.cpp:113-125 */
+        const int cell = d->atom_cell[atom];
+        if (count[cell] > 0 && d->magnetic[s->type[atom]]) {
+            d->atom_B[atom] = bx[cell];
+            d->atom_B[(size_t)n + atom] = by[cell];
+            d->atom_B[2 * (size_t)n + atom] = bz[cell];
+            d->atom_Hd[atom] = hx[cell];
+            d->atom_Hd[(size_t)n + atom] = hy[cell];
+            d->atom_Hd[2 * (size_t)n + atom] = hz[cell];
+        }
+    }
+    free(count);
+}
+
+/* After a Heun step or a Monte Carlo sweep: the refresh of dipole::calculate_field when the time
+ * reaches a multiple of the update rate (src/dipole/field.cpp:52-58). */
+static void dipole_tick(const or_sys* s, const double* sx, const double* sy, const double* sz) {
+    if (!s->dip) return;
+    s->dip->time++;
+    if (s->dip->time % s->dip->update_rate == 0) or_dipole_update(s, sx, sy, sz);
 }
 
 /* ---- integrator: sim::LLG_Heun ------------------------------------------------------------ */
@@ -359,8 +488,10 @@ int or_llg_heun(const or_sys* s, const or_cond* c, double* sx, double* sy, doubl
     memset(&a, 0, sizeof a);
     if (alloc_arrays(&a, s->n)) { free_arrays(&a); return -1; }
     set_threads(nthreads);
-    for (int k = 0; k < nsteps; ++k)
+    for (int k = 0; k < nsteps; ++k) {
         heun_step(s, c, sx, sy, sz, noise ? noise + (size_t)k * 3 * s->n : NULL, seed, step0 + (uint64_t)k, &a);
+        dipole_tick(s, sx, sy, sz);
+    }
     free_arrays(&a);
     return 0;
 }
@@ -453,6 +584,11 @@ double or_spin_energy(const or_sys* s, const or_cond* c, const double* sx_a, con
     energy += exchange_energy(s, sx_a, sy_a, sz_a, atom, sx, sy, sz);
     energy += anisotropy_energy(s, s->type[atom], sx, sy, sz);
     energy += applied_energy(c, sx, sy, sz);
+    if (s->dip) {                                               /* simulate/energy.cpp:123-125, 171 */
+        const double* H = s->dip->atom_Hd;
+        const size_t n = (size_t)s->n;
+        energy += -1.0 * (H[atom] * sx + H[n + atom] * sy + H[2 * n + atom] * sz);
+    }
     return energy;
 }
 
@@ -476,6 +612,12 @@ void or_energy(const or_sys* s, const or_cond* c, const double* sx, const double
         ani[mask[atom]] += anisotropy_energy(s, s->type[atom], sx[atom], sy[atom], sz[atom]) * s->moment[atom];
     for (int atom = 0; atom < n; ++atom)
         app[mask[atom]] += applied_energy(c, sx[atom], sy[atom], sz[atom]) * s->moment[atom];
+    if (s->dip) {                                               /* statistics/energy.cpp:198-203 */
+        const double* H = s->dip->atom_Hd;
+        for (int atom = 0; atom < n; ++atom)
+            mag[mask[atom]] += -1.0 * (H[atom] * sx[atom] + H[(size_t)n + atom] * sy[atom] +
+                                       H[2 * (size_t)n + atom] * sz[atom]) * s->moment[atom];
+    }
     for (int id = 0; id < mask_size; ++id) mag[id] = 0.5 * mag[id];
     for (int id = 0; id < mask_size; ++id) tot[id] = exc[id] + ani[id] + app[id] + mag[id];
     for (int id = 0; id < mask_size; ++id)
@@ -536,6 +678,7 @@ void or_mc_checkerboard(const or_sys* s, const or_cond* c, double* sx, double* s
         *tot_moves += (double)s->n;
         *tot_rej += rej;
         adapt(sigma, (double)s->n, rej);
+        dipole_tick(s, sx, sy, sz);
     }
 }
 
@@ -560,6 +703,7 @@ void or_mc_random_site(const or_sys* s, const or_cond* c, double* sx, double* sy
         *tot_moves += (double)nmoves;
         *tot_rej += rej;
         adapt(sigma, (double)nmoves, rej);
+        dipole_tick(s, sx, sy, sz);
     }
 }
 

@@ -35,7 +35,7 @@ OUT = os.path.dirname(os.path.abspath(__file__))
 
 
 def deck(crystal, L, a, materials, pbc=(True, True, True), T=0.0, H=0.0, Hdir=(0, 0, 1), dmi_cutoff=None,
-         Lz=None):
+         Lz=None, extra=()):
     lines = [
         f"create:crystal-structure={crystal}",
         f"dimensions:unit-cell-size = {a} !A",
@@ -58,6 +58,7 @@ def deck(crystal, L, a, materials, pbc=(True, True, True), T=0.0, H=0.0, Hdir=(0
             lines.append(f"create:periodic-boundaries-{axis}")
     if dmi_cutoff is not None:
         lines.append(f"exchange:dmi-cutoff-range = {dmi_cutoff} !A")
+    lines.extend(extra)
     mat = [f"material:num-materials={len(materials)}"]
     for k, m in enumerate(materials, start=1):
         for key, val in m.items():
@@ -157,6 +158,74 @@ def case(name, crystal, L, a, materials, steps, snaps, T, H=0.0, Hdir=(0, 0, 1),
           f"interactions={meta['num_interactions']}")
 
 
+def read_dipole(path, n, nc, nl):
+    raw = open(path, "rb").read()
+    off = 0
+
+    def take(dtype, count):
+        nonlocal off
+        a = np.frombuffer(raw, dtype=dtype, count=count, offset=off).copy()
+        off += a.nbytes
+        return a
+
+    d = dict(local_cell=take(np.int32, nl), atom_cell=take(np.int32, n), cell_count=take(np.int32, nc),
+             volume=take(np.float64, nc), tensor=take(np.float64, 6 * nl * nc).reshape(6, nl, nc),
+             cell_mag=take(np.float64, 3 * nc).reshape(3, nc), cell_B=take(np.float64, 3 * nc).reshape(3, nc),
+             cell_Hd=take(np.float64, 3 * nc).reshape(3, nc), atom_B=take(np.float64, 3 * n).reshape(3, n),
+             atom_Hd=take(np.float64, 3 * n).reshape(3, n))
+    assert off == len(raw), (off, len(raw))
+    return d
+
+
+def dipole_case(name, L, Lz, a, materials, cell_size, rate, steps, snaps, T, seed=13):
+    """Macrocell dipole field (tensor solver, open boundaries): the reference's cells, tensor and
+    fields for two random spin sets, and an LLG trajectory with the field refreshed every `rate`
+    steps as sim::increment_time does."""
+    extra = ["dipole:solver=tensor", f"dipole:field-update-rate={rate}", f"cells:macro-cell-size = {cell_size} !A"]
+    with tempfile.TemporaryDirectory(prefix="vref_dip_") as d:
+        inp, mat = deck("sc", L, a, materials, (False, False, False), T, 0.0, (0, 0, 1), None, Lz, extra)
+        open(os.path.join(d, "input"), "w").write(inp)
+        open(os.path.join(d, "test.mat"), "w").write(mat)
+        run(d, "dump")
+        meta = read_meta(d)
+        sysd = read_system(d, meta)
+        n = meta["num_atoms"]
+        rng = np.random.default_rng(seed)
+        sp = []
+        for _ in range(2):
+            g = rng.standard_normal((3, n))
+            sp.append(g / np.sqrt((g * g).sum(axis=0)))
+        sp[0].tofile(os.path.join(d, "spins_in.bin"))
+        sp[1].tofile(os.path.join(d, "spins2.bin"))
+        run(d, "dipole")
+        dm = {k.split()[0]: k.split()[1] for k in open(os.path.join(d, "meta_dipole.txt"))}
+        nc, nl = int(dm["num_cells"]), int(dm["num_local_cells"])
+        d0 = read_dipole(os.path.join(d, "dipole.bin"), n, nc, nl)
+        d1 = read_dipole(os.path.join(d, "dipole2.bin"), n, nc, nl)
+        blocks = [O.thermal_noise(seed, n, s) for s in range(steps + 1)]
+        np.concatenate([b.reshape(-1) for b in blocks]).tofile(os.path.join(d, "noise.bin"))
+        run(d, "llg", steps, 1)
+        traj = np.fromfile(os.path.join(d, "traj.bin"), dtype=np.float64).reshape(steps, 3, n)
+    for k in ("local_cell", "atom_cell", "cell_count", "volume", "tensor"):
+        assert np.array_equal(d0[k], d1[k]), k
+    np.savez_compressed(
+        os.path.join(OUT, f"ref_{name}.npz"),
+        L=L, Lz=Lz, a=a, T=T, noise_seed=seed, update_rate=rate, macro_cell_size=float(dm["macro_cell_size"]),
+        spins_in=sp[0], spins2=sp[1], snap_steps=np.array(snaps), snaps=traj[np.array(snaps) - 1],
+        dt=meta["dt"], half_dt=meta["half_dt"], exchange_type=meta["exchange_type"],
+        materials=np.array([[m[k] for k in ("alpha", "gamma_rel", "mu_s_SI", "one_oneplusalpha_sq",
+                                            "alpha_oneplusalpha_sq", "H_th_sigma", "ku2", "mu_s_muB")]
+                            for m in meta["materials"]]),
+        type=sysd["type"], moment=sysd["moment"], nbr_start=sysd["nbr_start"], nbr_end=sysd["nbr_end"],
+        nbr=sysd["nbr"], nbr_iid=sysd["nbr_iid"], exchange=sysd["exchange"], coords=sysd["coords"],
+        num_cells=nc, local_cell=d0["local_cell"], atom_cell=d0["atom_cell"], cell_count=d0["cell_count"],
+        volume=d0["volume"], tensor=d0["tensor"],
+        **{f"{k}0": d0[k] for k in ("cell_mag", "cell_B", "cell_Hd", "atom_B", "atom_Hd")},
+        **{f"{k}1": d1[k] for k in ("cell_mag", "cell_B", "cell_Hd", "atom_B", "atom_Hd")},
+    )
+    print(f"{name}: n={n} cells={nc} local={nl} rate={rate}")
+
+
 def mc_case(name, L, a, materials, temps, sweeps, every, equil):
     rows = []
     for T in temps:
@@ -182,20 +251,28 @@ CO = {"material-name": "Co", "damping-constant": 0.1, "exchange-matrix[1]": 6.06
       "uniaxial-anisotropy-direction": "0,0,1", "initial-spin-direction": "random"}
 
 
-def main():
+def main(only=()):
+    """Writes every fixture, or those named in `only` (e.g. `dipole_sc10`)."""
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
+
+    def want(name):
+        return not only or name in only
+
     # SC, scalar exchange, uniaxial anisotropy, applied field, thermal noise
-    case("sc6_uniaxial", "sc", 6, 3.54, [CO], steps=200, snaps=[1, 10, 200], T=300.0, H=0.5)
+    if want("sc6_uniaxial"):
+        case("sc6_uniaxial", "sc", 6, 3.54, [CO], steps=200, snaps=[1, 10, 200], T=300.0, H=0.5)
     # FCC, cubic anisotropy, alpha 0.05, T = 600 K
     fcc = dict(CO, **{"damping-constant": 0.05, "second-order-uniaxial-anisotropy-constant": 0.0,
                       "fourth-order-cubic-anisotropy-constant": 3.0e-24})
-    case("fcc3_cubic", "fcc", 3, 3.54, [fcc], steps=100, snaps=[1, 100], T=600.0, H=0.0)
+    if want("fcc3_cubic"):
+        case("fcc3_cubic", "fcc", 3, 3.54, [fcc], steps=100, snaps=[1, 100], T=600.0, H=0.0)
     # BCC, rotated cubic basis, field along (1,0,1)/sqrt2
     bcc = dict(CO, **{"second-order-uniaxial-anisotropy-constant": 0.0,
                       "fourth-order-cubic-anisotropy-constant": -2.0e-24,
                       "cubic-anisotropy-direction-1": "1,1,0", "cubic-anisotropy-direction-2": "-1,1,0"})
-    case("bcc4_rotated", "bcc", 4, 2.87, [bcc], steps=100, snaps=[1, 100], T=100.0, H=2.0, Hdir=(1, 0, 1))
+    if want("bcc4_rotated"):
+        case("bcc4_rotated", "bcc", 4, 2.87, [bcc], steps=100, snaps=[1, 100], T=100.0, H=2.0, Hdir=(1, 0, 1))
     # two-material SC film, open in z, ferrimagnetic inter-material exchange and interfacial DMI:
     # tensorial exchange with the reference's DMI tensors
     m1 = {"material-name": "A", "damping-constant": 0.1, "exchange-matrix[1]": 5e-21,
@@ -204,16 +281,27 @@ def main():
     m2 = {"material-name": "B", "damping-constant": 0.1, "exchange-matrix[1]": -1.2e-21,
           "exchange-matrix[2]": 5e-21, "atomic-spin-moment": "7.6 !muB", "dmi-constant[1]": 0.3e-21,
           "minimum-height": 0.5, "maximum-height": 1.0, "initial-spin-direction": "random"}
-    case("film_dmi", "sc", 5, 2.5, [m1, m2], steps=100, snaps=[1, 100], T=300.0, H=0.0,
-         pbc=(True, True, False), dmi_cutoff=3.6, Lz=4)
+    if want("film_dmi"):
+        case("film_dmi", "sc", 5, 2.5, [m1, m2], steps=100, snaps=[1, 100], T=300.0, H=0.0,
+             pbc=(True, True, False), dmi_cutoff=3.6, Lz=4)
     # the CPU-runnable workload: SC 20^3, 1000 steps at 300 K
-    case("c0_sc20", "sc", 20, 3.54, [CO], steps=1000, snaps=[1000], T=300.0, H=0.0, seed=1, spin_seed=1,
-         slim=True)
+    if want("c0_sc20"):
+        case("c0_sc20", "sc", 20, 3.54, [CO], steps=1000, snaps=[1000], T=300.0, H=0.0, seed=1, spin_seed=1,
+             slim=True)
+    # macrocell dipole field, tensor solver: SC 10x10x6 block, open boundaries, two materials in
+    # layers (1.72 and 2.5 mu_B), cells of 2x2x2 unit cells, field refreshed every 10 steps
+    da = dict(CO, **{"maximum-height": 0.5, "exchange-matrix[2]": 6.064e-21})
+    db = dict(CO, **{"material-name": "Co2", "atomic-spin-moment": "2.5 !muB", "minimum-height": 0.5,
+                     "exchange-matrix[2]": 6.064e-21})
+    if want("dipole_sc10"):
+        dipole_case("dipole_sc10", 10, 6, 3.54, [da, db], cell_size=7.08, rate=10, steps=30,
+                    snaps=[1, 9, 10, 11, 30], T=300.0)
     # Metropolis M(T) of the reference random-site algorithm, SC 8^3, no anisotropy
     mcm = dict(CO, **{"second-order-uniaxial-anisotropy-constant": 0.0, "initial-spin-direction": "0,0,1"})
-    mc_case("mc_sc8", 8, 3.54, [mcm], temps=[100.0, 300.0, 500.0, 600.0, 700.0, 900.0],
-            sweeps=6000, every=5, equil=1000)
+    if want("mc_sc8"):
+        mc_case("mc_sc8", 8, 3.54, [mcm], temps=[100.0, 300.0, 500.0, 600.0, 700.0, 900.0],
+                sweeps=6000, every=5, equil=1000)
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))

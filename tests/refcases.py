@@ -23,6 +23,7 @@ CASES = {
                                           kc_vector2=(-1.0, 1.0, 0.0))], ani=dict(cubic_rotated=True)),
     "film_dmi": dict(mats=[S.Material(mu_s_muB=1.6, alpha=0.1), S.Material(mu_s_muB=7.6, alpha=0.1)], ani={}),
     "c0_sc20": dict(mats=[S.Material(**CO)], ani=dict(uniaxial=True)),
+    "dipole_sc10": dict(mats=[S.Material(**CO), S.Material(**dict(CO, mu_s_muB=2.5))], ani=dict(uniaxial=True)),
 }
 
 
@@ -48,4 +49,13 @@ def system_of(name: str) -> S.System:
 
 def conditions(name: str):
     f = load(name)
+    if "H" not in f.files:
+        return float(f["T"]), np.zeros(3)
     return float(f["T"]), np.array(S.applied_field(float(f["H"]), f["Hdir"]))
+
+
+def dipole_args(name: str, spins_key: str = "spins_in"):
+    """Arguments of Oracle.set_dipole / Context.set_dipole for a dipole fixture."""
+    f = load(name)
+    return dict(spins=f[spins_key], num_cells=int(f["num_cells"]), local_cell=f["local_cell"],
+                atom_cell=f["atom_cell"], volume=f["volume"], tensor=f["tensor"], update_rate=int(f["update_rate"]))

@@ -98,6 +98,26 @@ def test_dropin_applied_field_physics(gpu, exe, tmp_path):
     assert np.abs(spins[:3] - exp).max() < 1e-6
 
 
+def test_dropin_dipole_equals_reference_cpu(gpu, exe, tmp_path):
+    """Macrocell dipole field (tensor solver, 2x2x2-cell macrocells, refreshed every 10 steps by
+    dipole::calculate_field on the host and handed to the device through
+    gpu::transfer_dipole_fields_from_cpu_to_gpu): SC 10x10x6 block with open boundaries, T = 0,
+    0.5 T, 200 steps.  Device path == reference CPU (<= 1e-10); the same deck without the dipole
+    field ends more than 1e-6 away, so the field is in the comparison."""
+    head = ("create:crystal-structure=sc\ndimensions:unit-cell-size = 3.54 !A\n"
+            "dimensions:system-size-x = 35.39 !A\ndimensions:system-size-y = 35.39 !A\n"
+            "dimensions:system-size-z = 21.23 !A\n")
+    dip = "dipole:solver=tensor\ndipole:field-update-rate=10\ncells:macro-cell-size = 7.08 !A\n"
+    mat = CO.format(alpha=0.1, ku=4.644e-24, init="random")
+    kw = dict(T=0.0, inc=100, steps=200, H=0.5, Hdir="0,0,1")
+    cpu, n1 = run_deck(tmp_path, "cpu", head + dip, mat, acc="off", **kw)
+    dev, n2 = run_deck(tmp_path, "gpu", head + dip, mat, acc="on", **kw)
+    nodip, _ = run_deck(tmp_path, "nodip", head, mat, acc="off", **kw)
+    assert n1 == n2 == 600
+    assert np.abs(dev - cpu).max() <= 1e-10
+    assert np.abs(nodip - cpu).max() > 1e-6
+
+
 DECKS = os.path.join(ROOT, "tests", "golden", "decks")
 
 

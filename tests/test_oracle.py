@@ -37,6 +37,63 @@ def test_thermal_noise_is_standard_normal():
     assert abs(np.corrcoef(g, h)[0, 1]) < 0.01
 
 
+# ---------------------------------------------------------------- macrocell dipole field
+def test_dipole_field_equals_reference():
+    """cells::mag, dipole::internal::update_field (tensor solver) and the atom unroll of the
+    reference, bit for bit, for the time-0 field and for a second spin set after an update."""
+    f = RC.load("dipole_sc10")
+    s = RC.system_of("dipole_sc10")
+    o = O.Oracle(s)
+    args = RC.dipole_args("dipole_sc10")
+    o.set_dipole(**args)
+    for k, sp in enumerate(("spins_in", "spins2")):
+        if k:
+            o.dipole_update(f[sp])
+        B, Hd, cm, cB, cH = o.dipole_fields()
+        assert np.array_equal(cm, f[f"cell_mag{k}"])
+        assert np.array_equal(cB, f[f"cell_B{k}"]) and np.array_equal(cH, f[f"cell_Hd{k}"])
+        assert np.array_equal(B, f[f"atom_B{k}"]) and np.array_equal(Hd, f[f"atom_Hd{k}"])
+    # the counts the oracle derives from the cells: magnetic atoms per cell, as the reference's
+    assert np.array_equal(np.bincount(f["atom_cell"], minlength=int(f["num_cells"])), f["cell_count"])
+
+
+def test_dipole_field_is_the_tensor_product():
+    """The same field from an independent dense product: B = 0.9274 T m / mu_B over occupied
+    cells, the demag field adding -(1/2) 8 pi/(3V) m on the diagonal; equal to 1e-12 relative."""
+    f = RC.load("dipole_sc10")
+    nc, lc = int(f["num_cells"]), f["local_cell"]
+    m = f["cell_mag1"] / 9.27400915e-24
+    occ = f["cell_count"] > 0
+    T = f["tensor"]
+    full = np.zeros((3, 3, len(lc), nc))
+    for (a, b), t in zip(((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)), T):
+        full[a, b] = t
+        full[b, a] = t
+    B = 0.927400915 * np.einsum("abij,bj->ai", full[:, :, :, occ], m[:, occ])
+    self_term = -0.5 * 8.0 * np.pi / (3.0 * f["volume"][lc]) * m[:, lc] * 0.927400915
+    scale = np.abs(B).max()
+    assert np.abs(B - f["cell_B1"][:, lc]).max() <= 1e-12 * scale
+    assert np.abs(B + self_term - f["cell_Hd1"][:, lc]).max() <= 1e-12 * scale
+
+
+def test_dipole_trajectory_equals_reference():
+    """sim::LLG_Heun with the dipole field refreshed every 10 steps by sim::increment_time:
+    identical spins at steps 1, 9, 10, 11 (just after the first refresh) and 30."""
+    f = RC.load("dipole_sc10")
+    s = RC.system_of("dipole_sc10")
+    T, H = RC.conditions("dipole_sc10")
+    o = O.Oracle(s)
+    o.set_conditions(T, H)
+    o.set_dipole(**RC.dipole_args("dipole_sc10"))
+    seed = int(f["noise_seed"])
+    cur, last = f["spins_in"], 0
+    for k, st in enumerate(int(v) for v in f["snap_steps"]):
+        noise = np.stack([O.thermal_noise(seed, s.n, q) for q in range(last + 1, st + 1)])
+        cur = o.llg_heun(cur, st - last, noise=noise)
+        last = st
+        assert np.array_equal(cur, f["snaps"][k]), st
+
+
 # ---------------------------------------------------------------- reference golden vectors
 @pytest.mark.parametrize("name", GOLDEN_CASES)
 def test_fields_equal_reference(name):

@@ -104,6 +104,19 @@ class vmx_fit_params(C.Structure):
     ]
 
 
+class vmx_dipole(C.Structure):
+    _fields_ = [
+        ("num_cells", C.c_int32),
+        ("num_local_cells", C.c_int32),
+        ("local_cell", _P(C.c_int32)),
+        ("atom_cell", _P(C.c_int32)),
+        ("magnetic", _P(C.c_int32)),
+        ("volume", _P(C.c_double)),
+        ("tensor", _P(C.c_double)),
+        ("update_rate", C.c_int32),
+    ]
+
+
 _VP = C.c_void_p
 _DP = _P(C.c_double)
 
@@ -146,6 +159,10 @@ SIGNATURES = [
     ("vmx_checkpoint_save", C.c_int, [_VP, C.c_void_p, C.c_uint64]),
     ("vmx_checkpoint_load", C.c_int, [_VP, C.c_void_p, C.c_uint64]),
     ("vmx_get_launch_config", C.c_int, [_VP, _P(C.c_int32), _P(C.c_int32)]),
+    ("vmx_set_dipole", C.c_int, [_VP, _P(vmx_dipole)]),
+    ("vmx_update_dipole", C.c_int, [_VP]),
+    ("vmx_get_dipole_field", C.c_int, [_VP, _DP, _DP, _DP, _DP, _DP, _DP]),
+    ("vmx_set_atom_field", C.c_int, [_VP, _DP, _DP, _DP]),
 ]
 
 _lib = None

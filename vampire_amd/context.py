@@ -206,6 +206,49 @@ class Context:
         L.check(self.lib.vmx_autotune(self.h, int(steps_per_trial), C.byref(v)))
         return int(v.value)
 
+    # --- macrocell dipole field ---------------------------------------------------------------
+    def set_dipole(self, num_cells: int, local_cell, atom_cell, volume, tensor, update_rate: int,
+                   magnetic=None, spins: Optional[np.ndarray] = None) -> None:
+        """Device dipole solver (tensor method) on the reference's cells and cell-cell tensor
+        ([6][len(local_cell)][num_cells]); the field of the current spins (or of `spins`, set
+        first) is formed now and refreshed every `update_rate` Heun steps."""
+        if spins is not None:
+            self.set_spins(spins)
+        lc = np.ascontiguousarray(local_cell, dtype=np.int32)
+        ac = np.ascontiguousarray(atom_cell, dtype=np.int32)
+        vol = np.ascontiguousarray(volume, dtype=np.float64)
+        ten = np.ascontiguousarray(tensor, dtype=np.float64).reshape(-1)
+        if ac.size != self.n or vol.size != num_cells or ten.size != 6 * lc.size * num_cells:
+            raise ValueError("dipole arrays do not match the system and cell counts")
+        mag = None if magnetic is None else np.ascontiguousarray(magnetic, dtype=np.int32)
+        d = L.vmx_dipole(num_cells=int(num_cells), num_local_cells=int(lc.size),
+                         local_cell=lc.ctypes.data_as(C.POINTER(C.c_int32)),
+                         atom_cell=ac.ctypes.data_as(C.POINTER(C.c_int32)),
+                         magnetic=None if mag is None else mag.ctypes.data_as(C.POINTER(C.c_int32)),
+                         volume=L.dptr(vol), tensor=L.dptr(ten), update_rate=int(update_rate))
+        L.check(self.lib.vmx_set_dipole(self.h, C.byref(d)))
+
+    def clear_dipole(self) -> None:
+        L.check(self.lib.vmx_set_dipole(self.h, None))
+
+    def update_dipole(self) -> None:
+        L.check(self.lib.vmx_update_dipole(self.h))
+
+    def dipole_field(self):
+        """(B, mu0 Hd): per-atom [3][n] fields of the last refresh, caller's order."""
+        out = [np.empty(self.n) for _ in range(6)]
+        L.check(self.lib.vmx_get_dipole_field(self.h, *[L.dptr(a) for a in out]))
+        return np.stack(out[:3]), np.stack(out[3:])
+
+    def set_atom_field(self, h: Optional[np.ndarray]) -> None:
+        """A per-atom field [3][n] (tesla) added to H_ext in the dipole field's place; None removes it."""
+        if h is None:
+            L.check(self.lib.vmx_set_atom_field(self.h, None, None, None))
+            return
+        hh = np.ascontiguousarray(h, dtype=np.float64).reshape(3, self.n)
+        x, y, z = (np.ascontiguousarray(hh[k]) for k in range(3))
+        L.check(self.lib.vmx_set_atom_field(self.h, L.dptr(x), L.dptr(y), L.dptr(z)))
+
     # --- Monte Carlo -------------------------------------------------------------------------
     def mc_sweep(self, nsweeps: int = 1) -> None:
         L.check(self.lib.vmx_mc_sweep(self.h, int(nsweeps)))

@@ -208,6 +208,77 @@ __global__ void k_mc_adapt(int nrep, const int* __restrict__ rep_off, double* si
     rejects[r] = 0.0;
 }
 
+// ---- macrocell dipole field, tensor solver ------------------------------------------------
+// Cell moments, cells::mag (src/cells/mag.cpp:53-66): one thread per cell sums its magnetic
+// atoms in ascending caller index, the order of the reference's atom loop.
+template <typename R>
+__global__ void k_dip_cell_mag(int nc, const int* __restrict__ start, const int* __restrict__ atoms,
+                               const uint8_t* __restrict__ type, const double* __restrict__ mus,
+                               const R* __restrict__ sx, const R* __restrict__ sy, const R* __restrict__ sz,
+                               double* __restrict__ mag) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double mx = 0.0, my = 0.0, mz = 0.0;
+    for (int a = start[c]; a < start[c + 1]; ++a) {
+        const int i = atoms[a];
+        const double m = mus[type[i]];
+        mx += (double)sx[i] * m;
+        my += (double)sy[i] * m;
+        mz += (double)sz[i] * m;
+    }
+    mag[c] = mx; mag[(size_t)nc + c] = my; mag[2 * (size_t)nc + c] = mz;
+}
+
+// Cell fields, dipole::internal::update_field (src/dipole/update.cpp:52-93): one thread per cell
+// with magnetic atoms, the j loop and every sum in the reference's order.  T holds the tensor as
+// [xx, xy, xz, yy, yz, zz][source cell j][row lc] so that a wavefront reads consecutive rows.
+__global__ void k_dip_cell_field(int nl, int nc, const int* __restrict__ local_cell,
+                                 const int* __restrict__ count, const double* __restrict__ volume,
+                                 const double* __restrict__ mag, const double* __restrict__ T,
+                                 double* __restrict__ B, double* __restrict__ Hd) {
+    const int lc = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lc >= nl) return;
+    const int i = local_cell[lc];
+    if (count[i] <= 0) return;
+    const double imuB = 1.0 / 9.27400915e-24;
+    const size_t ncs = (size_t)nc, ts = (size_t)nc * nl;
+    const double eightPI_three_cell_volume = 8.0 * M_PI / (3.0 * volume[i]);
+    const double self_demag = eightPI_three_cell_volume;
+    const double mx_i = mag[i] * imuB, my_i = mag[ncs + i] * imuB, mz_i = mag[2 * ncs + i] * imuB;
+    double bx = self_demag * mx_i * 0.0, by = self_demag * my_i * 0.0, bz = self_demag * mz_i * 0.0;
+    double hx = -0.5 * self_demag * mx_i, hy = -0.5 * self_demag * my_i, hz = -0.5 * self_demag * mz_i;
+    for (int j = 0; j < nc; ++j) {
+        if (count[j] > 0) {
+            const double mx = mag[j] * imuB, my = mag[ncs + j] * imuB, mz = mag[2 * ncs + j] * imuB;
+            const size_t o = (size_t)j * nl + lc;
+            const double txx = T[o], txy = T[ts + o], txz = T[2 * ts + o];
+            const double tyy = T[3 * ts + o], tyz = T[4 * ts + o], tzz = T[5 * ts + o];
+            bx += (mx * txx + my * txy + mz * txz);
+            by += (mx * txy + my * tyy + mz * tyz);
+            bz += (mx * txz + my * tyz + mz * tzz);
+            hx += (mx * txx + my * txy + mz * txz);
+            hy += (mx * txy + my * tyy + mz * tyz);
+            hz += (mx * txz + my * tyz + mz * tzz);
+        }
+    }
+    B[i] = bx * 9.27400915e-01; B[ncs + i] = by * 9.27400915e-01; B[2 * ncs + i] = bz * 9.27400915e-01;
+    Hd[i] = hx * 9.27400915e-01; Hd[ncs + i] = hy * 9.27400915e-01; Hd[2 * ncs + i] = hz * 9.27400915e-01;
+}
+
+// Atom fields from cell fields (src/dipole/field.cpp:113-125): atoms of non-magnetic materials
+// (cell -1) keep their zero field.
+template <typename R>
+__global__ void k_dip_unroll(int n, int nc, const int* __restrict__ acell, const double* __restrict__ B,
+                             R* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = acell[i];
+    if (c < 0) return;
+    out[i] = (R)B[c];
+    out[(size_t)n + i] = (R)B[(size_t)nc + c];
+    out[2 * (size_t)n + i] = (R)B[2 * (size_t)nc + c];
+}
+
 }  // namespace
 
 // ============================================================================================
@@ -240,6 +311,7 @@ struct vmx_ctx {
     }
     int am() const { return am_of(tp, jm, wsel); }
     std::vector<int> rowh_, colh_;   // host CSR (graph colouring for Monte Carlo)
+    std::vector<int> type_h;         // material ids, caller order
     uint32_t ani = 0, fld = 0;
     double dt = 0.0, half_dt = 0.0;
     uint64_t step = 0;       // host mirror of *step_base
@@ -277,6 +349,38 @@ struct vmx_ctx {
     int nblocks_fused = 0;        // entries of blk_fused (each FusedAtoms * block atoms)
     std::vector<MagStat*> mags;
     std::vector<EnergyStat*> energies;
+
+    // Macrocell dipole field.  hdip ([3][n] R, internal order) is added to H_ext when hdip_on;
+    // with dip.on the device solver refreshes it every dip.rate Heun steps.
+    struct Dipole {
+        int on = 0;
+        int nc = 0, nl = 0, rate = 1;
+        long long time = 0;       // Heun steps since the field was set up
+        DevBuf start, atoms;      // per cell: its magnetic atoms (internal index), ascending caller index
+        DevBuf local_cell, count, volume, tensor, mag, B, Hd, acell, mus;
+        std::vector<int> acell_h; // internal order, -1: no field
+    } dip;
+    int hdip_on = 0;
+    DevBuf hdip;
+    void drop_graph() {
+        if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+        graph_key = -1;
+    }
+    template <typename R>
+    int dipole_refresh() {
+        const Spin3<R> S = spins<R>(s_idx);
+        hipLaunchKernelGGL((k_dip_cell_mag<R>), dim3((dip.nc + 127) / 128), dim3(128), 0, stream, dip.nc,
+                           dip.start.as<int>(), dip.atoms.as<int>(), type.as<uint8_t>(), dip.mus.as<double>(),
+                           S.x, S.y, S.z, dip.mag.as<double>());
+        hipLaunchKernelGGL(k_dip_cell_field, dim3((dip.nl + 63) / 64), dim3(64), 0, stream, dip.nl, dip.nc,
+                           dip.local_cell.as<int>(), dip.count.as<int>(), dip.volume.as<double>(),
+                           dip.mag.as<double>(), dip.tensor.as<double>(), dip.B.as<double>(), dip.Hd.as<double>());
+        hipLaunchKernelGGL((k_dip_unroll<R>), dim3((n + 255) / 256), dim3(256), 0, stream, n, dip.nc,
+                           dip.acell.as<int>(), dip.B.as<double>(), hdip.as<R>());
+        VMX_HIP(hipGetLastError());
+        return VMX_OK;
+    }
 
     // Monte Carlo
     bool coloured = false;
@@ -362,6 +466,7 @@ struct vmx_ctx {
         s.rep_off = rep_off.as<int>();
         s.rep_key = rep_key.as<uint2>();
         s.step_base = step_base.as<unsigned long long>();
+        s.hdip = hdip_on ? hdip.as<R>() : nullptr;
         s.ani = ani; s.fld = fld;
         s.dt = (R)dt; s.half_dt = (R)half_dt;
         return s;
@@ -422,9 +527,7 @@ struct vmx_ctx {
         nblocks = (int)tab.size();
         nblocks_fused = (int)tabf.size();
         // a captured graph holds the tables' addresses: never replay one across a rebuild
-        if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
-        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
-        graph_key = -1;
+        drop_graph();
         VMX_TRY(blk.alloc(tab.size() * sizeof(int4)));
         VMX_HIP(hipMemcpy(blk.p, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice));
         VMX_TRY(blk_fused.alloc(tabf.size() * sizeof(int4)));
@@ -576,6 +679,9 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
             VMX_HIP(mark(5));
             nk = 5;
         }
+        // sim::increment_time then dipole::calculate_field: refresh when the step count since
+        // set-up reaches a multiple of the update rate (src/dipole/field.cpp:52-58)
+        if (dip.on && (dip.time + k + 1) % dip.rate == 0) VMX_TRY(dipole_refresh<R>());
         if (kernel_ms) {
             VMX_HIP(hipEventSynchronize(kev[nk]));
             for (int e = 0; e < nk; ++e) {
@@ -593,6 +699,7 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
                        (unsigned)nsteps);
     VMX_HIP(hipGetLastError());
     step += (uint64_t)nsteps;
+    if (dip.on) dip.time += nsteps;
     return VMX_OK;
 }
 
@@ -1225,6 +1332,7 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
     }
 
     std::unique_ptr<vmx_ctx> c(new vmx_ctx());
+    c->type_h.assign(sys->type, sys->type + n);
     c->q = q;
     c->pi.swap(pi);
     c->inv.swap(inv);
@@ -1418,6 +1526,8 @@ int vmx_llg_heun_graph(vmx_ctx* c, int32_t nsteps, int32_t nrepeat) {
     if (nsteps <= 0 || nsteps % 2) return fail(VMX_ERR_ARG, "graph step count must be positive and even");
     if (nrepeat < 0) return fail(VMX_ERR_ARG, "nrepeat must be >= 0");
     if (c->kernel != VMX_KERNEL_FUSED) return fail(VMX_ERR_STATE, "graphs are recorded from the fused kernels");
+    if (c->dip.on)
+        return fail(VMX_ERR_STATE, "graph replay does not refresh the dipole field; use vmx_llg_heun");
     if (!(c->dt > 0.0)) return fail(VMX_ERR_STATE, "time step not set (vmx_set_time_step)");
     VMX_TRY(c->use_device());
     if (!c->graph_exec || c->graph_key != c->graph_key_of(nsteps)) {
@@ -1473,6 +1583,7 @@ int vmx_checkpoint_save(vmx_ctx* c, void* buf, uint64_t bytes) {
     if (!c || !buf) return fail(VMX_ERR_ARG, "null argument");
     if (bytes < chk_bytes(c)) return fail(VMX_ERR_ARG, "checkpoint buffer of %llu bytes, need %llu",
                                           (unsigned long long)bytes, (unsigned long long)chk_bytes(c));
+    if (c->hdip_on) return fail(VMX_ERR_STATE, "checkpoints do not hold the dipole field; clear it first");
     VMX_TRY(c->use_device());
     char* p = static_cast<char*>(buf);
     auto put = [&](const void* v, size_t b) { std::memcpy(p, v, b); p += b; };
@@ -1535,6 +1646,8 @@ int vmx_checkpoint_load(vmx_ctx* c, const void* buf, uint64_t bytes) {
 int vmx_mc_sweep(vmx_ctx* c, int32_t nsweeps) {
     if (!c) return fail(VMX_ERR_ARG, "null context");
     if (nsweeps < 0) return fail(VMX_ERR_ARG, "nsweeps must be >= 0");
+    if (c->hdip_on)
+        return fail(VMX_ERR_STATE, "Metropolis sweeps do not take the dipole field's energy; clear it first");
     VMX_TRY(c->use_device());
     return c->precision == VMX_FP64 ? c->mc_sweeps<double>(nsweeps) : c->mc_sweeps<float>(nsweeps);
 }
@@ -1629,6 +1742,8 @@ int vmx_stats_add_energy(vmx_ctx* c, int32_t mask_size, const int32_t* mask, int
 
 int vmx_stats_update(vmx_ctx* c) {
     if (!c) return fail(VMX_ERR_ARG, "null context");
+    if (c->hdip_on && !c->energies.empty())
+        return fail(VMX_ERR_STATE, "device energy statistics do not include the magnetostatic energy of the dipole field");
     VMX_TRY(c->use_device());
     return c->precision == VMX_FP64 ? c->stats_update<double>() : c->stats_update<float>();
 }
@@ -1776,6 +1891,7 @@ int vmx_time_kernels(vmx_ctx* c, int32_t nsteps, double* avg_ms, int32_t* nkerne
 int vmx_fit_forward(vmx_ctx* c, const vmx_fit_params* p, const double* h, const double* spins0, int32_t nsteps,
                     int32_t record_every, int32_t checkpoint_every, double* mz) {
     if (!c || !p || !spins0 || !mz) return fail(VMX_ERR_ARG, "null argument");
+    if (c->hdip_on) return fail(VMX_ERR_STATE, "the fitting step has no dipole field; clear it first");
     if (nsteps <= 0 || record_every <= 0 || checkpoint_every <= 0 || nsteps % record_every)
         return fail(VMX_ERR_ARG, "nsteps must be a positive multiple of record_every");
     if (c->nrep != 1 || c->nmat != 1 || c->et != ET_ISO || c->jm != JM_UNIFORM)
@@ -1831,6 +1947,13 @@ int vmx_autotune(vmx_ctx* c, int32_t steps_per_trial, int32_t* chosen) {
     if (steps_per_trial <= 0) return fail(VMX_ERR_ARG, "steps_per_trial must be positive");
     if (!(c->dt > 0.0)) return fail(VMX_ERR_STATE, "time step not set (vmx_set_time_step)");
     VMX_TRY(c->use_device());
+    // the trials read the dipole field but must not refresh it or advance its clock
+    struct PauseDipole {
+        vmx_ctx* c;
+        int on;
+        ~PauseDipole() { c->dip.on = on; }
+    } pause{c, c->dip.on};
+    c->dip.on = 0;
     // save spins and step counter
     const size_t bytes = 3 * (size_t)c->n * c->rsize();
     DevBuf save;
@@ -1865,3 +1988,151 @@ int vmx_autotune(vmx_ctx* c, int32_t steps_per_trial, int32_t* chosen) {
 }
 
 }  // extern "C"
+
+// ---- macrocell dipole field ------------------------------------------------------------------
+int vmx_set_dipole(vmx_ctx* c, const vmx_dipole* d) {
+    g_last_error.clear();
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    VMX_TRY(c->use_device());
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    c->drop_graph();
+    if (!d) {
+        c->dip.on = 0;
+        c->hdip_on = 0;
+        return VMX_OK;
+    }
+    if (c->nrep != 1) return fail(VMX_ERR_STATE, "the dipole field needs a single replica (this context has %d)", c->nrep);
+    const int n = c->n, nc = d->num_cells, nl = d->num_local_cells;
+    if (nc <= 0 || nl <= 0 || nl > nc) return fail(VMX_ERR_ARG, "bad cell counts (%d cells, %d local)", nc, nl);
+    if (!d->local_cell || !d->atom_cell || !d->volume || !d->tensor)
+        return fail(VMX_ERR_ARG, "null dipole array");
+    if (d->update_rate < 1) return fail(VMX_ERR_ARG, "update_rate must be >= 1");
+    const std::vector<int>& type_h = c->type_h;
+    std::vector<int> count(nc, 0);
+    std::vector<std::vector<int>> members(nc);
+    std::vector<int> acell(n, -1);
+    for (int i = 0; i < n; ++i) {                   // caller order: the order cells::mag sums in
+        const int cell = d->atom_cell[i];
+        if (cell < 0 || cell >= nc) return fail(VMX_ERR_ARG, "atom %d in cell %d of %d", i, cell, nc);
+        const int m = type_h[i];
+        if (d->magnetic && !d->magnetic[m]) continue;
+        const int ii = c->q > 1 ? c->pi[i] : i;
+        members[cell].push_back(ii);
+        acell[ii] = cell;
+        count[cell]++;
+    }
+    std::vector<char> is_local(nc, 0);
+    for (int l = 0; l < nl; ++l) {
+        const int cell = d->local_cell[l];
+        if (cell < 0 || cell >= nc || is_local[cell]) return fail(VMX_ERR_ARG, "bad or repeated local cell %d", cell);
+        is_local[cell] = 1;
+    }
+    for (int cell = 0; cell < nc; ++cell)
+        if (count[cell] > 0 && !is_local[cell])
+            return fail(VMX_ERR_ARG, "cell %d holds magnetic atoms but has no tensor row", cell);
+    std::vector<int> start(nc + 1, 0), atoms;
+    atoms.reserve(n);
+    for (int cell = 0; cell < nc; ++cell) {
+        atoms.insert(atoms.end(), members[cell].begin(), members[cell].end());
+        start[cell + 1] = (int)atoms.size();
+    }
+    // tensor [6][nl][nc] (row = local cell) to [6][nc][nl]
+    const size_t ts = (size_t)nl * nc;
+    std::vector<double> tt(6 * ts);
+    for (int k = 0; k < 6; ++k)
+        for (int l = 0; l < nl; ++l)
+            for (int j = 0; j < nc; ++j) tt[k * ts + (size_t)j * nl + l] = d->tensor[k * ts + (size_t)l * nc + j];
+    std::vector<double> mus(c->nmat);
+    for (int m = 0; m < c->nmat; ++m) mus[m] = c->mats[m].mu_s_SI;
+    auto up = [&](DevBuf& b, const void* src, size_t bytes) -> int {
+        VMX_TRY(b.alloc(bytes));
+        VMX_HIP(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+        return VMX_OK;
+    };
+    vmx_ctx::Dipole& D = c->dip;
+    VMX_TRY(up(D.start, start.data(), start.size() * sizeof(int)));
+    VMX_TRY(up(D.atoms, atoms.data(), std::max<size_t>(1, atoms.size()) * sizeof(int)));
+    VMX_TRY(up(D.local_cell, d->local_cell, (size_t)nl * sizeof(int)));
+    VMX_TRY(up(D.count, count.data(), (size_t)nc * sizeof(int)));
+    VMX_TRY(up(D.volume, d->volume, (size_t)nc * sizeof(double)));
+    VMX_TRY(up(D.tensor, tt.data(), tt.size() * sizeof(double)));
+    VMX_TRY(up(D.acell, acell.data(), (size_t)n * sizeof(int)));
+    VMX_TRY(up(D.mus, mus.data(), mus.size() * sizeof(double)));
+    for (DevBuf* b : {&D.mag, &D.B, &D.Hd}) {
+        VMX_TRY(b->alloc(3 * (size_t)nc * sizeof(double)));
+        VMX_HIP(hipMemset(b->p, 0, 3 * (size_t)nc * sizeof(double)));
+    }
+    VMX_TRY(c->hdip.alloc(3 * (size_t)n * c->rsize()));
+    VMX_HIP(hipMemset(c->hdip.p, 0, 3 * (size_t)n * c->rsize()));
+    D.acell_h = acell;
+    D.nc = nc; D.nl = nl; D.rate = d->update_rate; D.time = 0;
+    D.on = 1;
+    c->hdip_on = 1;
+    // the field of the current spins at time 0, as dipole::initialize computes it
+    VMX_TRY(c->precision == VMX_FP64 ? c->dipole_refresh<double>() : c->dipole_refresh<float>());
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    return VMX_OK;
+}
+
+int vmx_update_dipole(vmx_ctx* c) {
+    g_last_error.clear();
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    if (!c->dip.on) return fail(VMX_ERR_STATE, "no dipole field set (vmx_set_dipole)");
+    VMX_TRY(c->use_device());
+    VMX_TRY(c->precision == VMX_FP64 ? c->dipole_refresh<double>() : c->dipole_refresh<float>());
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    return VMX_OK;
+}
+
+int vmx_get_dipole_field(vmx_ctx* c, double* bx, double* by, double* bz, double* hx, double* hy, double* hz) {
+    g_last_error.clear();
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    if (!c->dip.on) return fail(VMX_ERR_STATE, "no dipole field set (vmx_set_dipole)");
+    VMX_TRY(c->use_device());
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    const int n = c->n, nc = c->dip.nc;
+    std::vector<double> B(3 * (size_t)nc), H(3 * (size_t)nc);
+    VMX_HIP(hipMemcpy(B.data(), c->dip.B.p, B.size() * sizeof(double), hipMemcpyDeviceToHost));
+    VMX_HIP(hipMemcpy(H.data(), c->dip.Hd.p, H.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double* out[6] = {bx, by, bz, hx, hy, hz};
+    for (int i = 0; i < n; ++i) {
+        const int cell = c->dip.acell_h[c->q > 1 ? c->pi[i] : i];
+        for (int k = 0; k < 6; ++k) {
+            if (!out[k]) continue;
+            const std::vector<double>& src = k < 3 ? B : H;
+            out[k][i] = cell < 0 ? 0.0 : src[(size_t)(k % 3) * nc + cell];
+        }
+    }
+    return VMX_OK;
+}
+
+int vmx_set_atom_field(vmx_ctx* c, const double* hx, const double* hy, const double* hz) {
+    g_last_error.clear();
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    VMX_TRY(c->use_device());
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    if (!hx && !hy && !hz) {
+        c->drop_graph();
+        c->dip.on = 0;
+        c->hdip_on = 0;
+        return VMX_OK;
+    }
+    if (!hx || !hy || !hz) return fail(VMX_ERR_ARG, "null field component");
+    if (c->dip.on) return fail(VMX_ERR_STATE, "the device dipole solver owns the field (vmx_set_dipole(ctx, NULL) first)");
+    const int n = c->n;
+    std::vector<double> host(3 * (size_t)n), in(3 * (size_t)n);
+    std::copy(hx, hx + n, in.begin());
+    std::copy(hy, hy + n, in.begin() + n);
+    std::copy(hz, hz + n, in.begin() + 2 * (size_t)n);
+    c->to_internal(in.data(), host.data(), 3);
+    VMX_TRY(c->hdip.alloc(3 * (size_t)n * c->rsize()));
+    if (c->precision == VMX_FP64) {
+        VMX_HIP(hipMemcpy(c->hdip.p, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(host.begin(), host.end());
+        VMX_HIP(hipMemcpy(c->hdip.p, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    if (!c->hdip_on) c->drop_graph();
+    c->hdip_on = 1;
+    return VMX_OK;
+}

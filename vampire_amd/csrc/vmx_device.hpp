@@ -97,6 +97,7 @@ struct SysDev {
     const int* __restrict__ rep_off;         // [nrep+1]
     const uint2* __restrict__ rep_key;       // [nrep] Philox key
     const unsigned long long* __restrict__ step_base;  // Heun steps completed before this launch batch
+    const R* __restrict__ hdip;              // [3][n] per-atom dipole field (x, y, z blocks), or null
     uint32_t ani;                            // ANI_* flags
     uint32_t fld;                            // FLD_* flags
     R dt, half_dt;
@@ -445,9 +446,10 @@ __device__ __forceinline__ void row_field(const SysDev<R>& s, const RowIdx<AM>& 
     add_anisotropy(s.ani, mc, sx, sy, sz, hx, hy, hz, fx, fy, fz);
 }
 
-// ---- external field at atom i (thermal + applied), src/simulate/fields.cpp:120-289 ----------
+// ---- external field at atom i (thermal + applied + dipole), src/simulate/fields.cpp:120-310 ---
 // The array starts at 0; the thermal term overwrites it with N(0,1)*sigma (generate, then *=),
-// then the applied field is added.
+// then the applied field is added, and last the macrocell dipole field when one is set
+// (calculate_dipolar_fields, fields.cpp:166, 291-310).
 template <typename R>
 __device__ __forceinline__ void ext_field(const SysDev<R>& s, const Cond<R>& c, int rep, int i, int local,
                                           uint64_t step, const double* __restrict__ noise,
@@ -466,6 +468,9 @@ __device__ __forceinline__ void ext_field(const SysDev<R>& s, const Cond<R>& c, 
     }
     if (s.fld & FLD_APPLIED) {
         ex += c.happ[0]; ey += c.happ[1]; ez += c.happ[2];
+    }
+    if (s.hdip) {
+        ex += s.hdip[i]; ey += s.hdip[(size_t)s.n + i]; ez += s.hdip[2 * (size_t)s.n + i];
     }
 }
 template <typename R>

@@ -139,11 +139,6 @@ void initialize() {
                      "and bilinear exchange only" << std::endl;
         err::vexit();
     }
-    if (dipole::activated) {
-        std::cerr << "Error: the GPU backend does not compute the dipole field" << std::endl;
-        err::vexit();
-    }
-
     vmx_system s{};
     s.num_atoms = n;
     s.num_materials = nmat;
@@ -177,6 +172,8 @@ void initialize() {
     check(vmx_set_spins(g_ctx, atoms::x_spin_array.data(), atoms::y_spin_array.data(), atoms::z_spin_array.data()),
           "vmx_set_spins");
     push_conditions();
+    // dipole::initialize has computed the time-0 field on the host already (sim.cpp:270-295)
+    transfer_dipole_fields_from_cpu_to_gpu();
 
     // statistics computed on the device unless the input asks for CPU statistics
     g_mags.clear();
@@ -212,8 +209,14 @@ void transfer_spin_positions_from_gpu_to_cpu() {
           "vmx_get_spins");
 }
 
+// dipole::calculate_field computes the field on the host from spins copied out by
+// transfer_spin_positions_from_gpu_to_cpu and hands it over here (src/dipole/field.cpp:61-80);
+// the device adds it to H_ext after the thermal and applied fields, as fields.cpp:291-310 does.
 void transfer_dipole_fields_from_cpu_to_gpu() {
-    // the dipole field is not part of this backend; gpu::initialize refuses systems that need it
+    if (!g_ctx || !dipole::activated) return;
+    check(vmx_set_atom_field(g_ctx, dipole::atom_dipolar_field_array_x.data(), dipole::atom_dipolar_field_array_y.data(),
+                             dipole::atom_dipolar_field_array_z.data()),
+          "vmx_set_atom_field");
 }
 
 namespace config {
