@@ -334,15 +334,18 @@ typedef struct vmx_dipole {
     const double* volume;          /* [num_cells] cells::volume_array (A^3), self-demag term      */
     const double* tensor;          /* [6][num_local_cells][num_cells]: rij_tensor_xx, _xy, _xz,
                                       _yy, _yz, _zz [local cell][cell] (dipole/internal.hpp)      */
-    int32_t update_rate;           /* dipole::update_rate: Heun steps between refreshes          */
+    int32_t update_rate;           /* dipole::update_rate: steps (Heun steps or Metropolis
+                                      sweeps) between refreshes                                  */
 } vmx_dipole;
 
 /* Sets the dipole solver and computes the field of the current spins (time 0, as
- * dipole::initialize does, initialize.cpp:143); vmx_llg_heun then refreshes it after every step
- * whose count since this call is a multiple of update_rate (dipole::calculate_field after
- * sim::increment_time, field.cpp:43-71, sim.cpp:191-200).  NULL removes the field.  One replica
- * only; graph replay, Metropolis sweeps, device energy statistics and checkpoints refuse while a
- * dipole field is set. */
+ * dipole::initialize does, initialize.cpp:143); vmx_llg_heun and vmx_mc_sweep then refresh it
+ * after every step or sweep whose count since this call is a multiple of update_rate
+ * (dipole::calculate_field after sim::increment_time, field.cpp:43-71, sim.cpp:191-200).  The
+ * Metropolis energy and the energy statistics take the magnetostatic term -mu0Hd.S
+ * (simulate/energy.cpp:123-125, 171; statistics/energy.cpp:198-208).  NULL removes the field.
+ * One replica only; graph replay, checkpoints and the fitting step refuse while a dipole field
+ * is set. */
 int vmx_set_dipole(vmx_ctx* ctx, const vmx_dipole* dipole);
 
 /* Recomputes the field from the current spins now (e.g. after vmx_set_spins). */
@@ -357,7 +360,8 @@ int vmx_get_dipole_field(vmx_ctx* ctx, double* bx, double* by, double* bz, doubl
 /* A per-atom field (tesla, caller's order) added to H_ext where the dipole field goes, for callers
  * that compute the dipole field themselves: gpu::transfer_dipole_fields_from_cpu_to_gpu
  * (src/gpu/transfer.cpp:39, called from dipole::calculate_field, field.cpp:80) uploads
- * dipole::atom_dipolar_field_array_* with it.  Three NULLs remove it. */
+ * dipole::atom_dipolar_field_array_* with it.  Three NULLs remove it.  It carries no demag field,
+ * so Metropolis sweeps and device energy statistics refuse while it is set. */
 int vmx_set_atom_field(vmx_ctx* ctx, const double* hx, const double* hy, const double* hz);
 
 #ifdef __cplusplus

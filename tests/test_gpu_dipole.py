@@ -112,6 +112,55 @@ def test_uploaded_atom_field_equals_device_solver(gpu, precision):
     assert np.array_equal(out[0], out[1])
 
 
+def test_dipole_metropolis_matches_oracle(gpu):
+    """FP64 checkerboard sweeps with the device dipole field: the single-spin energy takes
+    -mu0Hd.S last (sim::calculate_spin_energy) and the field is refreshed every 10 sweeps, as
+    sim::increment_time does after a Monte Carlo step; 25 sweeps equal the oracle's colour-ordered
+    Metropolis to 1e-12, with the same rejection count and cone width."""
+    f = RC.load(NAME)
+    s = RC.system_of(NAME)
+    o = O.Oracle(s)
+    o.set_conditions(300.0, (0.0, 0.0, 0.2))
+    o.set_dipole(**RC.dipole_args(NAME))
+    ref, sigma, moves, rej = o.mc_checkerboard(f["spins_in"], 25, seed=77, sweep0=0, sigma=60.0)
+    with Context(s, precision=FP64, seed=77) as c:
+        c.set_conditions(300.0, (0.0, 0.0, 0.2))
+        c.set_dipole(**_dipole_kwargs(f), spins=f["spins_in"])
+        c.mc_sweep(25)
+        dev = c.get_spins()
+        dsig, dmov, drej = c.mc_state()
+        B, _ = c.dipole_field()
+    assert np.abs(dev - ref).max() <= 1e-12
+    assert dmov[0] == moves and drej[0] == rej
+    assert abs(dsig[0] - sigma) <= 1e-12 * sigma
+    # both refreshed from their sweep-20 spins
+    assert np.abs(B - o.dipole_fields()[0]).max() <= 1e-12 * np.abs(B).max()
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_dipole_energy_statistics(gpu, precision):
+    """Energy statistics with the device dipole field: the magnetostatic row,
+    1/2 sum -mu0Hd.S mu per mask (statistics/energy.cpp:198-208), is non-zero and equals the
+    oracle's, as do the total and the other terms (1e-12 FP64, 1e-6 FP32, relative)."""
+    f = RC.load(NAME)
+    s = RC.system_of(NAME)
+    T, H = RC.conditions(NAME)
+    mask = np.ascontiguousarray(s.type, dtype=np.int32)     # one mask per material, id 2 empty
+    o = O.Oracle(s)
+    o.set_conditions(T, H)
+    o.set_dipole(**RC.dipole_args(NAME))
+    re = o.energy(f["spins_in"], mask, 3)
+    with Context(s, precision=precision, seed=3) as c:
+        c.set_conditions(T, H)
+        c.set_dipole(**_dipole_kwargs(f), spins=f["spins_in"])
+        he = c.add_energy(mask, 3)
+        c.stats_update()
+        ecur, _, _ = c.energy(he)
+    assert np.abs(re[4, :2]).min() > 0
+    tol = 1e-12 if precision == FP64 else 1e-6
+    np.testing.assert_allclose(ecur, re, rtol=tol, atol=tol * np.abs(re).max())
+
+
 def test_dipole_refusals(gpu):
     """Paths that would not refresh or account for the field refuse with an error; clearing the
     field restores them."""
@@ -123,8 +172,6 @@ def test_dipole_refusals(gpu):
         with pytest.raises(VmxError, match="dipole"):
             c.llg_heun_graph(10, 1)
         with pytest.raises(VmxError, match="dipole"):
-            c.mc_sweep(1)
-        with pytest.raises(VmxError, match="dipole"):
             c.checkpoint()
         with pytest.raises(VmxError, match="dipole solver"):
             c.set_atom_field(f["atom_B0"])
@@ -133,6 +180,15 @@ def test_dipole_refusals(gpu):
         with pytest.raises(VmxError, match="cell"):
             c.set_dipole(**bad)
         c.clear_dipole()
-        c.mc_sweep(1)
         c.llg_heun_graph(10, 1)
         assert len(c.checkpoint()) > 0
+        # an uploaded field has no demag field: Metropolis and device energies refuse
+        c.set_atom_field(f["atom_B0"])
+        with pytest.raises(VmxError, match="demag"):
+            c.mc_sweep(1)
+        c.add_energy(np.zeros(s.n, np.int32), 2)
+        with pytest.raises(VmxError, match="demag"):
+            c.stats_update()
+        c.set_atom_field(None)
+        c.mc_sweep(1)
+        c.stats_update()

@@ -160,18 +160,18 @@ __global__ void k_mag_finish(int nblk, const double* __restrict__ partial, int n
 }
 
 // Energies [5][nbins]: total, exchange, anisotropy, applied, magnetostatic
-// (src/statistics/energy.cpp:141-258; no dipole field, so magnetostatic is zero).
+// (src/statistics/energy.cpp:141-258; the magnetostatic term is zero without a device dipole field).
 __global__ void k_energy_finish(int nblk, const double* __restrict__ partial, int nbins, double* cur,
                                 double* mean, const int* __restrict__ empty, int nempty,
                                 long long* counter) {
-    const int W = 3 * nbins;
+    const int W = 4 * nbins;
     for (int k = threadIdx.x; k < W; k += blockDim.x) {
         double acc = 0.0;
         for (int b = 0; b < nblk; ++b) acc += partial[(size_t)b * W + k];
-        const int id = k / 3, term = k % 3;     // term: 0 exchange, 1 anisotropy, 2 applied
+        const int id = k / 4, term = k % 4;     // term: exchange, anisotropy, applied, magnetostatic
         cur[(size_t)(1 + term) * nbins + id] = acc;
     }
-    for (int id = threadIdx.x; id < nbins; id += blockDim.x) { cur[id] = 0.0; cur[(size_t)4 * nbins + id] = 0.0; }
+    for (int id = threadIdx.x; id < nbins; id += blockDim.x) cur[id] = 0.0;
     __threadfence_block();
     __syncthreads();
     const int ms = nbins - 1;
@@ -269,7 +269,7 @@ __global__ void k_dip_cell_field(int nl, int nc, const int* __restrict__ local_c
 // (cell -1) keep their zero field.
 template <typename R>
 __global__ void k_dip_unroll(int n, int nc, const int* __restrict__ acell, const double* __restrict__ B,
-                             R* __restrict__ out) {
+                             const double* __restrict__ Hd, R* __restrict__ out, double* __restrict__ hd) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int c = acell[i];
@@ -277,6 +277,9 @@ __global__ void k_dip_unroll(int n, int nc, const int* __restrict__ acell, const
     out[i] = (R)B[c];
     out[(size_t)n + i] = (R)B[(size_t)nc + c];
     out[2 * (size_t)n + i] = (R)B[2 * (size_t)nc + c];
+    hd[i] = Hd[c];
+    hd[(size_t)n + i] = Hd[(size_t)nc + c];
+    hd[2 * (size_t)n + i] = Hd[2 * (size_t)nc + c];
 }
 
 }  // namespace
@@ -355,9 +358,10 @@ struct vmx_ctx {
     struct Dipole {
         int on = 0;
         int nc = 0, nl = 0, rate = 1;
-        long long time = 0;       // Heun steps since the field was set up
+        long long time = 0;       // Heun steps and Metropolis sweeps since the field was set up
         DevBuf start, atoms;      // per cell: its magnetic atoms (internal index), ascending caller index
         DevBuf local_cell, count, volume, tensor, mag, B, Hd, acell, mus;
+        DevBuf hd_atom;           // [3][n] doubles, internal order: mu0 Hd for the energies
         std::vector<int> acell_h; // internal order, -1: no field
     } dip;
     int hdip_on = 0;
@@ -377,7 +381,8 @@ struct vmx_ctx {
                            dip.local_cell.as<int>(), dip.count.as<int>(), dip.volume.as<double>(),
                            dip.mag.as<double>(), dip.tensor.as<double>(), dip.B.as<double>(), dip.Hd.as<double>());
         hipLaunchKernelGGL((k_dip_unroll<R>), dim3((n + 255) / 256), dim3(256), 0, stream, n, dip.nc,
-                           dip.acell.as<int>(), dip.B.as<double>(), hdip.as<R>());
+                           dip.acell.as<int>(), dip.B.as<double>(), dip.Hd.as<double>(), hdip.as<R>(),
+                           dip.hd_atom.as<double>());
         VMX_HIP(hipGetLastError());
         return VMX_OK;
     }
@@ -467,6 +472,7 @@ struct vmx_ctx {
         s.rep_key = rep_key.as<uint2>();
         s.step_base = step_base.as<unsigned long long>();
         s.hdip = hdip_on ? hdip.as<R>() : nullptr;
+        s.hd = dip.on ? dip.hd_atom.as<double>() : nullptr;
         s.ani = ani; s.fld = fld;
         s.dt = (R)dt; s.half_dt = (R)half_dt;
         return s;
@@ -756,7 +762,7 @@ int vmx_ctx::stats_update() {
             VMX_HIP(hipMemcpy(atom_rep.p, ar.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
         }
         for (EnergyStat* st : energies) {
-            VMX_TRY(partial.alloc((size_t)grid * 3 * st->nbins * sizeof(double)));
+            VMX_TRY(partial.alloc((size_t)grid * 4 * st->nbins * sizeof(double)));
             VMX_HIP(Ops<R>::energy_partial(grid, stream, et, am(),s, S, st->mask.as<int>(), mm.as<double>(),
                                            nrep > 1 ? atom_rep.as<int>() : nullptr, st->nbins,
                                            partial.as<double>()));
@@ -837,6 +843,8 @@ int vmx_ctx::mc_sweeps(int nsweeps) {
                            mc_tot_rej.as<double>());
         VMX_HIP(hipGetLastError());
         ++sweep;
+        // a sweep advances sim::time like a Heun step (sim.cpp:608-615, 191-200)
+        if (dip.on && ++dip.time % dip.rate == 0) VMX_TRY(dipole_refresh<R>());
     }
     return VMX_OK;
 }
@@ -1646,8 +1654,9 @@ int vmx_checkpoint_load(vmx_ctx* c, const void* buf, uint64_t bytes) {
 int vmx_mc_sweep(vmx_ctx* c, int32_t nsweeps) {
     if (!c) return fail(VMX_ERR_ARG, "null context");
     if (nsweeps < 0) return fail(VMX_ERR_ARG, "nsweeps must be >= 0");
-    if (c->hdip_on)
-        return fail(VMX_ERR_STATE, "Metropolis sweeps do not take the dipole field's energy; clear it first");
+    if (c->hdip_on && !c->dip.on)
+        return fail(VMX_ERR_STATE, "an uploaded dipole field has no demag field for the Metropolis energy; "
+                                   "use vmx_set_dipole or clear it");
     VMX_TRY(c->use_device());
     return c->precision == VMX_FP64 ? c->mc_sweeps<double>(nsweeps) : c->mc_sweeps<float>(nsweeps);
 }
@@ -1742,8 +1751,9 @@ int vmx_stats_add_energy(vmx_ctx* c, int32_t mask_size, const int32_t* mask, int
 
 int vmx_stats_update(vmx_ctx* c) {
     if (!c) return fail(VMX_ERR_ARG, "null context");
-    if (c->hdip_on && !c->energies.empty())
-        return fail(VMX_ERR_STATE, "device energy statistics do not include the magnetostatic energy of the dipole field");
+    if (c->hdip_on && !c->dip.on && !c->energies.empty())
+        return fail(VMX_ERR_STATE, "an uploaded dipole field has no demag field for the magnetostatic energy; "
+                                   "use vmx_set_dipole or clear it");
     VMX_TRY(c->use_device());
     return c->precision == VMX_FP64 ? c->stats_update<double>() : c->stats_update<float>();
 }
@@ -2064,6 +2074,8 @@ int vmx_set_dipole(vmx_ctx* c, const vmx_dipole* d) {
     }
     VMX_TRY(c->hdip.alloc(3 * (size_t)n * c->rsize()));
     VMX_HIP(hipMemset(c->hdip.p, 0, 3 * (size_t)n * c->rsize()));
+    VMX_TRY(D.hd_atom.alloc(3 * (size_t)n * sizeof(double)));
+    VMX_HIP(hipMemset(D.hd_atom.p, 0, 3 * (size_t)n * sizeof(double)));
     D.acell_h = acell;
     D.nc = nc; D.nl = nl; D.rate = d->update_rate; D.time = 0;
     D.on = 1;

@@ -98,6 +98,7 @@ struct SysDev {
     const uint2* __restrict__ rep_key;       // [nrep] Philox key
     const unsigned long long* __restrict__ step_base;  // Heun steps completed before this launch batch
     const R* __restrict__ hdip;              // [3][n] per-atom dipole field (x, y, z blocks), or null
+    const double* __restrict__ hd;           // [3][n] per-atom mu0 Hd of the device dipole solver, or null
     uint32_t ani;                            // ANI_* flags
     uint32_t fld;                            // FLD_* flags
     R dt, half_dt;
@@ -907,24 +908,32 @@ __device__ __forceinline__ R applied_energy(const Cond<R>& c, R sx, R sy, R sz) 
     return -(c.hglob[0] * sx + c.hglob[1] * sy + c.hglob[2] * sz);
 }
 
-// W = 3: exchange, anisotropy, applied, each times m_i (energy.cpp:155-196)
+// Magnetostatic single-spin energy -mu0Hd . S (src/dipole/energy.cpp; simulate/energy.cpp:123-125)
+template <typename R>
+__device__ __forceinline__ R magnetostatic_energy(const SysDev<R>& s, int i, R sx, R sy, R sz) {
+    const size_t n = (size_t)s.n;
+    return R(-1.0) * ((R)s.hd[i] * sx + (R)s.hd[n + i] * sy + (R)s.hd[2 * n + i] * sz);
+}
+
+// W = 4: exchange, anisotropy, applied, magnetostatic, each times m_i (energy.cpp:155-203); the
+// last is zero without a device dipole field
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(256)
 k_energy_partial(SysDev<R> s, Spin3<R> S, const int* __restrict__ mask, const double* __restrict__ mm,
                  const int* __restrict__ atom_rep, int nbins, double* __restrict__ partial) {
     extern __shared__ double hist[];
-    for (int k = threadIdx.x; k < 3 * nbins; k += blockDim.x) hist[k] = 0.0;
+    for (int k = threadIdx.x; k < 4 * nbins; k += blockDim.x) hist[k] = 0.0;
     __syncthreads();
     int cur = -1;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
         const int id = mask[i];
         if (id != cur) {
             if (cur >= 0) {
-                atomicAdd(&hist[3 * cur + 0], a0); atomicAdd(&hist[3 * cur + 1], a1);
-                atomicAdd(&hist[3 * cur + 2], a2);
+                atomicAdd(&hist[4 * cur + 0], a0); atomicAdd(&hist[4 * cur + 1], a1);
+                atomicAdd(&hist[4 * cur + 2], a2); atomicAdd(&hist[4 * cur + 3], a3);
             }
-            cur = id; a0 = a1 = a2 = 0.0;
+            cur = id; a0 = a1 = a2 = a3 = 0.0;
         }
         const R sx = S.x[i], sy = S.y[i], sz = S.z[i];
         const int m = s.type[i];
@@ -933,14 +942,15 @@ k_energy_partial(SysDev<R> s, Spin3<R> S, const int* __restrict__ mask, const do
         a0 += (double)exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, sx, sy, sz) * w;
         a1 += (double)anisotropy_energy(s, m, sx, sy, sz) * w;
         a2 += (double)applied_energy(s.cond[rep * s.nmat + m], sx, sy, sz) * w;
+        if (s.hd) a3 += (double)magnetostatic_energy(s, i, sx, sy, sz) * w;
     }
     if (cur >= 0) {
-        atomicAdd(&hist[3 * cur + 0], a0); atomicAdd(&hist[3 * cur + 1], a1);
-        atomicAdd(&hist[3 * cur + 2], a2);
+        atomicAdd(&hist[4 * cur + 0], a0); atomicAdd(&hist[4 * cur + 1], a1);
+        atomicAdd(&hist[4 * cur + 2], a2); atomicAdd(&hist[4 * cur + 3], a3);
     }
     __syncthreads();
-    double* out = partial + (size_t)blockIdx.x * 3 * nbins;
-    for (int k = threadIdx.x; k < 3 * nbins; k += blockDim.x) out[k] = hist[k];
+    double* out = partial + (size_t)blockIdx.x * 4 * nbins;
+    for (int k = threadIdx.x; k < 4 * nbins; k += blockDim.x) out[k] = hist[k];
 }
 
 // ======================= checkerboard Metropolis =============================================
@@ -982,10 +992,14 @@ k_mc_colour(SysDev<R> s, Spin3<R> S, const int* __restrict__ list, int count, in
         R nx = ox + g0 * angle, ny = oy + g1 * angle, nz = oz + g2 * angle;
         const R r = inv_norm(nx, ny, nz);
         nx *= r; ny *= r; nz *= r;
-        const R e_old = (R(0) + exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, ox, oy, oz)
-                         + anisotropy_energy(s, m, ox, oy, oz)) + applied_energy(c, ox, oy, oz);
-        const R e_new = (R(0) + exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, nx, ny, nz)
-                         + anisotropy_energy(s, m, nx, ny, nz)) + applied_energy(c, nx, ny, nz);
+        R e_old = (R(0) + exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, ox, oy, oz)
+                   + anisotropy_energy(s, m, ox, oy, oz)) + applied_energy(c, ox, oy, oz);
+        R e_new = (R(0) + exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, nx, ny, nz)
+                   + anisotropy_energy(s, m, nx, ny, nz)) + applied_energy(c, nx, ny, nz);
+        if (s.hd) {                        // last term of sim::calculate_spin_energy (energy.cpp:171)
+            e_old += magnetostatic_energy(s, i, ox, oy, oz);
+            e_new += magnetostatic_energy(s, i, nx, ny, nz);
+        }
         const R de = (e_new - e_old) * s.mat[m].mu_s_SI * R(1.07828231e23);
         bool accept = de < R(0);
         if (!accept) {
