@@ -201,6 +201,13 @@ int vmx_set_conditions(vmx_ctx* ctx, int32_t replica, double temperature, const 
  *   used to feed the device exactly the stream a CPU run consumed. */
 int vmx_llg_heun(vmx_ctx* ctx, int32_t nsteps, const double* noise);
 
+/* nsteps LLG midpoint steps (sim::LLG_Midpoint, src/simulate/LLGMidpoint.cpp:87-186; integrator
+ * 2 of sim::integrate_serial, src/simulate/sim.cpp:616-621): fields at S, Cayley rotation about
+ * F = H + alpha S x H to the unnormalised midpoint, spin field at the midpoint with the same
+ * external field, rotation of S.  Noise, step counter and dipole refresh as vmx_llg_heun.  The
+ * reference has no device version (LLGMidpoint.cpp:212-216). */
+int vmx_llg_midpoint(vmx_ctx* ctx, int32_t nsteps, const double* noise);
+
 /* Records nsteps Heun steps of the fused kernels into a HIP graph once and replays it
  * nrepeat times (launch-bound small systems and replica batches).  Noise is Philox. */
 int vmx_llg_heun_graph(vmx_ctx* ctx, int32_t nsteps, int32_t nrepeat);

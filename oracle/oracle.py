@@ -69,6 +69,8 @@ def load():
         lib.or_llg_heun.argtypes = [_P(or_sys), _P(or_cond), _DP, _DP, _DP, C.c_int, _DP, C.c_uint64,
                                     C.c_uint64, C.c_int]
         lib.or_llg_heun.restype = C.c_int
+        lib.or_llg_midpoint.argtypes = lib.or_llg_heun.argtypes
+        lib.or_llg_midpoint.restype = C.c_int
         lib.or_magnetization.argtypes = [C.c_int, _DP, _DP, _DP, _DP, _IP, C.c_int, _DP]
         lib.or_energy.argtypes = [_P(or_sys), _P(or_cond), _DP, _DP, _DP, _IP, C.c_int, _DP]
         lib.or_spin_energy.argtypes = [_P(or_sys), _P(or_cond), _DP, _DP, _DP, C.c_int, C.c_double, C.c_double,
@@ -201,6 +203,16 @@ class Oracle:
         nz = None if noise is None else np.ascontiguousarray(noise, dtype=np.float64).reshape(-1)
         rc = self.lib.or_llg_heun(C.byref(self.sys), C.byref(self.cond), *[_d(a) for a in sp], int(nsteps),
                                   None if nz is None else _d(nz), seed, step0, threads)
+        if rc != 0:
+            raise MemoryError("oracle allocation failed")
+        return np.stack(sp)
+
+    def llg_midpoint(self, spins: np.ndarray, nsteps: int, noise: Optional[np.ndarray] = None, seed: int = 0,
+                     step0: int = 0, threads: int = 1) -> np.ndarray:
+        sp = [np.ascontiguousarray(spins[k], dtype=np.float64).copy() for k in range(3)]
+        nz = None if noise is None else np.ascontiguousarray(noise, dtype=np.float64).reshape(-1)
+        rc = self.lib.or_llg_midpoint(C.byref(self.sys), C.byref(self.cond), *[_d(a) for a in sp], int(nsteps),
+                                      None if nz is None else _d(nz), seed, step0, threads)
         if rc != 0:
             raise MemoryError("oracle allocation failed")
         return np.stack(sp)

@@ -2,7 +2,8 @@
 // oracle/ref/Makefile) to produce golden vectors for the oracle.  Development container only.
 //
 //   ref_harness <rundir> dump                       system after cs::create + sim::run set-up
-//   ref_harness <rundir> llg <nsteps> <every>       sim::LLG_Heun with injected noise
+//   ref_harness <rundir> llg <nsteps> <every> [mid] sim::LLG_Heun (sim::LLG_Midpoint with `mid`) with
+//                                                   injected noise
 //   ref_harness <rundir> mc <nsweeps> <T> <every>   montecarlo::mc_step, reference RNG
 //   ref_harness <rundir> dipole                     macrocell dipole set-up and fields (dipole.bin)
 //
@@ -239,6 +240,7 @@ int main(int argc, char** argv) {
     if (cmd == "llg") {
         const int nsteps = std::atoi(argv[3]);
         const int every = std::atoi(argv[4]);
+        const bool midpoint = argc > 5 && std::string(argv[5]) == "mid";
         std::vector<double> noise = read_doubles("noise.bin");
         // one extra 3N block at the front feeds the field dump before the first step
         if (noise.size() != (size_t)(nsteps + 1) * 3 * n) {
@@ -253,7 +255,8 @@ int main(int argc, char** argv) {
         }
         std::ofstream tf("traj.bin", std::ios::binary);
         for (int s = 1; s <= nsteps; ++s) {
-            sim::LLG_Heun();
+            if (midpoint) sim::LLG_Midpoint();
+            else sim::LLG_Heun();
             // sim::increment_time refreshes the dipole field after each step (sim.cpp:191-200)
             dipole::calculate_field(s, atoms::x_spin_array, atoms::y_spin_array, atoms::z_spin_array);
             if (s % every == 0) {

@@ -20,6 +20,7 @@
  *                         calculate_thermal_fields         fields.cpp:246-289
  *                         calculate_applied_fields         fields.cpp:171-244
  *   or_llg_heun           sim::LLG_Heun                    src/simulate/LLGHeun.cpp:160-282
+ *   or_llg_midpoint       sim::LLG_Midpoint                src/simulate/LLGMidpoint.cpp:87-186
  *   or_magnetization      magnetization_statistic_t::calculate_magnetization
  *                                                          src/statistics/magnetization.cpp:134-180
  *   or_energy             energy_statistic_t::calculate    src/statistics/energy.cpp:141-258
@@ -490,6 +491,78 @@ int or_llg_heun(const or_sys* s, const or_cond* c, double* sx, double* sy, doubl
     set_threads(nthreads);
     for (int k = 0; k < nsteps; ++k) {
         heun_step(s, c, sx, sy, sz, noise ? noise + (size_t)k * 3 * s->n : NULL, seed, step0 + (uint64_t)k, &a);
+        dipole_tick(s, sx, sy, sz);
+    }
+    free_arrays(&a);
+    return 0;
+}
+
+/* ---- integrator: sim::LLG_Midpoint ------------------------------------------------------- */
+static void midpoint_step(const or_sys* s, const or_cond* c, double* sx, double* sy, double* sz,
+                          const double* noise, uint64_t seed, uint64_t step, heun_arrays* a) {
+    const int num_atoms = s->n;
+    for (int atom = 0; atom < num_atoms; atom++) {             /* LLGMidpoint.cpp:97-101 */
+        a->x0[atom] = sx[atom]; a->y0[atom] = sy[atom]; a->z0[atom] = sz[atom];
+    }
+    or_spin_fields(s, sx, sy, sz, a->hsx, a->hsy, a->hsz);      /* :104 */
+    or_external_fields(s, c, noise, seed, step, a->hex, a->hey, a->hez);   /* :105 */
+#pragma omp parallel for schedule(static)
+    for (int atom = 0; atom < num_atoms; atom++) {             /* :108-141 */
+        const or_material* m = &s->mat[s->type[atom]];
+        const double alpha = m->alpha;
+        const double beta = -1.0 * c->dt * m->one_oneplusalpha_sq * 0.5;
+        const double beta2 = beta * beta;
+        const double S[3] = {sx[atom], sy[atom], sz[atom]};
+        const double H[3] = {a->hsx[atom] + a->hex[atom], a->hsy[atom] + a->hey[atom], a->hsz[atom] + a->hez[atom]};
+        const double F[3] = {H[0] + alpha * (S[1] * H[2] - S[2] * H[1]),
+                             H[1] + alpha * (S[2] * H[0] - S[0] * H[2]),
+                             H[2] + alpha * (S[0] * H[1] - S[1] * H[0])};
+        const double FdotF = F[0] * F[0] + F[1] * F[1] + F[2] * F[2];
+        const double beta2FdotS = beta2 * (F[0] * S[0] + F[1] * S[1] + F[2] * S[2]);
+        const double one_o_one_plus_beta2FdotF = 1.0 / (1.0 + beta2 * FdotF);
+        const double one_minus_beta2FdotF = 1.0 - beta2 * FdotF;
+        a->xs[atom] = (S[0] + one_o_one_plus_beta2FdotF * (S[0] * one_minus_beta2FdotF +
+                                                          2.0 * (beta * (F[1] * S[2] - F[2] * S[1]) + F[0] * beta2FdotS))) * 0.5;
+        a->ys[atom] = (S[1] + one_o_one_plus_beta2FdotF * (S[1] * one_minus_beta2FdotF +
+                                                          2.0 * (beta * (F[2] * S[0] - F[0] * S[2]) + F[1] * beta2FdotS))) * 0.5;
+        a->zs[atom] = (S[2] + one_o_one_plus_beta2FdotF * (S[2] * one_minus_beta2FdotF +
+                                                          2.0 * (beta * (F[0] * S[1] - F[1] * S[0]) + F[2] * beta2FdotS))) * 0.5;
+    }
+    for (int atom = 0; atom < num_atoms; atom++) {             /* :144-148 */
+        sx[atom] = a->xs[atom]; sy[atom] = a->ys[atom]; sz[atom] = a->zs[atom];
+    }
+    or_spin_fields(s, sx, sy, sz, a->hsx, a->hsy, a->hsz);      /* :151 */
+#pragma omp parallel for schedule(static)
+    for (int atom = 0; atom < num_atoms; atom++) {             /* :154-183 */
+        const or_material* m = &s->mat[s->type[atom]];
+        const double alpha = m->alpha;
+        const double beta = -1.0 * c->dt * m->one_oneplusalpha_sq * 0.5;
+        const double beta2 = beta * beta;
+        const double M[3] = {sx[atom], sy[atom], sz[atom]};
+        const double S[3] = {a->x0[atom], a->y0[atom], a->z0[atom]};
+        const double H[3] = {a->hsx[atom] + a->hex[atom], a->hsy[atom] + a->hey[atom], a->hsz[atom] + a->hez[atom]};
+        const double F[3] = {H[0] + alpha * (M[1] * H[2] - M[2] * H[1]),
+                             H[1] + alpha * (M[2] * H[0] - M[0] * H[2]),
+                             H[2] + alpha * (M[0] * H[1] - M[1] * H[0])};
+        const double FdotF = F[0] * F[0] + F[1] * F[1] + F[2] * F[2];
+        const double beta2FdotS = beta2 * (F[0] * S[0] + F[1] * S[1] + F[2] * S[2]);
+        const double one_o_one_plus_beta2FdotF = 1.0 / (1.0 + beta2 * FdotF);
+        const double one_minus_beta2FdotF = 1.0 - beta2 * FdotF;
+        sx[atom] = one_o_one_plus_beta2FdotF * (S[0] * one_minus_beta2FdotF + 2.0 * (beta * (F[1] * S[2] - F[2] * S[1]) + F[0] * beta2FdotS));
+        sy[atom] = one_o_one_plus_beta2FdotF * (S[1] * one_minus_beta2FdotF + 2.0 * (beta * (F[2] * S[0] - F[0] * S[2]) + F[1] * beta2FdotS));
+        sz[atom] = one_o_one_plus_beta2FdotF * (S[2] * one_minus_beta2FdotF + 2.0 * (beta * (F[0] * S[1] - F[1] * S[0]) + F[2] * beta2FdotS));
+    }
+}
+
+/* nsteps midpoint steps; noise [nsteps][3][n] or NULL for the Philox stream at steps step0.. */
+int or_llg_midpoint(const or_sys* s, const or_cond* c, double* sx, double* sy, double* sz, int nsteps,
+                    const double* noise, uint64_t seed, uint64_t step0, int nthreads) {
+    heun_arrays a;
+    memset(&a, 0, sizeof a);
+    if (alloc_arrays(&a, s->n)) { free_arrays(&a); return -1; }
+    set_threads(nthreads);
+    for (int k = 0; k < nsteps; ++k) {
+        midpoint_step(s, c, sx, sy, sz, noise ? noise + (size_t)k * 3 * s->n : NULL, seed, step0 + (uint64_t)k, &a);
         dipole_tick(s, sx, sy, sz);
     }
     free_arrays(&a);

@@ -115,9 +115,10 @@ def digest(*arrays):
 
 
 def case(name, crystal, L, a, materials, steps, snaps, T, H=0.0, Hdir=(0, 0, 1), pbc=(True, True, True),
-         seed=11, spin_seed=5, dmi_cutoff=None, Lz=None, slim=False):
+         seed=11, spin_seed=5, dmi_cutoff=None, Lz=None, slim=False, integrator="heun"):
     """slim: store the neighbour list and exchange table as a SHA-256 digest (plus the distinct J
-    values) instead of the arrays; the builder must reproduce them."""
+    values) instead of the arrays; the builder must reproduce them.  integrator: "heun"
+    (sim::LLG_Heun) or "midpoint" (sim::LLG_Midpoint) for the trajectory."""
     with tempfile.TemporaryDirectory(prefix="vref_") as d:
         inp, mat = deck(crystal, L, a, materials, pbc, T, H, Hdir, dmi_cutoff, Lz)
         open(os.path.join(d, "input"), "w").write(inp)
@@ -133,7 +134,7 @@ def case(name, crystal, L, a, materials, steps, snaps, T, H=0.0, Hdir=(0, 0, 1),
         # noise: block for the field dump at step "-1" is Philox step 0, then steps 1..nsteps
         blocks = [O.thermal_noise(seed, n, s) for s in range(steps + 1)]
         np.concatenate([b.reshape(-1) for b in blocks]).tofile(os.path.join(d, "noise.bin"))
-        run(d, "llg", steps, 1)
+        run(d, "llg", steps, 1, *(["mid"] if integrator == "midpoint" else []))
         fields = np.fromfile(os.path.join(d, "fields.bin"), dtype=np.float64).reshape(6, n)
         traj = np.fromfile(os.path.join(d, "traj.bin"), dtype=np.float64).reshape(steps, 3, n)
         stats_after = np.fromfile(os.path.join(d, "stats.bin"), dtype=np.float64)
@@ -145,7 +146,7 @@ def case(name, crystal, L, a, materials, steps, snaps, T, H=0.0, Hdir=(0, 0, 1),
         fields = fields[:, :0]
     np.savez_compressed(
         os.path.join(OUT, f"ref_{name}.npz"),
-        crystal=crystal, L=L, a=a, T=T, H=H, Hdir=np.array(Hdir, dtype=float), pbc=np.array(pbc),
+        crystal=crystal, L=L, a=a, T=T, H=H, Hdir=np.array(Hdir, dtype=float), pbc=np.array(pbc), integrator=integrator,
         noise_seed=seed, spins_in=spins, snap_steps=np.array(snaps),
         snaps=traj[np.array(snaps) - 1], fields=fields, stats_after=stats_after,
         dt=meta["dt"], half_dt=meta["half_dt"], exchange_type=meta["exchange_type"],
@@ -284,6 +285,13 @@ def main(only=()):
     if want("film_dmi"):
         case("film_dmi", "sc", 5, 2.5, [m1, m2], steps=100, snaps=[1, 100], T=300.0, H=0.0,
              pbc=(True, True, False), dmi_cutoff=3.6, Lz=4)
+    # the LLG midpoint integrator (sim::LLG_Midpoint) on the SC uniaxial and two-material DMI decks
+    if want("sc6_uniaxial_mid"):
+        case("sc6_uniaxial_mid", "sc", 6, 3.54, [CO], steps=100, snaps=[1, 10, 100], T=300.0, H=0.5,
+             integrator="midpoint")
+    if want("film_dmi_mid"):
+        case("film_dmi_mid", "sc", 5, 2.5, [m1, m2], steps=100, snaps=[1, 100], T=300.0, H=0.0,
+             pbc=(True, True, False), dmi_cutoff=3.6, Lz=4, integrator="midpoint")
     # the CPU-runnable workload: SC 20^3, 1000 steps at 300 K
     if want("c0_sc20"):
         case("c0_sc20", "sc", 20, 3.54, [CO], steps=1000, snaps=[1000], T=300.0, H=0.0, seed=1, spin_seed=1,

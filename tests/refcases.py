@@ -23,6 +23,8 @@ CASES = {
                                           kc_vector2=(-1.0, 1.0, 0.0))], ani=dict(cubic_rotated=True)),
     "film_dmi": dict(mats=[S.Material(mu_s_muB=1.6, alpha=0.1), S.Material(mu_s_muB=7.6, alpha=0.1)], ani={}),
     "c0_sc20": dict(mats=[S.Material(**CO)], ani=dict(uniaxial=True)),
+    "sc6_uniaxial_mid": dict(mats=[S.Material(**CO)], ani=dict(uniaxial=True)),
+    "film_dmi_mid": dict(mats=[S.Material(mu_s_muB=1.6, alpha=0.1), S.Material(mu_s_muB=7.6, alpha=0.1)], ani={}),
     "dipole_sc10": dict(mats=[S.Material(**CO), S.Material(**dict(CO, mu_s_muB=2.5))], ani=dict(uniaxial=True)),
 }
 

@@ -37,6 +37,61 @@ def test_thermal_noise_is_standard_normal():
     assert abs(np.corrcoef(g, h)[0, 1]) < 0.01
 
 
+# ---------------------------------------------------------------- LLG midpoint integrator
+@pytest.mark.parametrize("name", ["sc6_uniaxial_mid", "film_dmi_mid"])
+def test_midpoint_trajectory_equals_reference(name):
+    """sim::LLG_Midpoint of the reference fed the same noise stream: identical spins (SC uniaxial
+    in 0.5 T; two-material film with tensorial exchange and DMI)."""
+    f = RC.load(name)
+    s = RC.system_of(name)
+    T, H = RC.conditions(name)
+    o = O.Oracle(s)
+    o.set_conditions(T, H)
+    seed = int(f["noise_seed"])
+    cur, last = f["spins_in"], 0
+    for k, st in enumerate(int(v) for v in f["snap_steps"]):
+        noise = np.stack([O.thermal_noise(seed, s.n, q) for q in range(last + 1, st + 1)])
+        cur = o.llg_midpoint(cur, st - last, noise=noise)
+        last = st
+        assert np.array_equal(cur, f["snaps"][k]), st
+
+
+def test_midpoint_single_spin_precession_closed_form():
+    """The reference's AppliedField case (10 T, alpha = 1, S0 = x) through the midpoint scheme:
+    the analytic damped precession to 1e-6, and |S| = 1 to rounding without any renormalisation
+    (the Cayley rotation is orthogonal; a dropped or mis-signed term breaks the norm by O(dt^2)
+    per step)."""
+    s = _isolated(1, S.Material(alpha=1.0))
+    o = O.Oracle(s)
+    o.set_conditions(0.0, (0.0, 0.0, 10.0))
+    sp = np.array([[1.0], [0.0], [0.0]])
+    worst = 0.0
+    for k in range(1, 51):
+        sp = o.llg_midpoint(sp, 200)
+        a = k * 200 * 1e-16 * 1.76e11 * 10.0 / 2.0
+        exp = np.array([np.cos(a) / np.cosh(a), np.sin(a) / np.cosh(a), np.tanh(a)])
+        worst = max(worst, np.abs(sp[:, 0] - exp).max())
+        assert abs(np.linalg.norm(sp[:, 0]) - 1.0) < 1e-11      # rounding only, over 10^4 steps
+    assert worst < 1e-6
+
+
+def test_midpoint_is_second_order():
+    """Halving dt cuts the deterministic (T=0) global error by ~4 (the implicit midpoint rule is
+    second order), on an exchange-coupled SC 4^3 system in 1 T."""
+    sp0 = S.random_spins(64, 2)
+    ref_sys = S.ferromagnet("sc", 4, alpha=0.2, dt_SI=1e-17 / 64)
+    o = O.Oracle(ref_sys)
+    o.set_conditions(0.0, (0.0, 0.0, 1.0))
+    ref = o.llg_midpoint(sp0, 64 * 50)
+    errs = []
+    for div in (1, 2, 4):
+        od = O.Oracle(S.ferromagnet("sc", 4, alpha=0.2, dt_SI=1e-17 / div))
+        od.set_conditions(0.0, (0.0, 0.0, 1.0))
+        errs.append(np.abs(od.llg_midpoint(sp0, div * 50) - ref).max())
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert 3.2 < r1 < 4.8 and 3.2 < r2 < 4.8, errs
+
+
 # ---------------------------------------------------------------- macrocell dipole field
 def test_dipole_field_equals_reference():
     """cells::mag, dipole::internal::update_field (tensor solver) and the atom unroll of the

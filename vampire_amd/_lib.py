@@ -134,6 +134,7 @@ SIGNATURES = [
     ("vmx_set_time_step", C.c_int, [_VP, C.c_double, C.c_double]),
     ("vmx_set_conditions", C.c_int, [_VP, C.c_int32, C.c_double, _DP, C.c_uint32]),
     ("vmx_llg_heun", C.c_int, [_VP, C.c_int32, _DP]),
+    ("vmx_llg_midpoint", C.c_int, [_VP, C.c_int32, _DP]),
     ("vmx_llg_heun_graph", C.c_int, [_VP, C.c_int32, C.c_int32]),
     ("vmx_get_step", C.c_int, [_VP, _P(C.c_int64)]),
     ("vmx_set_step", C.c_int, [_VP, C.c_int64]),

@@ -153,6 +153,16 @@ class Context:
                 raise ValueError(f"noise must hold {nsteps}x3x{self.n} values")
             L.check(self.lib.vmx_llg_heun(self.h, int(nsteps), L.dptr(nz.reshape(-1))))
 
+    def llg_midpoint(self, nsteps: int = 1, noise: Optional[np.ndarray] = None) -> None:
+        """nsteps steps of the LLG midpoint integrator (sim::LLG_Midpoint); noise as llg_heun."""
+        if noise is None:
+            L.check(self.lib.vmx_llg_midpoint(self.h, int(nsteps), None))
+        else:
+            nz = np.ascontiguousarray(noise, dtype=np.float64)
+            if nz.size != nsteps * 3 * self.n:
+                raise ValueError(f"noise must hold {nsteps}x3x{self.n} values")
+            L.check(self.lib.vmx_llg_midpoint(self.h, int(nsteps), L.dptr(nz.reshape(-1))))
+
     def llg_heun_graph(self, nsteps: int, nrepeat: int) -> None:
         L.check(self.lib.vmx_llg_heun_graph(self.h, int(nsteps), int(nrepeat)))
 

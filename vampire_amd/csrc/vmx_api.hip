@@ -558,7 +558,7 @@ struct vmx_ctx {
             }
             c.kc4 = (R)a.kc4;
             c.mu_s_SI = (R)a.mu_s_SI;
-            c.pad = (R)0;
+            c.alpha = (R)a.alpha;
         }
         VMX_TRY(mat.alloc(mc.size() * sizeof(MatConst<R>)));
         VMX_HIP(hipMemcpy(mat.p, mc.data(), mc.size() * sizeof(MatConst<R>), hipMemcpyHostToDevice));
@@ -602,6 +602,8 @@ struct vmx_ctx {
 
     template <typename R>
     int heun_steps(int nsteps, const double* noise, double* kernel_ms = nullptr);
+    template <typename R>
+    int midpoint_steps(int nsteps, const double* noise);
     template <typename R>
     int capture_graph(int nsteps);
     template <typename R>
@@ -699,6 +701,42 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
     }
     if (kernel_ms)
         for (int e = 0; e < 6; ++e) (void)hipEventDestroy(kev[e]);
+    VMX_HIP(hipEventRecord(ev1, stream));
+    timed_steps = nsteps;
+    hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(1), 0, stream, step_base.as<unsigned long long>(),
+                       (unsigned)nsteps);
+    VMX_HIP(hipGetLastError());
+    step += (uint64_t)nsteps;
+    if (dip.on) dip.time += nsteps;
+    return VMX_OK;
+}
+
+// sim::LLG_Midpoint (src/simulate/LLGMidpoint.cpp:87-186) with the fused workgroup organisation:
+// predictor into the spare buffer, corrector into the output buffer, roles swapped per step.
+template <typename R>
+int vmx_ctx::midpoint_steps(int nsteps, const double* noise) {
+    const SysDev<R> sf = sys_fused<R>();
+    const Launch Lf = launch_fused();
+    const double* nd = nullptr;
+    if (noise) {
+        VMX_TRY(noise_dev.alloc(3 * (size_t)n * sizeof(double)));
+        nd = noise_dev.as<double>();
+    }
+    VMX_HIP(hipEventRecord(ev0, stream));
+    for (int k = 0; k < nsteps; ++k) {
+        if (noise) {
+            noise_host.resize(3 * (size_t)n);
+            to_internal(noise + (size_t)k * 3 * n, noise_host.data(), 3);
+            VMX_HIP(hipStreamSynchronize(stream));
+            VMX_HIP(hipMemcpy(noise_dev.p, noise_host.data(), 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+        }
+        const int p_idx = 3 - s_idx - o_idx;
+        VMX_HIP(Ops<R>::mid_predictor(Lf, et, am(), sf, spins<R>(s_idx), spins<R>(p_idx), field<R>(), nd, (uint32_t)k));
+        VMX_HIP(Ops<R>::mid_corrector(Lf, et, am(), sf, spins<R>(s_idx), spins<R>(p_idx), spins<R>(o_idx), field<R>(),
+                                      nd, (uint32_t)k));
+        std::swap(s_idx, o_idx);
+        if (dip.on && (dip.time + k + 1) % dip.rate == 0) VMX_TRY(dipole_refresh<R>());
+    }
     VMX_HIP(hipEventRecord(ev1, stream));
     timed_steps = nsteps;
     hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(1), 0, stream, step_base.as<unsigned long long>(),
@@ -1527,6 +1565,15 @@ int vmx_llg_heun(vmx_ctx* c, int32_t nsteps, const double* noise) {
     if (nsteps == 0) return VMX_OK;
     VMX_TRY(c->use_device());
     return c->precision == VMX_FP64 ? c->heun_steps<double>(nsteps, noise) : c->heun_steps<float>(nsteps, noise);
+}
+
+int vmx_llg_midpoint(vmx_ctx* c, int32_t nsteps, const double* noise) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    if (nsteps < 0) return fail(VMX_ERR_ARG, "nsteps must be >= 0");
+    if (!(c->dt > 0.0)) return fail(VMX_ERR_STATE, "time step not set (vmx_set_time_step)");
+    if (nsteps == 0) return VMX_OK;
+    VMX_TRY(c->use_device());
+    return c->precision == VMX_FP64 ? c->midpoint_steps<double>(nsteps, noise) : c->midpoint_steps<float>(nsteps, noise);
 }
 
 int vmx_llg_heun_graph(vmx_ctx* c, int32_t nsteps, int32_t nrepeat) {

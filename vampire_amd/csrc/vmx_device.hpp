@@ -57,7 +57,7 @@ struct MatConst {            // per material, set once (mp::material after set_d
     R kc4;                   // cubic constant (T)
     R e1[3], e2[3], e3[3];   // rotated cubic basis
     R mu_s_SI;               // for the Monte Carlo energy scale
-    R pad;
+    R alpha;                 // Gilbert damping (the midpoint integrator's F = H + alpha S x H)
 };
 
 template <typename R>
@@ -755,6 +755,151 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D, S
         const R r = inv_norm(n0, n1, n2);
         if (ok[u]) {
             O.x[i] = n0 * r; O.y[i] = n1 * r; O.z[i] = n2 * r;
+        }
+    }
+}
+
+// ======================= LLG midpoint: sim::LLG_Midpoint (src/simulate/LLGMidpoint.cpp) ======
+// The step rotates a spin about F = H + alpha M x H in the Cayley form
+//   S' = [S (1 - b^2 F.F) + 2 (b (F x S) + b^2 (F.S) F)] / (1 + b^2 F.F),  b = -dt one_opa2 / 2:
+// first with M = S and the fields at S, keeping the midpoint (S + S')/2 unnormalised
+// (LLGMidpoint.cpp:114-141), then with M the midpoint, the spin field at M and the same external
+// field, applied to the initial S (:151-183).  Term order is the reference's.
+template <typename R>
+__device__ __forceinline__ void cayley(R alpha, R beta, R m0, R m1, R m2, R h0, R h1, R h2, R s0, R s1, R s2,
+                                       R& o0, R& o1, R& o2) {
+    const R beta2 = beta * beta;
+    const R F0 = h0 + alpha * (m1 * h2 - m2 * h1);
+    const R F1 = h1 + alpha * (m2 * h0 - m0 * h2);
+    const R F2 = h2 + alpha * (m0 * h1 - m1 * h0);
+    const R FdotF = F0 * F0 + F1 * F1 + F2 * F2;
+    const R beta2FdotS = beta2 * (F0 * s0 + F1 * s1 + F2 * s2);
+    const R one_o_one_plus_beta2FdotF = R(1) / (R(1) + beta2 * FdotF);
+    const R one_minus_beta2FdotF = R(1) - beta2 * FdotF;
+    o0 = one_o_one_plus_beta2FdotF * (s0 * one_minus_beta2FdotF + R(2) * (beta * (F1 * s2 - F2 * s1) + F0 * beta2FdotS));
+    o1 = one_o_one_plus_beta2FdotF * (s1 * one_minus_beta2FdotF + R(2) * (beta * (F2 * s0 - F0 * s2) + F1 * beta2FdotS));
+    o2 = one_o_one_plus_beta2FdotF * (s2 * one_minus_beta2FdotF + R(2) * (beta * (F0 * s1 - F1 * s0) + F2 * beta2FdotS));
+}
+
+template <typename R>
+__device__ __forceinline__ R midpoint_beta(const SysDev<R>& s, const MatConst<R>& mc) {
+    return R(-1) * s.dt * mc.one_opa2 * R(0.5);      // LLGMidpoint.cpp:118
+}
+
+// Midpoint predictor: fields at S, M = (S + S')/2 into P; H_ext into E when kept.  Same workgroup
+// organisation as the fused Heun kernels (block table, own data first, LDS tables).
+template <typename R, int ET, int AM>
+__global__ void __launch_bounds__(512)
+k_mid_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> E, const double* __restrict__ noise,
+                uint32_t step_off) {
+    constexpr int U = FusedAtoms<R>::value;
+    constexpr bool kStaged = Access<AM>::TP != TP_CSR;
+    const int lb = logical_block(s.xcd_remap);
+    if (lb >= s.nblocks) return;
+    const int4 b = s.blk[lb];
+    int idx[U];
+    bool ok[U];
+    fused_atoms<U>(b, idx, ok);
+    const int rep = b.y;
+    const SpinRsrc<R> rs(s, S.x, S.y, S.z);
+    int m[U], head[U];
+    R sx[U], sy[U], sz[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        m[u] = s.nmat == 1 ? 0 : (int)s.type[idx[u]];
+        rs.load(idx[u], sx[u], sy[u], sz[u]);
+        if (kStaged) head[u] = row_head<R, AM>(s, idx[u]);
+    }
+    stage_fused<R, AM>(s, rep);
+    if (!ok[0]) return;
+    const uint64_t step = *s.step_base + step_off;
+    const UniformJ<R, ET> ju(s, Access<AM>::JM);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int i = idx[u];
+        const MatConst<R>& mc = FusedLds<R>::mat()[m[u]];
+        R fx, fy, fz;
+        if constexpr (kStaged) {
+            RowIdx<AM> ri;
+            RowSpins<R, AM> rv;
+            fused_row_index(s, i, head[u], ri);
+            row_load(rs, ri, rv);
+            row_field<R, ET, AM>(s, ri, rv, ju.v, mc, sx[u], sy[u], sz[u], fx, fy, fz);
+        } else {
+            spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m[u], sx[u], sy[u], sz[u], fx, fy, fz);
+        }
+        const Cond<R>& c = FusedLds<R>::cond(s)[m[u]];
+        R ex, ey, ez;
+        ext_field(s, c, rep, i, caller_local(s, rep, i - b.w), step, noise, ex, ey, ez);
+        R o0, o1, o2;
+        cayley(mc.alpha, midpoint_beta(s, mc), sx[u], sy[u], sz[u], fx + ex, fy + ey, fz + ez, sx[u], sy[u], sz[u],
+               o0, o1, o2);
+        if (ok[u]) {
+            if (E.x) {
+                E.x[i] = ex; E.y[i] = ey; E.z[i] = ez;
+            }
+            P.x[i] = (sx[u] + o0) * R(0.5); P.y[i] = (sy[u] + o1) * R(0.5); P.z[i] = (sz[u] + o2) * R(0.5);
+        }
+    }
+}
+
+// Midpoint corrector: spin field at M (P), the same H_ext (kept in E, or regenerated from the
+// same Philox counter), the rotation applied to the initial S, written to O.
+template <typename R, int ET, int AM>
+__global__ void __launch_bounds__(512)
+k_mid_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> E, const double* __restrict__ noise,
+                uint32_t step_off) {
+    constexpr int U = FusedAtoms<R>::value;
+    constexpr bool kStaged = Access<AM>::TP != TP_CSR;
+    const int lb = logical_block(s.xcd_remap);
+    if (lb >= s.nblocks) return;
+    const int4 b = s.blk[lb];
+    int idx[U];
+    bool ok[U];
+    fused_atoms<U>(b, idx, ok);
+    const int rep = b.y;
+    const SpinRsrc<R> rp(s, P.x, P.y, P.z);
+    int m[U], head[U];
+    R px[U], py[U], pz[U], ax[U], ay[U], az[U], hx[U], hy[U], hz[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int i = idx[u];
+        m[u] = s.nmat == 1 ? 0 : (int)s.type[i];
+        rp.load(i, px[u], py[u], pz[u]);
+        if (kStaged) head[u] = row_head<R, AM>(s, i);
+        ax[u] = S.x[i]; ay[u] = S.y[i]; az[u] = S.z[i];
+        if (E.x) { hx[u] = E.x[i]; hy[u] = E.y[i]; hz[u] = E.z[i]; }
+    }
+    stage_fused<R, AM>(s, rep);
+    if (!ok[0]) return;
+    const uint64_t step = *s.step_base + step_off;
+    const UniformJ<R, ET> ju(s, Access<AM>::JM);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int i = idx[u];
+        const MatConst<R>& mc = FusedLds<R>::mat()[m[u]];
+        R fx, fy, fz;
+        if constexpr (kStaged) {
+            RowIdx<AM> ri;
+            RowSpins<R, AM> rv;
+            fused_row_index(s, i, head[u], ri);
+            row_load(rp, ri, rv);
+            row_field<R, ET, AM>(s, ri, rv, ju.v, mc, px[u], py[u], pz[u], fx, fy, fz);
+        } else {
+            spin_field<R, ET, AM>(s, P.x, P.y, P.z, i, m[u], px[u], py[u], pz[u], fx, fy, fz);
+        }
+        R ex, ey, ez;
+        if (E.x) {
+            ex = hx[u]; ey = hy[u]; ez = hz[u];
+        } else {
+            const Cond<R>& c = FusedLds<R>::cond(s)[m[u]];
+            ext_field(s, c, rep, i, caller_local(s, rep, i - b.w), step, noise, ex, ey, ez);
+        }
+        R o0, o1, o2;
+        cayley(mc.alpha, midpoint_beta(s, mc), px[u], py[u], pz[u], fx + ex, fy + ey, fz + ez, ax[u], ay[u], az[u],
+               o0, o1, o2);
+        if (ok[u]) {
+            O.x[i] = o0; O.y[i] = o1; O.z[i] = o2;
         }
     }
 }

@@ -27,6 +27,12 @@ struct Ops {
     static hipError_t fused_corrector(const Launch& L, int et, int jm, const SysDev<R>& s, Spin3<R> S,
                                       Spin3<R> P, Spin3<R> O, Spin3<R> D, Spin3<R> E, const double* noise,
                                       uint32_t step_off);
+    // LLG midpoint step: predictor writes the midpoint P from S (E: H_ext kept, or null);
+    // corrector writes O from S and P
+    static hipError_t mid_predictor(const Launch& L, int et, int jm, const SysDev<R>& s, Spin3<R> S, Spin3<R> P,
+                                    Spin3<R> E, const double* noise, uint32_t step_off);
+    static hipError_t mid_corrector(const Launch& L, int et, int jm, const SysDev<R>& s, Spin3<R> S, Spin3<R> P,
+                                    Spin3<R> O, Spin3<R> E, const double* noise, uint32_t step_off);
     // plain Heun step
     static hipError_t spin_fields(const Launch& L, int et, int jm, const SysDev<R>& s, Spin3<R> S, Spin3<R> F);
     static hipError_t ext_fields(const Launch& L, const SysDev<R>& s, Spin3<R> E, const double* noise,
