@@ -24,6 +24,7 @@
 
 #include "atoms.hpp"
 #include "dipole.hpp"
+#include "montecarlo/internal.hpp"
 #include "errors.hpp"
 #include "exchange.hpp"
 #include "gpu.hpp"
@@ -191,6 +192,28 @@ void initialize() {
 void llg_heun() {
     push_conditions();
     check(vmx_llg_heun(g_ctx, 1, nullptr), "vmx_llg_heun");
+}
+
+// ---- integrators the reference runs on the host only ------------------------------------------
+// sim::integrate_serial has no accelerator dispatch for integrator 1 (Monte Carlo) or 2 (LLG
+// midpoint) (src/simulate/sim.cpp:608-621).  A VAMPIRE build that wants them on the device
+// declares these two in hdr/gpu.hpp and dispatches beside the Heun step (sim.cpp:600):
+//     if(gpu::acceleration) gpu::mc_step();      else montecarlo::mc_step(...);
+//     if(gpu::acceleration) gpu::llg_midpoint(); else sim::LLG_Midpoint();
+// One call is one sweep of N trial moves (colour-ordered on the device, random-site on the host,
+// montecarlo::mc_step, src/montecarlo/mc.cpp:29-127) or one midpoint step.
+void mc_step() {
+    if (montecarlo::internal::algorithm != montecarlo::internal::adaptive) {
+        std::cerr << "Error: the GPU backend implements the adaptive-cone Monte Carlo move only" << std::endl;
+        err::vexit();
+    }
+    push_conditions();
+    check(vmx_mc_sweep(g_ctx, 1), "vmx_mc_sweep");
+}
+
+void llg_midpoint() {
+    push_conditions();
+    check(vmx_llg_midpoint(g_ctx, 1, nullptr), "vmx_llg_midpoint");
 }
 
 void finalize() {
