@@ -667,7 +667,7 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
             VMX_HIP(Ops<R>::fused_predictor(Lf, et, am(), sf, spins<R>(s_idx), spins<R>(p_idx), euler<R>(),
                                             field<R>(), nd, (uint32_t)k));
             VMX_HIP(mark(1));
-            VMX_HIP(Ops<R>::fused_corrector(Lf, et, am(), sf, spins<R>(s_idx), spins<R>(p_idx), spins<R>(o_idx),
+            VMX_HIP(Ops<R>::fused_corrector(L, et, am(), s, spins<R>(s_idx), spins<R>(p_idx), spins<R>(o_idx),
                                             euler<R>(), field<R>(), nd, (uint32_t)k));
             VMX_HIP(mark(2));
             nk = 2;
@@ -751,13 +751,13 @@ template <typename R>
 int vmx_ctx::capture_graph(int nsteps) {
     if (graph_exec) { VMX_HIP(hipGraphExecDestroy(graph_exec)); graph_exec = nullptr; }
     if (graph) { VMX_HIP(hipGraphDestroy(graph)); graph = nullptr; }
-    const SysDev<R> s = sys_fused<R>();
-    const Launch L = launch_fused();
+    const SysDev<R> s = sys<R>(), sf = sys_fused<R>();
+    const Launch L = launch(), Lf = launch_fused();
     VMX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     int si = s_idx, oi = o_idx;
     for (int k = 0; k < nsteps; ++k) {
         const int pi = 3 - si - oi;
-        hipError_t e = Ops<R>::fused_predictor(L, et, am(), s, spins<R>(si), spins<R>(pi), euler<R>(), field<R>(),
+        hipError_t e = Ops<R>::fused_predictor(Lf, et, am(), sf, spins<R>(si), spins<R>(pi), euler<R>(), field<R>(),
                                                nullptr, (uint32_t)k);
         if (e == hipSuccess)
             e = Ops<R>::fused_corrector(L, et, am(), s, spins<R>(si), spins<R>(pi), spins<R>(oi), euler<R>(),
