@@ -173,6 +173,10 @@ def test_dipole_refusals(gpu):
             c.llg_heun_graph(10, 1)
         with pytest.raises(VmxError, match="dipole"):
             c.checkpoint()
+        with Context(s, precision=FP64, seed=2) as plain:
+            chk = plain.checkpoint()
+        with pytest.raises(VmxError, match="dipole"):
+            c.restore(chk)
         with pytest.raises(VmxError, match="dipole solver"):
             c.set_atom_field(f["atom_B0"])
         bad = dict(_dipole_kwargs(f))

@@ -1666,6 +1666,7 @@ int vmx_checkpoint_load(vmx_ctx* c, const void* buf, uint64_t bytes) {
     if (!c || !buf) return fail(VMX_ERR_ARG, "null argument");
     if (bytes < chk_bytes(c)) return fail(VMX_ERR_ARG, "checkpoint of %llu bytes, this context needs %llu",
                                           (unsigned long long)bytes, (unsigned long long)chk_bytes(c));
+    if (c->hdip_on) return fail(VMX_ERR_STATE, "checkpoints do not hold the dipole field; clear it first");
     const char* p = static_cast<const char*>(buf);
     auto get = [&](void* v, size_t b) { std::memcpy(v, p, b); p += b; };
     auto get64 = [&]() { uint64_t v; get(&v, 8); return v; };
