@@ -3,7 +3,7 @@ FP32 fused kernels (the headline value) beside FP64 fused and the plain FP64 ker
 HBM roofline of the Heun step and the CPU restatement of the reference timed on the host.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--L 100] [--cpu-seconds 10] [--no-cpu]
-                    [--batching-steps 200] [--legs fp32_fused,fp64_fused,fp64_plain]
+                    [--batching-steps 200] [--legs fp32_fused,fp64_fused,fp64_plain,fp32_midpoint]
 
 One "step" is one Heun step (predictor + corrector) over all 4,000,000 spins, inputs resident
 in HBM.  Also reported: the FP32-vs-FP64 drift of the magnetisation, the replica-batching leg
@@ -33,7 +33,7 @@ def b_alg(p: int, z: int, c_j: int = 0) -> int:
     return 12 * p + z * (4 + c_j * p) + 4
 
 
-def run_gpu(system, precision, kernel, steps, warmup, seed=7, T=600.0, use_graph=False, block=0):
+def run_gpu(system, precision, kernel, steps, warmup, seed=7, T=600.0, use_graph=False, block=0, integrator="heun"):
     from vampire_amd.context import Context
     spins = np.zeros((3, system.n))
     spins[2] = 1.0
@@ -43,11 +43,14 @@ def run_gpu(system, precision, kernel, steps, warmup, seed=7, T=600.0, use_graph
         c.set_spins(spins)
         c.autotune(20)
     c.set_spins(spins)
+    step = c.llg_midpoint if integrator == "midpoint" else c.llg_heun
     if warmup:
-        c.llg_heun(warmup)
+        step(warmup)
     c.synchronize()
     t0 = time.perf_counter()
-    if use_graph:
+    if integrator == "midpoint":
+        step(steps)
+    elif use_graph:
         chunk = 50
         c.llg_heun_graph(chunk, steps // chunk)
         rest = steps - chunk * (steps // chunk)
@@ -57,7 +60,8 @@ def run_gpu(system, precision, kernel, steps, warmup, seed=7, T=600.0, use_graph
         c.llg_heun(steps)
     c.synchronize()
     wall = time.perf_counter() - t0
-    kernel_ms = c.time_kernels(20)
+    # per-kernel HIP-event timings of the Heun step (the midpoint leg reports its wall time only)
+    kernel_ms = c.time_kernels(20) if integrator == "heun" else np.zeros(0)
     c.bench_launch = c.launch_config
     return c, wall, kernel_ms
 
@@ -283,7 +287,7 @@ def main():
     ap.add_argument("--block", type=int, default=-1, help="threads per workgroup; 0: library default, -1: autotune")
     ap.add_argument("--batching-steps", type=int, default=200, help="0 skips the replica-batching leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the C0 / C3 / C4 leg")
-    ap.add_argument("--legs", default="fp32_fused,fp64_fused,fp64_plain",
+    ap.add_argument("--legs", default="fp32_fused,fp64_fused,fp64_plain,fp32_midpoint",
                     help="comma list of fp32_fused, fp64_fused, fp64_plain (profiling runs use one)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with three 256 MiB device-to-device copies (counter scale for traffic)")
@@ -304,12 +308,13 @@ def main():
     print(f"# system: FCC {args.L}^3, {n} atoms, z={z}, built in {t_build:.1f} s", file=sys.stderr, flush=True)
 
     legs = [x for x in args.legs.split(",") if x]
-    table = {"fp32_fused": (FP32, FUSED), "fp64_fused": (FP64, FUSED), "fp64_plain": (FP64, PLAIN)}
+    table = {"fp32_fused": (FP32, FUSED, "heun"), "fp64_fused": (FP64, FUSED, "heun"),
+             "fp64_plain": (FP64, PLAIN, "heun"), "fp32_midpoint": (FP32, FUSED, "midpoint")}
     res = {}
     for name in legs:
-        prec, kern = table[name]
+        prec, kern, integ = table[name]
         c, wall, kms = run_gpu(system, prec, kern, args.steps, args.warmup, use_graph=args.graph and kern == FUSED,
-                               block=args.block)
+                               block=args.block, integrator=integ)
         res[name] = dict(wall=wall, kernel_ms=kms.tolist(), sps=n * args.steps / wall,
                          ms_per_step=1e3 * wall / args.steps,
                          launch={"block": c.bench_launch[0], "xcd_remap": c.bench_launch[1]})
@@ -401,6 +406,10 @@ def main():
         "fp64_plain": leg("fp64_plain", 8),
         "fp32_over_fp64_plain": (round(head["sps"] / res["fp64_plain"]["sps"], 3)
                                  if head is not None and "fp64_plain" in res else None),
+        "fp32_midpoint": ({"value": round(res["fp32_midpoint"]["sps"], 1),
+                           "ms_per_step": round(res["fp32_midpoint"]["ms_per_step"], 5),
+                           "integrator": "LLG midpoint (sim::LLG_Midpoint), same workload"}
+                          if "fp32_midpoint" in res else None),
         "fp32_vs_fp64_drift": drift,
         "batching": batching,
         "configs": configs,
