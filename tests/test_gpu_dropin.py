@@ -148,9 +148,9 @@ def test_reference_curie_deck_on_device(gpu, exe, tmp_path):
     """The reference's tests/curie-temperature deck, unmodified but for gpu:acceleration=on (SC
     22^3 cells, J = 11.2e-21 J, 0-1800 K in 100 K steps, 10k + 30k Heun steps each), run by
     sim::run through gpu:: on the device.  Its output (columns: time, T, mean |m| over the
-    temperature's loop, m direction and |m|, |m|, field alignment) must show the ferromagnet's M(T): |m| = 1 at 0 K,
-    falling with T, crossing 1/2 below the infinite-lattice Tc = 1.443 J/kB = 1171 K and near
-    zero above it."""
+    temperature's loop, m direction and |m|, |m|, field alignment) must show the ferromagnet's
+    M(T): |m| = 1 at 0 K, falling with T, crossing 1/2 below the infinite-lattice
+    Tc = 1.443 J/kB = 1171 K and near zero above it."""
     out = run_reference_deck(tmp_path, "gpu", "curie-temperature", "on")
     T, m = out[:, 1], out[:, 6]
     assert np.array_equal(T, np.arange(0.0, 1801.0, 100.0))
@@ -177,3 +177,25 @@ def test_reference_curie_deck_device_vs_cpu(gpu, exe, tmp_path):
     dev = run_reference_deck(tmp_path, "gpu", "curie-temperature", "on", edits)
     assert np.array_equal(cpu[:, 1], dev[:, 1]) and len(cpu) == 7
     assert np.abs(cpu[:, 2] - dev[:, 2]).max() <= 0.03, (cpu[:, 2], dev[:, 2])
+
+
+def test_reference_time_series_deck_brownian_relaxation(gpu, exe, tmp_path):
+    """The reference's tests/time-series deck (SC 22^3, J = 0, alpha = 1, 1000 K, spins +z: free
+    spins in a heat bath) with dt = 0.1 fs over 200 steps, run by the device and by the reference
+    CPU integrator.  Each spin diffuses freely on the sphere, so |m|(t) = exp(-t/tau) with
+    tau = (1 + alpha^2) mu_s / (2 alpha gamma k_B T) = 6.6 fs (Brown's rotational diffusion; the
+    thermal-field amplitude and the damping normalisation fix it).  Both runs follow it within 0.03
+    (about three times the 1/sqrt(N) noise) and agree with each other within 0.03."""
+    edits = [("sim:time-step=1.0E-15", "sim:time-step=1.0E-16"),
+             ("sim:total-time-steps=10000", "sim:total-time-steps=200"),
+             ("sim:time-steps-increment=1000", "sim:time-steps-increment=10")]
+    cpu = run_reference_deck(tmp_path, "cpu", "time-series", "off", edits)
+    dev = run_reference_deck(tmp_path, "gpu", "time-series", "on", edits)
+    t = cpu[:, 0]
+    assert np.array_equal(t, dev[:, 0]) and len(t) >= 4
+    alpha, gamma, kB, T, mu_s = 1.0, 1.76e11, 1.3806503e-23, 1000.0, 1.72 * 9.27400915e-24
+    tau = (1.0 + alpha ** 2) * mu_s / (2.0 * alpha * gamma * kB * T)
+    theory = np.exp(-t / tau)
+    for run in (cpu, dev):
+        assert np.abs(run[:, 5] - theory).max() <= 0.03, (run[:, 5], theory)
+    assert np.abs(cpu[:, 5] - dev[:, 5]).max() <= 0.03
