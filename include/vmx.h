@@ -123,10 +123,11 @@ typedef struct vmx_options {
     int32_t device;       /* HIP device ordinal; -1 = current device                            */
     uint64_t seed;        /* Philox key for thermal noise and Monte Carlo moves                 */
     int32_t topology;     /* VMX_TOPOLOGY_*: device layout of the neighbour list                */
-    int32_t recompute_euler; /* fused kernels: 0 = the predictor keeps the Euler derivative dS
-                             for the corrector, as the reference keeps its euler arrays; 1 = the
-                             corrector recomputes it (one neighbour gather more, 6 fewer reals
-                             of memory traffic per atom).  Results are identical.              */
+    int32_t recompute_euler; /* fused FP64 kernels: 0 = the predictor keeps the Euler derivative
+                             dS for the corrector, as the reference keeps its euler arrays; 1 =
+                             the corrector recomputes it (one neighbour gather more, 6 fewer
+                             reals of memory traffic per atom).  Results are identical.  FP32
+                             keeps nothing: its corrector rebuilds S + dt/2 dS from S and S'. */
     int32_t atom_order;   /* VMX_ATOM_ORDER_*: how the device stores atoms                     */
     int32_t keep_external_field; /* fused kernels: 0 = the corrector regenerates the external
                              (thermal + applied) field from the same Philox counter; 1 = the

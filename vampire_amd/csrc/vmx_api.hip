@@ -332,7 +332,7 @@ struct vmx_ctx {
     DevBuf spin[3];               // three buffers of 3N R each (S, P, O roles rotate)
     int s_idx = 0, o_idx = 2;     // current spin buffer and the corrector's output buffer
     DevBuf plainF, plainE, plainD;
-    int store_euler = 1;          // fused step: predictor keeps dS (FP64) or S + dt/2 dS (FP32)
+    int store_euler = 1;          // fused FP64 step: the predictor keeps dS (FP32 keeps nothing)
     DevBuf euler_buf;             // 3N R
     int store_field = 0;          // fused step: predictor keeps H_ext for the corrector
     DevBuf field_buf;             // 3N R
@@ -427,7 +427,7 @@ struct vmx_ctx {
     // Euler-derivative arrays of the fused step, or null arrays when the corrector recomputes dS
     template <typename R>
     Spin3<R> euler() const {
-        if (!store_euler) return Spin3<R>{nullptr, nullptr, nullptr};
+        if (!store_euler || sizeof(R) == sizeof(float)) return Spin3<R>{nullptr, nullptr, nullptr};
         return arr3<R>(euler_buf);
     }
     // kept external-field arrays, or null arrays when the corrector regenerates the field
@@ -1451,7 +1451,8 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
     VMX_TRY(c->staging.alloc(3 * (size_t)n * sizeof(double)));
     c->store_euler = opt->recompute_euler ? 0 : 1;
     c->store_field = opt->keep_external_field ? 1 : 0;
-    if (c->kernel == VMX_KERNEL_FUSED && c->store_euler) VMX_TRY(c->euler_buf.alloc(3 * (size_t)n * rs));
+    if (c->kernel == VMX_KERNEL_FUSED && c->store_euler && c->precision == VMX_FP64)
+        VMX_TRY(c->euler_buf.alloc(3 * (size_t)n * rs));
     if (c->kernel == VMX_KERNEL_FUSED && c->store_field) VMX_TRY(c->field_buf.alloc(3 * (size_t)n * rs));
     const size_t cond_bytes = (size_t)nrep * c->nmat * sizeof(Cond<double>) + 64;
     VMX_TRY(c->ring.init(std::max<size_t>(cond_bytes, 8 * (size_t)nrep + 64)));

@@ -498,9 +498,10 @@ __device__ __forceinline__ float inv_norm(float x, float y, float z) {
 }
 
 // ======================= fused kernels: two launches per Heun step ========================
-// FP32 keeps A = S + dt/2 dS instead of dS, so its corrector reads neither S nor dS: one array
-// less per atom.  FP64 keeps dS and forms S + dt/2 (dS + dS') in the reference's order, which is
-// what makes its results equal to the reference bit for bit.
+// FP32 keeps nothing between the two kernels: its corrector rebuilds A = S + dt/2 dS from S and
+// S' (S + dt dS = N S' with N = |S|^2 / (S . S'), dS being orthogonal to S), so the predictor
+// writes S' alone.  FP64 keeps dS (or recomputes it) and forms S + dt/2 (dS + dS') in the
+// reference's order, which is what makes its results equal to the reference bit for bit.
 template <typename R> struct FoldEuler { static constexpr bool value = false; };
 template <> struct FoldEuler<float> { static constexpr bool value = true; };
 
@@ -647,12 +648,8 @@ k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, Spin3<R> E, c
         R n0 = sx[u] + d0 * s.dt, n1 = sy[u] + d1 * s.dt, n2 = sz[u] + d2 * s.dt;
         const R r = inv_norm(n0, n1, n2);
         if (ok[u]) {
-            if (D.x) {
-                if constexpr (FoldEuler<R>::value) {
-                    D.x[i] = sx[u] + s.half_dt * d0; D.y[i] = sy[u] + s.half_dt * d1; D.z[i] = sz[u] + s.half_dt * d2;
-                } else {
-                    D.x[i] = d0; D.y[i] = d1; D.z[i] = d2;
-                }
+            if constexpr (!FoldEuler<R>::value) {
+                if (D.x) { D.x[i] = d0; D.y[i] = d1; D.z[i] = d2; }
             }
             if (E.x) {
                 E.x[i] = ex; E.y[i] = ey; E.z[i] = ez;
@@ -679,12 +676,12 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D, S
     bool ok[U];
     fused_atoms<U>(b, idx, ok);
     const int rep = b.y;
-    const bool keep = D.x != nullptr;
-    const bool fold = FoldEuler<R>::value && keep;
+    constexpr bool fold = FoldEuler<R>::value;    // FP32: A rebuilt from S and S'
+    const bool keep = !fold && D.x != nullptr;     // FP64: dS kept by the predictor
     const SpinRsrc<R> rp(s, P.x, P.y, P.z);
     int m[U], head[U];
     R px[U], py[U], pz[U];
-    R ax[U], ay[U], az[U];          // A (folded) or S
+    R ax[U], ay[U], az[U];          // S
     R dx[U], dy[U], dz[U];          // kept dS (FP64)
     R hx[U], hy[U], hz[U];          // kept H_ext
 #pragma unroll
@@ -693,12 +690,8 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D, S
         m[u] = s.nmat == 1 ? 0 : (int)s.type[i];
         rp.load(i, px[u], py[u], pz[u]);
         if (kStaged) head[u] = row_head<R, AM>(s, i);
-        if (fold) {
-            ax[u] = D.x[i]; ay[u] = D.y[i]; az[u] = D.z[i];
-        } else {
-            ax[u] = S.x[i]; ay[u] = S.y[i]; az[u] = S.z[i];
-            if (keep) { dx[u] = D.x[i]; dy[u] = D.y[i]; dz[u] = D.z[i]; }
-        }
+        ax[u] = S.x[i]; ay[u] = S.y[i]; az[u] = S.z[i];
+        if (keep) { dx[u] = D.x[i]; dy[u] = D.y[i]; dz[u] = D.z[i]; }
         if (E.x) { hx[u] = E.x[i]; hy[u] = E.y[i]; hz[u] = E.z[i]; }
     }
     stage_fused<R, AM>(s, rep);      // own data above is in flight meanwhile
@@ -730,9 +723,13 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D, S
         llg_rhs(mc.one_opa2, mc.alpha_opa2, px[u], py[u], pz[u], fx + ex, fy + ey, fz + ez, h0, h1, h2);
         R n0, n1, n2;
         if (fold) {
-            n0 = ax[u] + s.half_dt * h0;
-            n1 = ay[u] + s.half_dt * h1;
-            n2 = az[u] + s.half_dt * h2;
+            // A = S + dt/2 dS from S and S' = (S + dt dS) / |S + dt dS|: dS is orthogonal to S, so
+            // the norm is |S|^2 / (S . S') and A = (S + N S') / 2
+            const R N = (ax[u] * ax[u] + ay[u] * ay[u] + az[u] * az[u]) /
+                        (ax[u] * px[u] + ay[u] * py[u] + az[u] * pz[u]);
+            n0 = R(0.5) * (ax[u] + N * px[u]) + s.half_dt * h0;
+            n1 = R(0.5) * (ay[u] + N * py[u]) + s.half_dt * h1;
+            n2 = R(0.5) * (az[u] + N * pz[u]) + s.half_dt * h2;
         } else {
             R e0, e1, e2;
             if (keep) {
