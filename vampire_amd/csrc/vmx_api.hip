@@ -715,8 +715,8 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
 // predictor into the spare buffer, corrector into the output buffer, roles swapped per step.
 template <typename R>
 int vmx_ctx::midpoint_steps(int nsteps, const double* noise) {
-    const SysDev<R> sf = sys_fused<R>();
-    const Launch Lf = launch_fused();
+    const SysDev<R> s = sys<R>(), sf = sys_fused<R>();
+    const Launch L = launch(), Lf = launch_fused();
     const double* nd = nullptr;
     if (noise) {
         VMX_TRY(noise_dev.alloc(3 * (size_t)n * sizeof(double)));
@@ -732,7 +732,7 @@ int vmx_ctx::midpoint_steps(int nsteps, const double* noise) {
         }
         const int p_idx = 3 - s_idx - o_idx;
         VMX_HIP(Ops<R>::mid_predictor(Lf, et, am(), sf, spins<R>(s_idx), spins<R>(p_idx), field<R>(), nd, (uint32_t)k));
-        VMX_HIP(Ops<R>::mid_corrector(Lf, et, am(), sf, spins<R>(s_idx), spins<R>(p_idx), spins<R>(o_idx), field<R>(),
+        VMX_HIP(Ops<R>::mid_corrector(L, et, am(), s, spins<R>(s_idx), spins<R>(p_idx), spins<R>(o_idx), field<R>(),
                                       nd, (uint32_t)k));
         std::swap(s_idx, o_idx);
         if (dip.on && (dip.time + k + 1) % dip.rate == 0) VMX_TRY(dipole_refresh<R>());

@@ -842,12 +842,13 @@ k_mid_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> E, const double* _
 }
 
 // Midpoint corrector: spin field at M (P), the same H_ext (kept in E, or regenerated from the
-// same Philox counter), the rotation applied to the initial S, written to O.
+// same Philox counter), the rotation applied to the initial S, written to O.  One atom per
+// thread, like the Heun corrector, which loads as much own data.
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(512)
 k_mid_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> E, const double* __restrict__ noise,
                 uint32_t step_off) {
-    constexpr int U = FusedAtoms<R>::value;
+    constexpr int U = 1;                          // launched on the one-atom block table
     constexpr bool kStaged = Access<AM>::TP != TP_CSR;
     const int lb = logical_block(s.xcd_remap);
     if (lb >= s.nblocks) return;
