@@ -305,7 +305,8 @@ int vmx_fit_backward(vmx_ctx* ctx, const double* dL_dmz, double* grad, double* d
 
 /* Chooses the workgroup size (64, 128, 256, 512) and whether workgroups are remapped to give
  * each XCD one contiguous stretch of the system, by timing a few Heun steps of each candidate
- * on this system; the result applies to later calls.  The spins and step counter are restored
+ * on this system (the better of two trials of steps_per_trial steps); the result applies to
+ * later calls.  The spins and step counter are restored
  * afterwards.  Writes the chosen block size.  (Replaces the fixed gpu::num_threads launch
  * width of the reference backend, hdr/gpu.hpp:22.) */
 int vmx_autotune(vmx_ctx* ctx, int32_t steps_per_trial, int32_t* chosen_block_size);

@@ -2029,10 +2029,14 @@ int vmx_autotune(vmx_ctx* c, int32_t steps_per_trial, int32_t* chosen) {
             c->xcd_remap = x;
             VMX_TRY(c->build_blocks());
             VMX_TRY(vmx_llg_heun(c, 2, nullptr));   // warm-up
-            VMX_TRY(vmx_llg_heun(c, steps_per_trial, nullptr));
-            double ms = 0.0;
-            VMX_TRY(vmx_last_step_time_ms(c, &ms));
-            if (ms > 0.0 && ms < best) { best = ms; best_b = b; best_x = x; }
+            double ms = 1e300;                       // best of two trials: one is noisy
+            for (int t = 0; t < 2; ++t) {
+                VMX_TRY(vmx_llg_heun(c, steps_per_trial, nullptr));
+                double m = 0.0;
+                VMX_TRY(vmx_last_step_time_ms(c, &m));
+                if (m > 0.0 && m < ms) ms = m;
+            }
+            if (ms < best) { best = ms; best_b = b; best_x = x; }
         }
     }
     c->block = best_b;
