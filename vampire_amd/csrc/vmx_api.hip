@@ -331,6 +331,10 @@ struct vmx_ctx {
     DevBuf rep_m;                 // [nrep] replica size / q
     DevBuf spin[3];               // three buffers of 3N R each (S, P, O roles rotate)
     int s_idx = 0, o_idx = 2;     // current spin buffer and the corrector's output buffer
+    // The fused correctors read their own atom's S and gather only S' (or the midpoint), so they
+    // can write S_next over S, and a step touches two spin arrays instead of three; the FP64
+    // corrector that recomputes dS gathers S and cannot.
+    bool corrector_in_place() const { return precision == VMX_FP32 || store_euler; }
     DevBuf plainF, plainE, plainD;
     int store_euler = 1;          // fused FP64 step: the predictor keeps dS (FP32 keeps nothing)
     DevBuf euler_buf;             // 3N R
@@ -667,11 +671,12 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
             VMX_HIP(Ops<R>::fused_predictor(Lf, et, am(), sf, spins<R>(s_idx), spins<R>(p_idx), euler<R>(),
                                             field<R>(), nd, (uint32_t)k));
             VMX_HIP(mark(1));
-            VMX_HIP(Ops<R>::fused_corrector(L, et, am(), s, spins<R>(s_idx), spins<R>(p_idx), spins<R>(o_idx),
+            const int out = corrector_in_place() ? s_idx : o_idx;
+            VMX_HIP(Ops<R>::fused_corrector(L, et, am(), s, spins<R>(s_idx), spins<R>(p_idx), spins<R>(out),
                                             euler<R>(), field<R>(), nd, (uint32_t)k));
             VMX_HIP(mark(2));
             nk = 2;
-            std::swap(s_idx, o_idx);
+            if (out == o_idx) std::swap(s_idx, o_idx);
         } else {
             const Spin3<R> F = arr3<R>(plainF), E = arr3<R>(plainE), D = arr3<R>(plainD);
             VMX_HIP(mark(0));
@@ -732,9 +737,9 @@ int vmx_ctx::midpoint_steps(int nsteps, const double* noise) {
         }
         const int p_idx = 3 - s_idx - o_idx;
         VMX_HIP(Ops<R>::mid_predictor(Lf, et, am(), sf, spins<R>(s_idx), spins<R>(p_idx), field<R>(), nd, (uint32_t)k));
-        VMX_HIP(Ops<R>::mid_corrector(L, et, am(), s, spins<R>(s_idx), spins<R>(p_idx), spins<R>(o_idx), field<R>(),
+        // the midpoint corrector gathers only the midpoint: S_next overwrites S
+        VMX_HIP(Ops<R>::mid_corrector(L, et, am(), s, spins<R>(s_idx), spins<R>(p_idx), spins<R>(s_idx), field<R>(),
                                       nd, (uint32_t)k));
-        std::swap(s_idx, o_idx);
         if (dip.on && (dip.time + k + 1) % dip.rate == 0) VMX_TRY(dipole_refresh<R>());
     }
     VMX_HIP(hipEventRecord(ev1, stream));
@@ -759,15 +764,16 @@ int vmx_ctx::capture_graph(int nsteps) {
         const int pi = 3 - si - oi;
         hipError_t e = Ops<R>::fused_predictor(Lf, et, am(), sf, spins<R>(si), spins<R>(pi), euler<R>(), field<R>(),
                                                nullptr, (uint32_t)k);
+        const int out = corrector_in_place() ? si : oi;
         if (e == hipSuccess)
-            e = Ops<R>::fused_corrector(L, et, am(), s, spins<R>(si), spins<R>(pi), spins<R>(oi), euler<R>(),
+            e = Ops<R>::fused_corrector(L, et, am(), s, spins<R>(si), spins<R>(pi), spins<R>(out), euler<R>(),
                                         field<R>(), nullptr, (uint32_t)k);
         if (e != hipSuccess) {
             hipGraph_t g;
             (void)hipStreamEndCapture(stream, &g);
             return fail(VMX_ERR_HIP, "kernel launch during graph capture failed: %s", hipGetErrorString(e));
         }
-        std::swap(si, oi);
+        if (out == oi) std::swap(si, oi);
     }
     hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(1), 0, stream, step_base.as<unsigned long long>(),
                        (unsigned)nsteps);
