@@ -593,9 +593,9 @@ __device__ __forceinline__ void fused_row_index(const SysDev<R>& s, int i, int h
         row_index(s, i, head, s.pat_off, r);
 }
 
-// Predictor: S' = |S + dt f(S, H_spin(S) + H_ext)|, written to P.  With D.x set the Euler
-// derivative f(S, H) (FP64) or A (FP32) is kept for the corrector, as the reference keeps its
-// euler arrays (src/simulate/LLGHeun.cpp:201-211); with E.x set H_ext is kept too, as the
+// Predictor: S' = |S + dt f(S, H_spin(S) + H_ext)|, written to P.  With D.x set the FP64 Euler
+// derivative f(S, H) is kept for the corrector, as the reference keeps its euler arrays
+// (src/simulate/LLGHeun.cpp:201-211); with E.x set H_ext is kept too, as the
 // reference keeps its external-field arrays for both stages (LLGHeun.cpp:187-197, 228-237).
 // A thread first issues the loads of its atoms' own data (spin, material, stencil id), which
 // come from HBM, then takes its atoms one at a time through row indices, neighbour gathers (mostly
