@@ -451,20 +451,24 @@ __device__ __forceinline__ void row_field(const SysDev<R>& s, const RowIdx<AM>& 
 // The array starts at 0; the thermal term overwrites it with N(0,1)*sigma (generate, then *=),
 // then the applied field is added, and last the macrocell dipole field when one is set
 // (calculate_dipolar_fields, fields.cpp:166, 291-310).
+// The unit normals of the thermal field at atom i: injected (noise) or from Philox.
 template <typename R>
-__device__ __forceinline__ void ext_field(const SysDev<R>& s, const Cond<R>& c, int rep, int i, int local,
-                                          uint64_t step, const double* __restrict__ noise,
-                                          R& ex, R& ey, R& ez) {
+__device__ __forceinline__ void unit_normals(const SysDev<R>& s, int rep, int i, int local, uint64_t step,
+                                             const double* __restrict__ noise, R& gx, R& gy, R& gz) {
+    if (noise) {
+        gx = (R)noise[i];
+        gy = (R)noise[(size_t)s.n + i];
+        gz = (R)noise[2 * (size_t)s.n + i];
+    } else {
+        thermal_normals(s, rep, local, step, gx, gy, gz);
+    }
+}
+// H_ext from the atom's unit normals (read only when the thermal field is on).
+template <typename R>
+__device__ __forceinline__ void ext_from_normals(const SysDev<R>& s, const Cond<R>& c, int i, R gx, R gy, R gz,
+                                                 R& ex, R& ey, R& ez) {
     ex = R(0); ey = R(0); ez = R(0);
     if (s.fld & FLD_THERMAL) {
-        R gx, gy, gz;
-        if (noise) {
-            gx = (R)noise[i];
-            gy = (R)noise[(size_t)s.n + i];
-            gz = (R)noise[2 * (size_t)s.n + i];
-        } else {
-            thermal_normals(s, rep, local, step, gx, gy, gz);
-        }
         ex = gx * c.sigma; ey = gy * c.sigma; ez = gz * c.sigma;
     }
     if (s.fld & FLD_APPLIED) {
@@ -473,6 +477,14 @@ __device__ __forceinline__ void ext_field(const SysDev<R>& s, const Cond<R>& c, 
     if (s.hdip) {
         ex += s.hdip[i]; ey += s.hdip[(size_t)s.n + i]; ez += s.hdip[2 * (size_t)s.n + i];
     }
+}
+template <typename R>
+__device__ __forceinline__ void ext_field(const SysDev<R>& s, const Cond<R>& c, int rep, int i, int local,
+                                          uint64_t step, const double* __restrict__ noise,
+                                          R& ex, R& ey, R& ez) {
+    R gx = R(0), gy = R(0), gz = R(0);
+    if (s.fld & FLD_THERMAL) unit_normals(s, rep, i, local, step, noise, gx, gy, gz);
+    ext_from_normals(s, c, i, gx, gy, gz, ex, ey, ez);
 }
 template <typename R>
 __device__ __forceinline__ void ext_field(const SysDev<R>& s, int rep, int m, int i, int local,
@@ -563,6 +575,20 @@ __host__ __device__ inline size_t fused_lds_bytes(int nmat, int npat, int zmax, 
     const size_t pat = pat_lds ? (size_t)npat * (size_t)zmax : 0;
     return (((rec + 3) & ~(size_t)3) + pat) * sizeof(int);
 }
+// Thermal unit normals of a thread's atoms, formed before the LDS tables are staged: they depend
+// only on the atom and the step, so their arithmetic runs while the own-data loads are in flight.
+template <typename R, int U>
+__device__ __forceinline__ void early_normals(const SysDev<R>& s, int rep, const int4& b, const int (&idx)[U],
+                                              uint64_t step, const double* __restrict__ noise, bool want,
+                                              R (&gx)[U], R (&gy)[U], R (&gz)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        gx[u] = R(0); gy[u] = R(0); gz[u] = R(0);
+        if (want && (s.fld & FLD_THERMAL))
+            unit_normals(s, rep, idx[u], caller_local(s, rep, idx[u] - b.w), step, noise, gx[u], gy[u], gz[u]);
+    }
+}
+
 template <typename R, int AM>
 __device__ __forceinline__ void stage_fused(const SysDev<R>& s, int rep) {
     using L = FusedLds<R>;
@@ -598,8 +624,8 @@ __device__ __forceinline__ void fused_row_index(const SysDev<R>& s, int i, int h
 // (src/simulate/LLGHeun.cpp:201-211); with E.x set H_ext is kept too, as the
 // reference keeps its external-field arrays for both stages (LLGHeun.cpp:187-197, 228-237).
 // A thread first issues the loads of its atoms' own data (spin, material, stencil id), which
-// come from HBM, then takes its atoms one at a time through row indices, neighbour gathers (mostly
-// L2 hits) and the step.
+// come from HBM, and forms their thermal normals while those loads are in flight; it then takes
+// its atoms one at a time through row indices, neighbour gathers (mostly L2 hits) and the step.
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(512)
 k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, Spin3<R> E, const double* __restrict__ noise,
@@ -622,9 +648,11 @@ k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, Spin3<R> E, c
         rs.load(idx[u], sx[u], sy[u], sz[u]);
         if (kStaged) head[u] = row_head<R, AM>(s, idx[u]);
     }
+    const uint64_t step = *s.step_base + step_off;
+    R gx[U], gy[U], gz[U];
+    early_normals<R, U>(s, rep, b, idx, step, noise, true, gx, gy, gz);
     stage_fused<R, AM>(s, rep);
     if (!ok[0]) return;
-    const uint64_t step = *s.step_base + step_off;
     const UniformJ<R, ET> ju(s, Access<AM>::JM);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -642,7 +670,7 @@ k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, Spin3<R> E, c
         }
         const Cond<R>& c = FusedLds<R>::cond(s)[m[u]];
         R ex, ey, ez;
-        ext_field(s, c, rep, i, caller_local(s, rep, i - b.w), step, noise, ex, ey, ez);
+        ext_from_normals(s, c, i, gx[u], gy[u], gz[u], ex, ey, ez);
         R d0, d1, d2;
         llg_rhs(mc.one_opa2, mc.alpha_opa2, sx[u], sy[u], sz[u], fx + ex, fy + ey, fz + ez, d0, d1, d2);
         R n0 = sx[u] + d0 * s.dt, n1 = sy[u] + d1 * s.dt, n2 = sz[u] + d2 * s.dt;
@@ -694,9 +722,11 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D, S
         if (keep) { dx[u] = D.x[i]; dy[u] = D.y[i]; dz[u] = D.z[i]; }
         if (E.x) { hx[u] = E.x[i]; hy[u] = E.y[i]; hz[u] = E.z[i]; }
     }
+    const uint64_t step = *s.step_base + step_off;
+    R gx[U], gy[U], gz[U];                        // unless H_ext is read back
+    early_normals<R, U>(s, rep, b, idx, step, noise, E.x == nullptr, gx, gy, gz);
     stage_fused<R, AM>(s, rep);      // own data above is in flight meanwhile
     if (!ok[0]) return;
-    const uint64_t step = *s.step_base + step_off;
     const UniformJ<R, ET> ju(s, Access<AM>::JM);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -717,7 +747,7 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D, S
             ex = hx[u]; ey = hy[u]; ez = hz[u];
         } else {
             const Cond<R>& c = FusedLds<R>::cond(s)[m[u]];
-            ext_field(s, c, rep, i, caller_local(s, rep, i - b.w), step, noise, ex, ey, ez);
+            ext_from_normals(s, c, i, gx[u], gy[u], gz[u], ex, ey, ez);
         }
         R h0, h1, h2;
         llg_rhs(mc.one_opa2, mc.alpha_opa2, px[u], py[u], pz[u], fx + ex, fy + ey, fz + ez, h0, h1, h2);
@@ -808,9 +838,11 @@ k_mid_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> E, const double* _
         rs.load(idx[u], sx[u], sy[u], sz[u]);
         if (kStaged) head[u] = row_head<R, AM>(s, idx[u]);
     }
+    const uint64_t step = *s.step_base + step_off;
+    R gx[U], gy[U], gz[U];
+    early_normals<R, U>(s, rep, b, idx, step, noise, true, gx, gy, gz);
     stage_fused<R, AM>(s, rep);
     if (!ok[0]) return;
-    const uint64_t step = *s.step_base + step_off;
     const UniformJ<R, ET> ju(s, Access<AM>::JM);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -828,7 +860,7 @@ k_mid_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> E, const double* _
         }
         const Cond<R>& c = FusedLds<R>::cond(s)[m[u]];
         R ex, ey, ez;
-        ext_field(s, c, rep, i, caller_local(s, rep, i - b.w), step, noise, ex, ey, ez);
+        ext_from_normals(s, c, i, gx[u], gy[u], gz[u], ex, ey, ez);
         R o0, o1, o2;
         cayley(mc.alpha, midpoint_beta(s, mc), sx[u], sy[u], sz[u], fx + ex, fy + ey, fz + ez, sx[u], sy[u], sz[u],
                o0, o1, o2);
@@ -869,9 +901,11 @@ k_mid_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> E, con
         ax[u] = S.x[i]; ay[u] = S.y[i]; az[u] = S.z[i];
         if (E.x) { hx[u] = E.x[i]; hy[u] = E.y[i]; hz[u] = E.z[i]; }
     }
+    const uint64_t step = *s.step_base + step_off;
+    R gx[U], gy[U], gz[U];                        // unless H_ext is read back
+    early_normals<R, U>(s, rep, b, idx, step, noise, E.x == nullptr, gx, gy, gz);
     stage_fused<R, AM>(s, rep);
     if (!ok[0]) return;
-    const uint64_t step = *s.step_base + step_off;
     const UniformJ<R, ET> ju(s, Access<AM>::JM);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -892,7 +926,7 @@ k_mid_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> E, con
             ex = hx[u]; ey = hy[u]; ez = hz[u];
         } else {
             const Cond<R>& c = FusedLds<R>::cond(s)[m[u]];
-            ext_field(s, c, rep, i, caller_local(s, rep, i - b.w), step, noise, ex, ey, ez);
+            ext_from_normals(s, c, i, gx[u], gy[u], gz[u], ex, ey, ez);
         }
         R o0, o1, o2;
         cayley(mc.alpha, midpoint_beta(s, mc), px[u], py[u], pz[u], fx + ex, fy + ey, fz + ez, ax[u], ay[u], az[u],
