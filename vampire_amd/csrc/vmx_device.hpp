@@ -508,6 +508,10 @@ __device__ __forceinline__ double inv_norm(double x, double y, double z) {
 __device__ __forceinline__ float inv_norm(float x, float y, float z) {
     return __builtin_amdgcn_rsqf(x * x + y * y + z * z);
 }
+// a / b through the hardware reciprocal (about 1 ulp) in FP32, where b is of order one; an exact
+// division in FP64.
+__device__ __forceinline__ float rcp_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double rcp_div(double a, double b) { return a / b; }
 
 // ======================= fused kernels: two launches per Heun step ========================
 // FP32 keeps nothing between the two kernels: its corrector rebuilds A = S + dt/2 dS from S and
@@ -755,8 +759,8 @@ k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D, S
         if (fold) {
             // A = S + dt/2 dS from S and S' = (S + dt dS) / |S + dt dS|: dS is orthogonal to S, so
             // the norm is |S|^2 / (S . S') and A = (S + N S') / 2
-            const R N = (ax[u] * ax[u] + ay[u] * ay[u] + az[u] * az[u]) /
-                        (ax[u] * px[u] + ay[u] * py[u] + az[u] * pz[u]);
+            const R N = rcp_div(ax[u] * ax[u] + ay[u] * ay[u] + az[u] * az[u],
+                                ax[u] * px[u] + ay[u] * py[u] + az[u] * pz[u]);
             n0 = R(0.5) * (ax[u] + N * px[u]) + s.half_dt * h0;
             n1 = R(0.5) * (ay[u] + N * py[u]) + s.half_dt * h1;
             n2 = R(0.5) * (az[u] + N * pz[u]) + s.half_dt * h2;
