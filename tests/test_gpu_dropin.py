@@ -199,3 +199,41 @@ def test_reference_time_series_deck_brownian_relaxation(gpu, exe, tmp_path):
     for run in (cpu, dev):
         assert np.abs(run[:, 5] - theory).max() <= 0.03, (run[:, 5], theory)
     assert np.abs(cpu[:, 5] - dev[:, 5]).max() <= 0.03
+
+
+def run_checkpoint_series(tmp, acc):
+    """The same 300 K deck (FCC 3x3x3 cells, random start) run 200 steps without a break (A), and
+    run 100 steps with sim:save-checkpoint=end (B1) and then from its vampire0.chk to step 200 with
+    sim:load-checkpoint=continue (B2) or =restart (R); returns the final spins of A, B2 and R."""
+    head = LATTICE.format(crystal="fcc", size=10.61)
+    mat = CO.format(alpha=0.1, ku=4.644e-24, init="random")
+    runs = {"A": (200, ""), "B1": (100, "sim:save-checkpoint=end\n"),
+            "B2": (200, "sim:load-checkpoint=continue\n"), "R": (200, "sim:load-checkpoint=restart\n")}
+    out = {}
+    for name, (steps, extra) in runs.items():
+        d = tmp / f"{acc}_{name}"
+        d.mkdir()
+        kw = dict(T=300.0, inc=50, steps=steps, H=0.5, Hdir="0,0,1", acc=acc)
+        (d / "input").write_text(head + RUN.format(**kw) + extra)
+        (d / "Co.mat").write_text(mat)
+        if name in ("B2", "R"):
+            (d / "vampire0.chk").write_bytes((tmp / f"{acc}_B1" / "vampire0.chk").read_bytes())
+        r = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        snap = sorted(d.glob("spins-*.data"))[-1]      # the file counter continues from the checkpoint
+        out[name] = np.frombuffer(snap.read_bytes()[8:], dtype=np.float64).copy()
+    return out["A"], out["B2"], out["R"]
+
+
+def test_dropin_checkpoint_continue_resumes_noise_stream(gpu, exe, tmp_path):
+    """The reference's own checkpoint (src/utility/checkpoint.cpp, vampire<rank>.chk) through the
+    drop-in at 300 K.  The reference CPU path resumes bit for bit: its MT19937 state is in the file.
+    The device path resumes bit for bit too: the spins come back through gpu::finalize before
+    save_checkpoint (sim.cpp:521-524), and gpu::initialize starts the Philox counter at the restored
+    sim::time.  A restart from the same file (sim::time = 0) ends elsewhere, so the noise stream is
+    what the comparison sees."""
+    for acc in ("off", "on"):
+        a, b, r = run_checkpoint_series(tmp_path, acc)
+        assert a.shape == (324,)
+        assert np.array_equal(a, b), (acc, np.abs(a - b).max())
+        assert np.abs(a - r).max() > 1e-3, acc
