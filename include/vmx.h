@@ -217,6 +217,12 @@ int vmx_llg_heun_graph(vmx_ctx* ctx, int32_t nsteps, int32_t nrepeat);
 int vmx_get_step(vmx_ctx* ctx, int64_t* step);
 int vmx_set_step(vmx_ctx* ctx, int64_t step);
 
+/* Number of Metropolis sweeps taken so far (the Philox counter of vmx_mc_sweep).  With
+ * vmx_set_step, the device counterpart of the random-generator state that the reference's
+ * checkpoint restores (src/utility/checkpoint.cpp:166). */
+int vmx_get_sweep(vmx_ctx* ctx, int64_t* sweep);
+int vmx_set_sweep(vmx_ctx* ctx, int64_t sweep);
+
 /* Checkerboard Metropolis: nsweeps sweeps; each sweep visits every atom once, one graph colour
  * at a time (colours computed at the first call), with the adaptive Gaussian cone move and
  * acceptance of montecarlo::mc_step (src/montecarlo/mc.cpp:29-127, mc_moves.cpp:71-87,158-162).

@@ -72,3 +72,28 @@ def test_checkpoint_rejects_other_contexts(gpu):
             d.restore(chk[: len(chk) // 2])
         with pytest.raises(VmxError, match="not a libvmx checkpoint"):
             d.restore(b"X" * len(chk))
+
+
+def test_step_and_sweep_counters_drive_the_noise(gpu):
+    """The Heun step and Metropolis sweep counters are the Philox counters: they count the calls,
+    can be set (the drop-in starts both at the restored sim::time), and two contexts with equal
+    spins and counters draw equal noise, while a different counter draws different noise."""
+    s = S.ferromagnet("sc", 6)
+    sp = S.random_spins(s.n, 8)
+
+    def run(step, sweep):
+        with Context(s, precision=FP64, seed=5) as c:
+            c.set_conditions(400.0, (0.0, 0.0, 0.2))
+            c.set_spins(sp)
+            c.step, c.sweep = step, sweep
+            assert (c.step, c.sweep) == (step, sweep)
+            c.mc_sweep(2)
+            c.llg_heun(3)
+            assert (c.step, c.sweep) == (step + 3, sweep + 2)
+            return c.get_spins()
+
+    base = run(0, 0)
+    assert np.array_equal(run(0, 0), base)
+    assert np.array_equal(run(40, 17), run(40, 17))
+    assert np.abs(run(40, 0) - base).max() > 1e-3       # Heun noise from another counter
+    assert np.abs(run(0, 17) - base).max() > 1e-3       # Metropolis moves from another counter

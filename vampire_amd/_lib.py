@@ -138,6 +138,8 @@ SIGNATURES = [
     ("vmx_llg_heun_graph", C.c_int, [_VP, C.c_int32, C.c_int32]),
     ("vmx_get_step", C.c_int, [_VP, _P(C.c_int64)]),
     ("vmx_set_step", C.c_int, [_VP, C.c_int64]),
+    ("vmx_get_sweep", C.c_int, [_VP, _P(C.c_int64)]),
+    ("vmx_set_sweep", C.c_int, [_VP, C.c_int64]),
     ("vmx_mc_sweep", C.c_int, [_VP, C.c_int32]),
     ("vmx_mc_get_state", C.c_int, [_VP, _DP, _DP, _DP]),
     ("vmx_mc_set_sigma", C.c_int, [_VP, C.c_int32, C.c_double]),

@@ -143,6 +143,17 @@ class Context:
     def step(self, value: int) -> None:
         L.check(self.lib.vmx_set_step(self.h, int(value)))
 
+    @property
+    def sweep(self) -> int:
+        """Metropolis sweeps taken so far (the Philox counter of mc_sweep)."""
+        v = C.c_int64(0)
+        L.check(self.lib.vmx_get_sweep(self.h, C.byref(v)))
+        return int(v.value)
+
+    @sweep.setter
+    def sweep(self, value: int) -> None:
+        L.check(self.lib.vmx_set_sweep(self.h, int(value)))
+
     # --- integration ------------------------------------------------------------------------
     def llg_heun(self, nsteps: int = 1, noise: Optional[np.ndarray] = None) -> None:
         if noise is None:

@@ -1619,6 +1619,18 @@ int vmx_set_step(vmx_ctx* c, int64_t step) {
     return VMX_OK;
 }
 
+int vmx_get_sweep(vmx_ctx* c, int64_t* sweep) {
+    if (!c || !sweep) return fail(VMX_ERR_ARG, "null argument");
+    *sweep = (int64_t)c->sweep;
+    return VMX_OK;
+}
+
+int vmx_set_sweep(vmx_ctx* c, int64_t sweep) {
+    if (!c || sweep < 0) return fail(VMX_ERR_ARG, "bad argument");
+    c->sweep = (uint64_t)sweep;   // a launch argument of each sweep: nothing on the device to update
+    return VMX_OK;
+}
+
 // ---- checkpoints -------------------------------------------------------------------------
 // Layout (little-endian, 8-byte fields):
 //   "VMXCHK01" | n | nrep | precision | field flags | Heun step | Metropolis sweep | seed

@@ -170,12 +170,13 @@ void initialize() {
 
     check(vmx_create(&s, &o, &g_ctx), "vmx_create");
     check(vmx_set_time_step(g_ctx, mp::dt, mp::half_dt), "vmx_set_time_step");
-    // The thermal field's Philox counter starts at sim::time.  A fresh run starts at 0; with
-    // sim:load-checkpoint=continue, load_checkpoint has restored sim::time before this call
-    // (src/utility/checkpoint.cpp:176-178, called at src/simulate/sim.cpp:246), so the resumed run
-    // continues the noise stream of the run that wrote the file, as the reference does by
-    // restoring its MT19937 state (checkpoint.cpp:166).
+    // The Philox counters (Heun and midpoint steps, Metropolis sweeps) start at sim::time.  A
+    // fresh run starts at 0; with sim:load-checkpoint=continue, load_checkpoint has restored
+    // sim::time before this call (src/utility/checkpoint.cpp:176-178, called at
+    // src/simulate/sim.cpp:246), so the resumed run continues the noise stream of the run that
+    // wrote the file, as the reference does by restoring its MT19937 state (checkpoint.cpp:166).
     check(vmx_set_step(g_ctx, (int64_t)sim::time), "vmx_set_step");
+    check(vmx_set_sweep(g_ctx, (int64_t)sim::time), "vmx_set_sweep");
     check(vmx_set_spins(g_ctx, atoms::x_spin_array.data(), atoms::y_spin_array.data(), atoms::z_spin_array.data()),
           "vmx_set_spins");
     push_conditions();
