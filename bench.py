@@ -229,6 +229,67 @@ def cpu_sample(system, threads, seconds):
     return cpu_baseline(system, steps, threads), steps
 
 
+REF_EXE = os.path.join(ROOT, "vampire_amd", "shim", "_build", "vampire-vmx")
+# The CPU-baseline workload as a VAMPIRE input deck: FCC 100^3 cells (4M atoms), periodic,
+# J = 6.064e-21 J, Ku = 4.644e-24 J along z, mu_s = 1.72 mu_B, alpha = 0.05, spins +z, 600 K,
+# dt = 1e-16 s, sim:program=benchmark (src/program/bmark.cpp: the steps, then one statistics
+# update and one data line).
+REF_INPUT = """create:crystal-structure=fcc
+create:periodic-boundaries-x
+create:periodic-boundaries-y
+create:periodic-boundaries-z
+dimensions:unit-cell-size = 3.54 !A
+dimensions:system-size-x = {size} !A
+dimensions:system-size-y = {size} !A
+dimensions:system-size-z = {size} !A
+material:file=Co.mat
+sim:temperature=600
+sim:time-steps-increment={steps}
+sim:total-time-steps={steps}
+sim:time-step=1.0E-16
+sim:program=benchmark
+sim:integrator=llg-heun
+gpu:acceleration=off
+output:time-steps
+"""
+REF_MAT = """material:num-materials=1
+material[1]:material-name=Co
+material[1]:damping-constant=0.05
+material[1]:exchange-matrix[1]=6.064e-21
+material[1]:atomic-spin-moment=1.72 !muB
+material[1]:second-order-uniaxial-anisotropy-constant=4.644e-24
+material[1]:initial-spin-direction=0,0,1
+"""
+
+
+def reference_seconds(cells, steps):
+    """Wall time of one run of the reference's own CPU integrator on FCC cells^3: the drop-in
+    executable (VAMPIRE built from its sources, its src/gpu dispatchers replaced) with
+    gpu:acceleration=off runs sim::LLG_Heun on one thread, as vampire-serial does.  Set-up
+    included."""
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "input"), "w") as f:
+            f.write(REF_INPUT.format(size=f"{3.54 * cells - 0.01:.2f}", steps=steps))
+        with open(os.path.join(d, "Co.mat"), "w") as f:
+            f.write(REF_MAT)
+        t0 = time.perf_counter()
+        subprocess.run([REF_EXE], cwd=d, capture_output=True, timeout=900, check=True)
+        dt = time.perf_counter() - t0
+        rows = [ln for ln in open(os.path.join(d, "output")).read().splitlines() if ln.strip() and ln[0] != "#"]
+        assert rows and int(float(rows[-1].split()[0])) == steps, "reference run ended early"
+        return dt
+
+
+def reference_rate(n_atoms, cells, steps):
+    """Spin-steps/s of the reference's Heun step: the difference of a 2-step and a
+    (2 + steps)-step run, which cancels the set-up."""
+    t_short = reference_seconds(cells, 2)
+    t_long = reference_seconds(cells, 2 + steps)
+    return n_atoms * steps / max(t_long - t_short, 1e-9)
+
+
 def cpu_baseline(system, nsteps, threads):
     from oracle import oracle as O
     o = O.Oracle(system)
@@ -375,11 +436,24 @@ def main():
         nthreads = min(16, os.cpu_count() or 1)
         (v1, k1), (vN, kN) = (cpu_sample(system, 1, args.cpu_seconds),
                               cpu_sample(system, nthreads, args.cpu_seconds))
-        cpu = dict(value=round(vN, 1), unit="spin-steps/s", cores=nthreads, kind="port",
-                   sample=f"{kN} Heun steps of the same 4M-atom FCC system (FP64 oracle, OpenMP over {nthreads} "
-                          f"threads, >= {args.cpu_seconds:.0f} s); 1 thread: {k1} steps, {v1:.4g} spin-steps/s",
-                   value_1thread=round(v1, 1))
-        print(f"# cpu: 1 thread {v1:.3e}, {nthreads} threads {vN:.3e} spin-steps/s", file=sys.stderr, flush=True)
+        port = dict(value=round(vN, 1), cores=nthreads, steps=kN, value_1thread=round(v1, 1), steps_1thread=k1,
+                    what="FP64 oracle (the reference's loops restated in C), same workload, OpenMP")
+        print(f"# cpu port: 1 thread {v1:.3e}, {nthreads} threads {vN:.3e} spin-steps/s", file=sys.stderr, flush=True)
+        if os.path.exists(REF_EXE):
+            ref_steps = max(4, int(round(args.cpu_seconds * 2.5)))
+            vr = reference_rate(system.n, args.L, ref_steps)
+            cpu = dict(value=round(vr, 1), unit="spin-steps/s", cores=1, kind="reference",
+                       sample=f"{ref_steps} Heun steps of the same FCC {args.L}^3 system ({system.n} atoms, 600 K, spins +z) by the "
+                              f"reference's own sim::LLG_Heun: VAMPIRE built from its sources with gpu:acceleration=off, "
+                              f"one thread as vampire-serial; the rate is the difference of a 2-step and a "
+                              f"{2 + ref_steps}-step run, which cancels the set-up",
+                       port=port, port_1thread_over_reference=round(v1 / vr, 3))
+            print(f"# cpu reference: {vr:.3e} spin-steps/s (1 thread)", file=sys.stderr, flush=True)
+        else:
+            cpu = dict(value=port["value"], unit="spin-steps/s", cores=nthreads, kind="port",
+                       sample=f"{kN} Heun steps of the same 4M-atom FCC system (FP64 oracle, OpenMP over {nthreads} "
+                              f"threads, >= {args.cpu_seconds:.0f} s); 1 thread: {k1} steps, {v1:.4g} spin-steps/s",
+                       value_1thread=port["value_1thread"])
 
     def leg(name, p):
         if name not in res:
