@@ -1,14 +1,15 @@
 """Benchmark: stochastic LLG-Heun spin-steps/s at 4M atoms (FCC 100^3, 12 neighbours, 600 K),
 FP32 fused kernels (the headline value) beside FP64 fused and the plain FP64 kernels, with the
-HBM roofline of the Heun step and the CPU restatement of the reference timed on the host.
+HBM roofline of the Heun step and the reference's own CPU Heun step timed on the host.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--L 100] [--cpu-seconds 10] [--no-cpu]
                     [--batching-steps 200] [--legs fp32_fused,fp64_fused,fp64_plain,fp32_midpoint]
 
 One "step" is one Heun step (predictor + corrector) over all 4,000,000 spins, inputs resident
 in HBM.  Also reported: the FP32-vs-FP64 drift of the magnetisation, the replica-batching leg
-(256 SC replicas in one context against one context per replica) and the oracle timed on the
-host cores.  Prints one JSON line.
+(256 SC replicas in one context against one context per replica), and on the host the
+reference's sim::LLG_Heun (the drop-in executable with gpu:acceleration=off, one thread) with the
+oracle beside it.  Prints one JSON line.
 """
 from __future__ import annotations
 
