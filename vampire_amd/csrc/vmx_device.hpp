@@ -693,8 +693,9 @@ k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, Spin3<R> E, c
 
 // Corrector: dS' at S' with the same H_ext (read back when the predictor kept it, else
 // regenerated from the same Philox counter), then S_next = |S + dt/2 (dS + dS')|
-// (LLGHeun.cpp:257-269) with dS read back when the predictor kept it, else recomputed; FP32
-// with a kept A forms |A + dt/2 dS'|.  Own data of all the thread's atoms first, as above.
+// (LLGHeun.cpp:257-269) with dS read back when the predictor kept it, else recomputed (FP64);
+// FP32 rebuilds A = S + dt/2 dS from S and S' and forms |A + dt/2 dS'|.  S_next may overwrite S.
+// Own data of all the thread's atoms first, as above.
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(512)
 k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D, Spin3<R> E,
