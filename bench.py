@@ -357,8 +357,8 @@ def main():
                     help="end with three 256 MiB device-to-device copies (counter scale for traffic)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (replicas and spatial shards "
-              "of this path would run one process per GPU, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build runs on one GPU "
+              "(replicas and spatial z-slabs with a halo exchange would run one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     from vampire_amd import system as S
