@@ -1,7 +1,7 @@
 """ctypes binding of oracle/liboracle.so — TEST INFRASTRUCTURE.
 
-The oracle is the checker for the device path: only tests/, __graft_entry__.smoke() and
-bench.py's CPU-baseline leg use it.  See vmx_oracle.c for the reference functions it restates.
+The oracle is what the device path is checked against: only tests/, __graft_entry__.smoke()
+and bench.py's CPU-baseline leg use it.  See vmx_oracle.c for the reference functions it restates.
 """
 from __future__ import annotations
 

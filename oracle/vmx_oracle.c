@@ -3,8 +3,8 @@
  *
  * TEST INFRASTRUCTURE.  This file restates, in plain C and in the reference's own loop and
  * operation order, the VAMPIRE CPU functions the device path replaces.  It is used only by
- * tests/ (parity checks), __graft_entry__.smoke() (as the checker) and bench.py (the timed CPU
- * baseline).  The product path (vampire_amd/, libvmx.so) never calls it.
+ * tests/ (parity checks), __graft_entry__.smoke() (the reference result of its small run) and
+ * bench.py (the timed CPU baseline).  The product path (vampire_amd/, libvmx.so) never calls it.
  *
  * Built by __graft_entry__.build() as oracle/liboracle.so:
  *   gcc -O3 -std=c99 -ffp-contract=off -fopenmp -fPIC -shared vmx_oracle.c -lm
