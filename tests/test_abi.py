@@ -57,7 +57,7 @@ def _sys_struct(s, **over):
     return st, keep, mats
 
 
-@pytest.mark.parametrize("bad", ["neighbour", "iid", "type", "range", "precision", "block"])
+@pytest.mark.parametrize("bad", ["neighbour", "iid", "type", "range", "precision", "block", "empty"])
 def test_create_rejects_bad_input_on_host(bad):
     lib = L.load()
     s = S.ferromagnet("sc", 3)
@@ -76,6 +76,8 @@ def test_create_rejects_bad_input_on_host(bad):
     elif bad == "block":
         opt.block_size = 100
     st, keep, mats = _sys_struct(s, **over)
+    if bad == "empty":
+        st.num_atoms = 0
     h = C.c_void_p()
     rc = lib.vmx_create(C.byref(st), C.byref(opt), C.byref(h))
     assert rc == L.VMX_ERR_ARG
