@@ -1197,8 +1197,19 @@ k_mc_colour(SysDev<R> s, Spin3<R> S, const int* __restrict__ list, int count, in
             rej = 1.0;
         }
     }
-    // rejects per replica: a workgroup spans at most a few replicas; reduce when uniform
-    if (myrep >= 0 && rej != 0.0) atomicAdd(&rejects[myrep], rej);
+    // Rejections per replica.  Colour lists run in atom order, so a wavefront's lanes almost always
+    // share one replica: then one lane adds the wavefront's count, instead of one atomic per
+    // rejected move on a handful of contended addresses.  Counts are integers, exact in a double
+    // in any order.
+    const unsigned long long rejected = __ballot(rej != 0.0);
+    if (rejected == 0ull) return;
+    const int first = __ffsll(rejected) - 1;              // a lane that rejected
+    const int rep0 = __shfl(myrep, first);
+    if (__all(rej == 0.0 || myrep == rep0)) {
+        if ((int)__lane_id() == first) atomicAdd(&rejects[rep0], (double)__popcll(rejected));
+    } else if (rej != 0.0) {
+        atomicAdd(&rejects[myrep], rej);
+    }
 }
 
 }  // namespace vmx
