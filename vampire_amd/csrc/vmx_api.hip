@@ -127,15 +127,42 @@ __global__ void k_fill(size_t n, R* __restrict__ a, R v) {
 }
 
 // Reduces the workgroup partials in block order, then normalises (magnetization.cpp:150-178).
+// Sum over the nblk per-workgroup partials [nblk][W] of every value k < W, handed to put(k, sum).
+// With fewer values than threads, each value's partials are split over blockDim / W threads whose
+// sub-sums are then added in thread order: the whole block works and the order is fixed, so the
+// result is the same on every run.  With many values, one thread per value.
+template <typename F>
+__device__ void reduce_partials(int nblk, const double* __restrict__ partial, int W, F&& put) {
+    __shared__ double red[256];                   // the finish kernels run one block of 256 threads
+    const int per = (int)blockDim.x / W;
+    if (per >= 2) {
+        const int k = (int)threadIdx.x % W, j = (int)threadIdx.x / W;
+        double a = 0.0;
+        if (j < per) {
+#pragma unroll 8
+            for (int b = j; b < nblk; b += per) a += partial[(size_t)b * W + k];
+        }
+        red[threadIdx.x] = a;
+        __syncthreads();
+        if ((int)threadIdx.x < W) {
+            double t = 0.0;
+            for (int q = 0; q < per; ++q) t += red[q * W + (int)threadIdx.x];
+            put((int)threadIdx.x, t);
+        }
+    } else {
+        for (int k = threadIdx.x; k < W; k += blockDim.x) {
+            double acc = 0.0;
+            for (int b = 0; b < nblk; ++b) acc += partial[(size_t)b * W + k];
+            put(k, acc);
+        }
+    }
+}
+
 __global__ void k_mag_finish(int nblk, const double* __restrict__ partial, int nbins, double* cur,
                              double* mean, const int* __restrict__ empty, int nempty,
                              long long* counter) {
     const int W = 4 * nbins;
-    for (int k = threadIdx.x; k < W; k += blockDim.x) {
-        double acc = 0.0;
-        for (int b = 0; b < nblk; ++b) acc += partial[(size_t)b * W + k];
-        cur[k] = acc;
-    }
+    reduce_partials(nblk, partial, W, [&](int k, double v) { cur[k] = v; });
     __threadfence_block();
     __syncthreads();
     for (int id = threadIdx.x; id < nbins - 1; id += blockDim.x) {
@@ -165,12 +192,10 @@ __global__ void k_energy_finish(int nblk, const double* __restrict__ partial, in
                                 double* mean, const int* __restrict__ empty, int nempty,
                                 long long* counter) {
     const int W = 4 * nbins;
-    for (int k = threadIdx.x; k < W; k += blockDim.x) {
-        double acc = 0.0;
-        for (int b = 0; b < nblk; ++b) acc += partial[(size_t)b * W + k];
+    reduce_partials(nblk, partial, W, [&](int k, double v) {
         const int id = k / 4, term = k % 4;     // term: exchange, anisotropy, applied, magnetostatic
-        cur[(size_t)(1 + term) * nbins + id] = acc;
-    }
+        cur[(size_t)(1 + term) * nbins + id] = v;
+    });
     for (int id = threadIdx.x; id < nbins; id += blockDim.x) cur[id] = 0.0;
     __threadfence_block();
     __syncthreads();
@@ -786,8 +811,12 @@ int vmx_ctx::capture_graph(int nsteps) {
 template <typename R>
 int vmx_ctx::stats_update() {
     const Spin3<R> S = spins<R>(s_idx);
-    const int grid = std::max(1, std::min((n + 255) / 256, 512));
+    // Workgroups of the partial-sum kernels.  Each thread walks its atoms with a grid stride, one
+    // dependent memory round trip per atom, so masks with few bins get eight workgroups per CU
+    // (2048); masks with many bins keep 512, which bounds the partials k_*_finish reduces.
+    auto grid_for = [&](int nbins) { return std::max(1, std::min((n + 255) / 256, nbins <= 16 ? 2048 : 512)); };
     for (MagStat* st : mags) {
+        const int grid = grid_for(st->nbins);
         VMX_TRY(partial.alloc((size_t)grid * 4 * st->nbins * sizeof(double)));
         VMX_HIP(Ops<R>::mag_partial(grid, stream, n, S, st->mask.as<int>(), mm.as<double>(), st->nbins,
                                     partial.as<double>()));
@@ -806,6 +835,7 @@ int vmx_ctx::stats_update() {
             VMX_HIP(hipMemcpy(atom_rep.p, ar.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
         }
         for (EnergyStat* st : energies) {
+            const int grid = grid_for(st->nbins);
             VMX_TRY(partial.alloc((size_t)grid * 4 * st->nbins * sizeof(double)));
             VMX_HIP(Ops<R>::energy_partial(grid, stream, et, am(),s, S, st->mask.as<int>(), mm.as<double>(),
                                            nrep > 1 ? atom_rep.as<int>() : nullptr, st->nbins,
