@@ -1006,6 +1006,52 @@ k_plain_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> F, Spin3<R> E, S
 // so a system-wide mask (one id) costs one LDS atomic per thread.  Each workgroup writes its
 // histogram to partial[block][nbins*W]; k_stats_finish reduces the partials in block order.
 
+// Masks of at most MAXB bins: each thread keeps every bin's four sums in registers and adds each
+// atom to its bin with predicated adds, so the cost does not depend on how often the id changes
+// along the atoms (a random alloy's material mask changes at nearly every atom).  The workgroup
+// then reduces its threads' sums in a fixed order (wavefront shuffles, then the wavefronts in
+// order) into partial[block][nbins*4], the layout k_mag_partial writes.
+template <typename R, int MAXB>
+__global__ void __launch_bounds__(256)
+k_mag_partial_regs(int n, Spin3<R> S, const int* __restrict__ mask, const double* __restrict__ mm,
+                   int nbins, double* __restrict__ partial) {
+    double acc[MAXB][4];
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) { acc[b][0] = 0.0; acc[b][1] = 0.0; acc[b][2] = 0.0; acc[b][3] = 0.0; }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int id = mask[i];
+        const double m = mm[i];
+        const double v0 = (double)S.x[i] * m, v1 = (double)S.y[i] * m, v2 = (double)S.z[i] * m;
+#pragma unroll
+        for (int b = 0; b < MAXB; ++b) {
+            const bool hit = id == b;
+            acc[b][0] += hit ? v0 : 0.0;
+            acc[b][1] += hit ? v1 : 0.0;
+            acc[b][2] += hit ? v2 : 0.0;
+            acc[b][3] += hit ? m : 0.0;
+        }
+    }
+    __shared__ double red[4][4 * MAXB];            // [wavefront][bin * 4 + component], 256 threads
+    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double v = acc[b][c];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane == 0) red[w][4 * b + c] = v;
+        }
+    }
+    __syncthreads();
+    const int W = 4 * nbins;
+    if ((int)threadIdx.x < W) {
+        const int nw = ((int)blockDim.x + 63) >> 6;
+        double t = 0.0;
+        for (int q = 0; q < nw; ++q) t += red[q][threadIdx.x];
+        partial[(size_t)blockIdx.x * W + threadIdx.x] = t;
+    }
+}
+
 // W = 4: sum(m s_x), sum(m s_y), sum(m s_z), sum(m)   (magnetization.cpp:141-147)
 template <typename R>
 __global__ void __launch_bounds__(256)
