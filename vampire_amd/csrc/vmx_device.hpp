@@ -1006,6 +1006,33 @@ k_plain_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> F, Spin3<R> E, S
 // so a system-wide mask (one id) costs one LDS atomic per thread.  Each workgroup writes its
 // histogram to partial[block][nbins*W]; k_stats_finish reduces the partials in block order.
 
+// Fixed-order reduction of the threads' register histograms acc[MAXB][4] (bins < nbins <= MAXB)
+// into partial[block][nbins*4]: wavefront shuffles, then the wavefronts in order.  Workgroups of at
+// most 256 threads; every thread of the workgroup calls it.
+template <int MAXB>
+__device__ __forceinline__ void reduce_hist_regs(const double (&acc)[MAXB][4], int nbins,
+                                                 double* __restrict__ partial) {
+    __shared__ double red[4][4 * MAXB];            // [wavefront][bin * 4 + component]
+    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double v = acc[b][c];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane == 0) red[w][4 * b + c] = v;
+        }
+    }
+    __syncthreads();
+    const int W = 4 * nbins;
+    if ((int)threadIdx.x < W) {
+        const int nw = ((int)blockDim.x + 63) >> 6;
+        double t = 0.0;
+        for (int q = 0; q < nw; ++q) t += red[q][threadIdx.x];
+        partial[(size_t)blockIdx.x * W + threadIdx.x] = t;
+    }
+}
+
 // Masks of at most MAXB bins: each thread keeps every bin's four sums in registers and adds each
 // atom to its bin with predicated adds, so the cost does not depend on how often the id changes
 // along the atoms (a random alloy's material mask changes at nearly every atom).  The workgroup
@@ -1031,25 +1058,7 @@ k_mag_partial_regs(int n, Spin3<R> S, const int* __restrict__ mask, const double
             acc[b][3] += hit ? m : 0.0;
         }
     }
-    __shared__ double red[4][4 * MAXB];            // [wavefront][bin * 4 + component], 256 threads
-    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
-#pragma unroll
-    for (int b = 0; b < MAXB; ++b) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            double v = acc[b][c];
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-            if (lane == 0) red[w][4 * b + c] = v;
-        }
-    }
-    __syncthreads();
-    const int W = 4 * nbins;
-    if ((int)threadIdx.x < W) {
-        const int nw = ((int)blockDim.x + 63) >> 6;
-        double t = 0.0;
-        for (int q = 0; q < nw; ++q) t += red[q][threadIdx.x];
-        partial[(size_t)blockIdx.x * W + threadIdx.x] = t;
-    }
+    reduce_hist_regs<MAXB>(acc, nbins, partial);
 }
 
 // W = 4: sum(m s_x), sum(m s_y), sum(m s_z), sum(m)   (magnetization.cpp:141-147)
@@ -1150,6 +1159,32 @@ template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(256)
 k_energy_partial(SysDev<R> s, Spin3<R> S, const int* __restrict__ mask, const double* __restrict__ mm,
                  const int* __restrict__ atom_rep, int nbins, double* __restrict__ partial) {
+    if (nbins <= 4) {                // few bins: register histograms, as k_mag_partial_regs
+        double acc[4][4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) { acc[b][0] = 0.0; acc[b][1] = 0.0; acc[b][2] = 0.0; acc[b][3] = 0.0; }
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
+            const int id = mask[i];
+            const R sx = S.x[i], sy = S.y[i], sz = S.z[i];
+            const int m = s.type[i];
+            const int rep = atom_rep ? atom_rep[i] : 0;
+            const double w = mm[i];
+            const double e0 = (double)exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, sx, sy, sz) * w;
+            const double e1 = (double)anisotropy_energy(s, m, sx, sy, sz) * w;
+            const double e2 = (double)applied_energy(s.cond[rep * s.nmat + m], sx, sy, sz) * w;
+            const double e3 = s.hd ? (double)magnetostatic_energy(s, i, sx, sy, sz) * w : 0.0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const bool hit = id == b;
+                acc[b][0] += hit ? e0 : 0.0;
+                acc[b][1] += hit ? e1 : 0.0;
+                acc[b][2] += hit ? e2 : 0.0;
+                acc[b][3] += hit ? e3 : 0.0;
+            }
+        }
+        reduce_hist_regs<4>(acc, nbins, partial);
+        return;
+    }
     extern __shared__ double hist[];
     for (int k = threadIdx.x; k < 4 * nbins; k += blockDim.x) hist[k] = 0.0;
     __syncthreads();
