@@ -254,14 +254,18 @@ __global__ void k_dip_cell_mag(int nc, const int* __restrict__ start, const int*
     mag[c] = mx; mag[(size_t)nc + c] = my; mag[2 * (size_t)nc + c] = mz;
 }
 
-// Cell fields, dipole::internal::update_field (src/dipole/update.cpp:52-93): one thread per cell
-// with magnetic atoms, the j loop and every sum in the reference's order.  T holds the tensor as
+// Cell fields, dipole::internal::update_field (src/dipole/update.cpp:52-93): one thread per field
+// component of each cell with magnetic atoms, the j loop and every sum in the reference's order.  T holds the tensor as
 // [xx, xy, xz, yy, yz, zz][source cell j][row lc] so that a wavefront reads consecutive rows.
-__global__ void k_dip_cell_field(int nl, int nc, const int* __restrict__ local_cell,
+__global__ void __launch_bounds__(64) k_dip_cell_field(int nl, int nc, const int* __restrict__ local_cell,
                                  const int* __restrict__ count, const double* __restrict__ volume,
                                  const double* __restrict__ mag, const double* __restrict__ T,
                                  double* __restrict__ B, double* __restrict__ Hd) {
+    // One thread per (local cell, field component r): component r of B and Hd sums tensor row r
+    // only, so the three rows run in separate threads (three times the wavefronts) and every sum
+    // keeps its terms and order.
     const int lc = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = (int)blockIdx.y;
     if (lc >= nl) return;
     const int i = local_cell[lc];
     if (count[i] <= 0) return;
@@ -269,25 +273,47 @@ __global__ void k_dip_cell_field(int nl, int nc, const int* __restrict__ local_c
     const size_t ncs = (size_t)nc, ts = (size_t)nc * nl;
     const double eightPI_three_cell_volume = 8.0 * M_PI / (3.0 * volume[i]);
     const double self_demag = eightPI_three_cell_volume;
-    const double mx_i = mag[i] * imuB, my_i = mag[ncs + i] * imuB, mz_i = mag[2 * ncs + i] * imuB;
-    double bx = self_demag * mx_i * 0.0, by = self_demag * my_i * 0.0, bz = self_demag * mz_i * 0.0;
-    double hx = -0.5 * self_demag * mx_i, hy = -0.5 * self_demag * my_i, hz = -0.5 * self_demag * mz_i;
-    for (int j = 0; j < nc; ++j) {
-        if (count[j] > 0) {
-            const double mx = mag[j] * imuB, my = mag[ncs + j] * imuB, mz = mag[2 * ncs + j] * imuB;
-            const size_t o = (size_t)j * nl + lc;
-            const double txx = T[o], txy = T[ts + o], txz = T[2 * ts + o];
-            const double tyy = T[3 * ts + o], tyz = T[4 * ts + o], tzz = T[5 * ts + o];
-            bx += (mx * txx + my * txy + mz * txz);
-            by += (mx * txy + my * tyy + mz * tyz);
-            bz += (mx * txz + my * tyz + mz * tzz);
-            hx += (mx * txx + my * txy + mz * txz);
-            hy += (mx * txy + my * tyy + mz * tyz);
-            hz += (mx * txz + my * tyz + mz * tzz);
+    const double m_i = mag[r * ncs + i] * imuB;
+    double bf = self_demag * m_i * 0.0;
+    double hf = -0.5 * self_demag * m_i;
+    // row r of [xx xy xz; xy yy yz; xz yz zz] in the [xx, xy, xz, yy, yz, zz] storage
+    const int c0 = r == 0 ? 0 : (r == 1 ? 1 : 2);
+    const int c1 = r == 0 ? 1 : (r == 1 ? 3 : 4);
+    const int c2 = r == 0 ? 2 : (r == 1 ? 4 : 5);
+    const double* __restrict__ T0 = T + c0 * ts;
+    const double* __restrict__ T1 = T + c1 * ts;
+    const double* __restrict__ T2 = T + c2 * ts;
+    // Source cells in order (the reference's summation order), K at a time: the loads of all K
+    // are issued before the first is added, so the loop waits for memory once per K cells
+    // instead of once per cell.  Empty cells are loaded and skipped, as the reference skips them.
+    auto add = [&](int cnt, double gx, double gy, double gz, double t0, double t1, double t2) {
+        if (cnt > 0) {
+            const double mx = gx * imuB, my = gy * imuB, mz = gz * imuB;
+            bf += (mx * t0 + my * t1 + mz * t2);
+            hf += (mx * t0 + my * t1 + mz * t2);
         }
+    };
+    constexpr int K = 16;
+    int j = 0;
+    for (; j + K <= nc; j += K) {
+        int cnt[K];
+        double gx[K], gy[K], gz[K], t0[K], t1[K], t2[K];
+#pragma unroll
+        for (int u = 0; u < K; ++u) {
+            cnt[u] = count[j + u];
+            gx[u] = mag[j + u]; gy[u] = mag[ncs + j + u]; gz[u] = mag[2 * ncs + j + u];
+            const size_t o = (size_t)(j + u) * nl + lc;
+            t0[u] = T0[o]; t1[u] = T1[o]; t2[u] = T2[o];
+        }
+#pragma unroll
+        for (int u = 0; u < K; ++u) add(cnt[u], gx[u], gy[u], gz[u], t0[u], t1[u], t2[u]);
     }
-    B[i] = bx * 9.27400915e-01; B[ncs + i] = by * 9.27400915e-01; B[2 * ncs + i] = bz * 9.27400915e-01;
-    Hd[i] = hx * 9.27400915e-01; Hd[ncs + i] = hy * 9.27400915e-01; Hd[2 * ncs + i] = hz * 9.27400915e-01;
+    for (; j < nc; ++j) {
+        const size_t o = (size_t)j * nl + lc;
+        add(count[j], mag[j], mag[ncs + j], mag[2 * ncs + j], T0[o], T1[o], T2[o]);
+    }
+    B[r * ncs + i] = bf * 9.27400915e-01;
+    Hd[r * ncs + i] = hf * 9.27400915e-01;
 }
 
 // Atom fields from cell fields (src/dipole/field.cpp:113-125): atoms of non-magnetic materials
@@ -406,7 +432,7 @@ struct vmx_ctx {
         hipLaunchKernelGGL((k_dip_cell_mag<R>), dim3((dip.nc + 127) / 128), dim3(128), 0, stream, dip.nc,
                            dip.start.as<int>(), dip.atoms.as<int>(), type.as<uint8_t>(), dip.mus.as<double>(),
                            S.x, S.y, S.z, dip.mag.as<double>());
-        hipLaunchKernelGGL(k_dip_cell_field, dim3((dip.nl + 63) / 64), dim3(64), 0, stream, dip.nl, dip.nc,
+        hipLaunchKernelGGL(k_dip_cell_field, dim3((dip.nl + 63) / 64, 3), dim3(64), 0, stream, dip.nl, dip.nc,
                            dip.local_cell.as<int>(), dip.count.as<int>(), dip.volume.as<double>(),
                            dip.mag.as<double>(), dip.tensor.as<double>(), dip.B.as<double>(), dip.Hd.as<double>());
         hipLaunchKernelGGL((k_dip_unroll<R>), dim3((n + 255) / 256), dim3(256), 0, stream, n, dip.nc,
