@@ -244,7 +244,31 @@ __global__ void k_dip_cell_mag(int nc, const int* __restrict__ start, const int*
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     double mx = 0.0, my = 0.0, mz = 0.0;
-    for (int a = start[c]; a < start[c + 1]; ++a) {
+    // K atoms at a time: their indices, then their types and spins, then the moments are loaded
+    // before any is added, so each batch waits three memory round trips instead of each atom
+    constexpr int K = 16;
+    const int end = start[c + 1];
+    int a = start[c];
+    for (; a + K <= end; a += K) {
+        int i[K], ty[K];
+        double m[K], vx[K], vy[K], vz[K];
+#pragma unroll
+        for (int u = 0; u < K; ++u) i[u] = atoms[a + u];
+#pragma unroll
+        for (int u = 0; u < K; ++u) {
+            ty[u] = type[i[u]];
+            vx[u] = (double)sx[i[u]]; vy[u] = (double)sy[i[u]]; vz[u] = (double)sz[i[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < K; ++u) m[u] = mus[ty[u]];
+#pragma unroll
+        for (int u = 0; u < K; ++u) {
+            mx += vx[u] * m[u];
+            my += vy[u] * m[u];
+            mz += vz[u] * m[u];
+        }
+    }
+    for (; a < end; ++a) {
         const int i = atoms[a];
         const double m = mus[type[i]];
         mx += (double)sx[i] * m;
@@ -255,8 +279,9 @@ __global__ void k_dip_cell_mag(int nc, const int* __restrict__ start, const int*
 }
 
 // Cell fields, dipole::internal::update_field (src/dipole/update.cpp:52-93): one thread per field
-// component of each cell with magnetic atoms, the j loop and every sum in the reference's order.  T holds the tensor as
-// [xx, xy, xz, yy, yz, zz][source cell j][row lc] so that a wavefront reads consecutive rows.
+// component of each cell with magnetic atoms, the j loop and every sum in the reference's order.
+// T holds the tensor as [xx, xy, xz, yy, yz, zz][source cell j][row lc] so that a wavefront reads
+// consecutive rows.
 __global__ void __launch_bounds__(64) k_dip_cell_field(int nl, int nc, const int* __restrict__ local_cell,
                                  const int* __restrict__ count, const double* __restrict__ volume,
                                  const double* __restrict__ mag, const double* __restrict__ T,
