@@ -143,3 +143,25 @@ def _crossing(temps, m, dm, level=0.5):
     d1 = (t2 - t1) * (level - m2) / (m1 - m2) ** 2
     d2 = (t2 - t1) * (m1 - level) / (m1 - m2) ** 2
     return tc, float(np.hypot(d1 * dm[k], d2 * dm[k + 1]))
+
+
+@pytest.mark.parametrize("nbins", [2, 4, 6])
+def test_stats_mask_sizes_match_oracle(gpu, nbins):
+    """Masks whose id changes at every atom, of 2 and 4 bins (per-thread register histograms) and
+    6 bins (workgroup LDS histograms): magnetisation and energy statistics equal the oracle's."""
+    s = S.ferromagnet("fcc", 4)
+    o = O.Oracle(s)
+    o.set_conditions(250.0, (0.0, 0.2, 0.1))
+    mask = (np.arange(s.n) % nbins).astype(np.int32)
+    sp = S.random_spins(s.n, 12)
+    with Context(s, precision=FP64, seed=4) as c:
+        c.set_conditions(250.0, (0.0, 0.2, 0.1))
+        c.set_spins(sp)
+        hm = c.add_magnetization(mask, nbins)
+        he = c.add_energy(mask, nbins)
+        c.stats_update()
+        cur, _, _ = c.magnetization(hm)
+        ecur, _, _ = c.energy(he)
+    np.testing.assert_allclose(cur, o.magnetization(sp, mask, nbins), rtol=1e-12, atol=1e-14)
+    re = o.energy(sp, mask, nbins)
+    np.testing.assert_allclose(ecur, re, rtol=1e-12, atol=1e-12 * np.abs(re).max())
