@@ -454,7 +454,7 @@ struct vmx_ctx {
     template <typename R>
     int dipole_refresh() {
         const Spin3<R> S = spins<R>(s_idx);
-        hipLaunchKernelGGL((k_dip_cell_mag<R>), dim3((dip.nc + 127) / 128), dim3(128), 0, stream, dip.nc,
+        hipLaunchKernelGGL((k_dip_cell_mag<R>), dim3((dip.nc + 63) / 64), dim3(64), 0, stream, dip.nc,
                            dip.start.as<int>(), dip.atoms.as<int>(), type.as<uint8_t>(), dip.mus.as<double>(),
                            S.x, S.y, S.z, dip.mag.as<double>());
         hipLaunchKernelGGL(k_dip_cell_field, dim3((dip.nl + 63) / 64, 3), dim3(64), 0, stream, dip.nl, dip.nc,
