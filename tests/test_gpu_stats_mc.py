@@ -145,10 +145,11 @@ def _crossing(temps, m, dm, level=0.5):
     return tc, float(np.hypot(d1 * dm[k], d2 * dm[k + 1]))
 
 
-@pytest.mark.parametrize("nbins", [2, 4, 6])
+@pytest.mark.parametrize("nbins", [2, 4, 6, 10])
 def test_stats_mask_sizes_match_oracle(gpu, nbins):
-    """Masks whose id changes at every atom, of 2 and 4 bins (per-thread register histograms) and
-    6 bins (workgroup LDS histograms): magnetisation and energy statistics equal the oracle's."""
+    """Masks whose id changes at every atom: per-thread register histograms (magnetisation up to
+    8 bins, energy up to 4) and workgroup LDS histograms above; magnetisation and energy
+    statistics equal the oracle's."""
     s = S.ferromagnet("fcc", 4)
     o = O.Oracle(s)
     o.set_conditions(250.0, (0.0, 0.2, 0.1))
