@@ -133,6 +133,12 @@ typedef struct vmx_options {
                              (thermal + applied) field from the same Philox counter; 1 = the
                              predictor keeps it for the corrector, as the reference keeps its
                              field arrays (6 more reals of traffic per atom).  Same results.   */
+    int32_t fp64_normals; /* VMX_FP32 only, a measurement mode: 1 = each step's thermal normals
+                             are drawn by the FP64 generator (the normals of a VMX_FP64 context
+                             with the same seed) and rounded to float, one more kernel per step,
+                             so FP32 and FP64 runs differ by their arithmetic alone.  0 = the
+                             FP32 generator inside the step kernels (the fast path).  Graph
+                             replay refuses the mode.                                        */
 } vmx_options;
 
 /* Atom storage order on the device.  Invisible to the caller: every array passed in or out is in

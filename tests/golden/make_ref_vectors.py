@@ -285,6 +285,25 @@ def main(only=()):
     if want("film_dmi"):
         case("film_dmi", "sc", 5, 2.5, [m1, m2], steps=100, snaps=[1, 100], T=300.0, H=0.0,
              pbc=(True, True, False), dmi_cutoff=3.6, Lz=4)
+    # vectorial exchange: a three-value exchange-matrix selects exchange::vectorial
+    # (src/exchange/interface.cpp:86-95, initialize.cpp:63-64), diagonal Jxx, Jyy, Jzz per edge
+    # (exchange_fields.cpp:85-116).  One material (every edge the same J) ...
+    cov = dict(CO, **{"exchange-matrix[1]": "6.064e-21,5.2e-21,7.3e-21"})
+    if want("sc6_vectorial"):
+        case("sc6_vectorial", "sc", 6, 3.54, [cov], steps=200, snaps=[1, 10, 200], T=300.0, H=0.5,
+             Hdir=(1, 1, 1))
+    # ... and two materials in layers with antiparallel inter-layer coupling (per-edge J ids)
+    va = {"material-name": "A", "damping-constant": 0.1, "exchange-matrix[1]": "5e-21,4.1e-21,6.2e-21",
+          "exchange-matrix[2]": "-1.2e-21,-0.9e-21,-1.6e-21", "atomic-spin-moment": "1.6 !muB",
+          "second-order-uniaxial-anisotropy-constant": 2.0e-24, "uniaxial-anisotropy-direction": "0,0,1",
+          "minimum-height": 0.0, "maximum-height": 0.5, "initial-spin-direction": "random"}
+    vb = {"material-name": "B", "damping-constant": 0.05, "exchange-matrix[1]": "-1.2e-21,-0.9e-21,-1.6e-21",
+          "exchange-matrix[2]": "5.5e-21,6.1e-21,4.4e-21", "atomic-spin-moment": "2.4 !muB",
+          "second-order-uniaxial-anisotropy-constant": 0.0,
+          "minimum-height": 0.5, "maximum-height": 1.0, "initial-spin-direction": "random"}
+    if want("film_vectorial"):
+        case("film_vectorial", "sc", 6, 2.5, [va, vb], steps=100, snaps=[1, 100], T=200.0, H=1.0,
+             Hdir=(0, 1, 1), pbc=(True, True, False), Lz=4)
     # the LLG midpoint integrator (sim::LLG_Midpoint) on the SC uniaxial and two-material DMI decks
     if want("sc6_uniaxial_mid"):
         case("sc6_uniaxial_mid", "sc", 6, 3.54, [CO], steps=100, snaps=[1, 10, 100], T=300.0, H=0.5,

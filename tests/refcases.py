@@ -23,6 +23,13 @@ CASES = {
                                           kc_vector2=(-1.0, 1.0, 0.0))], ani=dict(cubic_rotated=True)),
     "film_dmi": dict(mats=[S.Material(mu_s_muB=1.6, alpha=0.1), S.Material(mu_s_muB=7.6, alpha=0.1)], ani={}),
     "c0_sc20": dict(mats=[S.Material(**CO)], ani=dict(uniaxial=True)),
+    # vectorial exchange (three-value exchange-matrix): one material, then two layered materials
+    "sc6_vectorial": dict(mats=[S.Material(**CO)], ani=dict(uniaxial=True),
+                          J=[[[6.064e-21, 5.2e-21, 7.3e-21]]]),
+    "film_vectorial": dict(mats=[S.Material(mu_s_muB=1.6, alpha=0.1, Ku=2.0e-24, ku_vector=(0.0, 0.0, 1.0)),
+                                 S.Material(mu_s_muB=2.4, alpha=0.05)], ani=dict(uniaxial=True),
+                           J=[[[5e-21, 4.1e-21, 6.2e-21], [-1.2e-21, -0.9e-21, -1.6e-21]],
+                              [[-1.2e-21, -0.9e-21, -1.6e-21], [5.5e-21, 6.1e-21, 4.4e-21]]]),
     "sc6_uniaxial_mid": dict(mats=[S.Material(**CO)], ani=dict(uniaxial=True)),
     "film_dmi_mid": dict(mats=[S.Material(mu_s_muB=1.6, alpha=0.1), S.Material(mu_s_muB=7.6, alpha=0.1)], ani={}),
     "dipole_sc10": dict(mats=[S.Material(**CO), S.Material(**dict(CO, mu_s_muB=2.5))], ani=dict(uniaxial=True)),
@@ -61,3 +68,34 @@ def dipole_args(name: str, spins_key: str = "spins_in"):
     f = load(name)
     return dict(spins=f[spins_key], num_cells=int(f["num_cells"]), local_cell=f["local_cell"],
                 atom_cell=f["atom_cell"], volume=f["volume"], tensor=f["tensor"], update_rate=int(f["update_rate"]))
+
+
+def block_mean(samples, nblocks: int = 10):
+    """Mean over axis 0 and its error from `nblocks` block means (autocorrelation-aware)."""
+    bm = np.array([b.mean(axis=0) for b in np.array_split(np.asarray(samples), nblocks)])
+    return bm.mean(axis=0), bm.std(axis=0, ddof=1) / np.sqrt(len(bm))
+
+
+def crossing(temps, m, dm, level=0.5):
+    """Temperature where the linear interpolation of m(T) crosses `level`, and its error from the
+    errors of the two bracketing points."""
+    k = int(np.nonzero((m[:-1] >= level) & (m[1:] < level))[0][0])
+    t1, t2, m1, m2 = temps[k], temps[k + 1], m[k], m[k + 1]
+    tc = t1 + (m1 - level) * (t2 - t1) / (m1 - m2)
+    d1 = (t2 - t1) * (level - m2) / (m1 - m2) ** 2
+    d2 = (t2 - t1) * (m1 - level) / (m1 - m2) ** 2
+    return tc, float(np.hypot(d1 * dm[k], d2 * dm[k + 1]))
+
+
+def assert_mc_curve(temps, mean, err, name: str = "mc_sc8"):
+    """<|m|>(T) against the reference's montecarlo::mc_step curve within 4 combined sigma + 0.01
+    (finite-run bias), and the <|m|> = 1/2 crossing within 4 combined sigma + 5 K."""
+    rows = load(name)["M_T"]
+    assert np.array_equal(np.asarray(temps, dtype=float), rows[:, 0])
+    for r in range(len(temps)):
+        tol = 4.0 * np.hypot(err[r], rows[r, 2]) + 0.01
+        assert abs(mean[r] - rows[r, 1]) <= tol, (temps[r], mean[r], rows[r, 1], tol)
+    tc_ref, dtc_ref = crossing(rows[:, 0], rows[:, 1], rows[:, 2])
+    tc, dtc = crossing(rows[:, 0], np.asarray(mean), np.asarray(err))
+    assert 500.0 < tc_ref < 700.0
+    assert abs(tc - tc_ref) <= 4.0 * np.hypot(dtc_ref, dtc) + 5.0, (tc, tc_ref, dtc, dtc_ref)

@@ -74,6 +74,51 @@ def test_c4_full_size_equals_oracle(gpu):
 
 
 @pytest.mark.timeout(300)
+def test_c4_full_size_fp32_pulse_equals_oracle(gpu):
+    """The film workload at its own precision: FP32 spins and fields, DMI tensor exchange, 10 %
+    dilution (~472k atoms), random spins, with the temperature set at every step along the
+    rising flank of a Gaussian 100 fs pulse from 300 K toward 1200 K (the per-step temperature of
+    temperature_pulse.cpp:288-298).  Fed the oracle's normals, 20 steps agree with the FP64 oracle
+    to 1e-5 per spin component (FP32 rounding, ~2e-7 per step), |S| = 1 to 1e-6."""
+    s = S.config_c4()
+    spins = S.random_spins(s.n, 4)
+    temps = [300.0 + 900.0 * np.exp(-((k * 1e-16 - 70e-15) / 100e-15) ** 2) for k in range(20)]
+    assert len(set(temps)) == len(temps)
+    o = O.Oracle(s)
+    ref = spins
+    with Context(s, precision=FP32, seed=3) as c:
+        c.set_spins(spins)
+        for k, T in enumerate(temps):
+            noise = O.thermal_noise(3, s.n, k)
+            o.set_conditions(T)
+            ref = o.llg_heun(ref, 1, noise=noise[None], threads=THREADS)
+            c.set_conditions(T)
+            c.llg_heun(1, noise=noise[None])
+        dev = c.get_spins()
+        assert c.topology[0] == "ell"
+    err = np.abs(dev - ref).max()
+    assert err <= 1e-5, err
+    assert np.abs(np.linalg.norm(dev, axis=0) - 1.0).max() <= 1e-6
+
+
+@pytest.mark.timeout(300)
+def test_c1_full_size_fp32_same_normals_equals_fp64(gpu, c1):
+    """FCC 100^3 at 600 K from +z: FP32 with the FP64 generator's normals (fp64_normals) against
+    the FP64 step over 20 steps: within 2e-5 per spin component, M/Ms within 1e-6."""
+    spins = np.zeros((3, c1.n))
+    spins[2] = 1.0
+    out = {}
+    for key, prec in (("fp64", FP64), ("fp32", FP32)):
+        with Context(c1, precision=prec, seed=7, fp64_normals=True) as c:
+            c.set_conditions(600.0)
+            c.set_spins(spins)
+            c.llg_heun(20)
+            out[key] = c.get_spins()
+    assert np.abs(out["fp32"] - out["fp64"]).max() <= 2e-5
+    assert np.abs(out["fp32"].mean(axis=1) - out["fp64"].mean(axis=1)).max() <= 1e-6
+
+
+@pytest.mark.timeout(300)
 def test_c2_full_batch_replicas_equal_standalone(gpu):
     """The 256-replica batch (SC, L in [8, 48], 8.67M atoms, T in [0, 1.5 Tc], alpha = 1, FP32):
     after 20 batched steps, the smallest, the largest and a middle replica equal the same replica

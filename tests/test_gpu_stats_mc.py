@@ -12,7 +12,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("topology", ["csr", "ell", "stencil"])
-@pytest.mark.parametrize("name", ["sc6_uniaxial", "fcc3_cubic", "bcc4_rotated", "film_dmi"])
+@pytest.mark.parametrize("name", ["sc6_uniaxial", "fcc3_cubic", "bcc4_rotated", "film_dmi", "sc6_vectorial",
+                                  "film_vectorial"])
 def test_stats_match_reference(gpu, name, topology):
     """System and material magnetisation and system energies after the reference's steps
     (stats_after = system M [8], material M [4(nmat+1)], system E [5x2])."""
@@ -94,11 +95,13 @@ def test_mc_checkerboard_matches_oracle(gpu, topology):
     assert abs(dsig[0] - sigma) <= 1e-12 * sigma
 
 
-def test_mc_curve_matches_reference(gpu):
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_mc_curve_matches_reference(gpu, precision):
     """Statistical parity: <|m|>(T) of checkerboard Metropolis on SC 8^3 against the reference
     random-site montecarlo::mc_step (its own MT19937 stream) within combined error bars
     (4 sigma + 0.01 for finite-run bias), temperatures 100-900 K, and the pseudo-critical
-    temperature (the <|m|> = 1/2 crossing) within 4 combined sigma + 5 K."""
+    temperature (the <|m|> = 1/2 crossing) within 4 combined sigma + 5 K; in FP64 and in FP32
+    (the precision of the replica-batching workload)."""
     f = RC.load("mc_sc8")
     rows = f["M_T"]
     temps = rows[:, 0]
@@ -106,7 +109,7 @@ def test_mc_curve_matches_reference(gpu):
     batch = S.concat_systems(systems)
     n = systems[0].n
     sweeps, equil, every = 6000, 1000, 5
-    with Context(batch, precision=FP64, seed=2024) as c:
+    with Context(batch, precision=precision, seed=2024) as c:
         for r, T in enumerate(temps):
             c.set_conditions(float(T), replica=r)
         sp = np.zeros((3, batch.n))
@@ -118,31 +121,8 @@ def test_mc_curve_matches_reference(gpu):
             c.mc_sweep(every)
             x = c.get_spins().reshape(3, len(temps), n).sum(axis=2)
             samples.append(np.sqrt((x * x).sum(axis=0)) / n)
-    samples = np.array(samples)
-    blocks = np.array_split(samples, 10)
-    bm = np.array([b.mean(axis=0) for b in blocks])
-    mean = bm.mean(axis=0)
-    err = bm.std(axis=0, ddof=1) / np.sqrt(len(bm))
-    for r in range(len(temps)):
-        tol = 4.0 * np.hypot(err[r], rows[r, 2]) + 0.01
-        assert abs(mean[r] - rows[r, 1]) <= tol, (temps[r], mean[r], rows[r, 1], tol)
-    # pseudo-critical temperature of the L = 8 system: where <|m|> crosses 1/2 (the reference's
-    # curve gives ~581 K; the infinite-lattice Tc of these parameters is ~634 K)
-    tc_ref, dtc_ref = _crossing(temps, rows[:, 1], rows[:, 2])
-    tc_dev, dtc_dev = _crossing(temps, mean, err)
-    assert 500.0 < tc_ref < 700.0
-    assert abs(tc_dev - tc_ref) <= 4.0 * np.hypot(dtc_ref, dtc_dev) + 5.0, (tc_dev, tc_ref, dtc_dev, dtc_ref)
-
-
-def _crossing(temps, m, dm, level=0.5):
-    """Temperature where the linear interpolation of m(T) crosses `level`, and its error from the
-    errors of the two bracketing points."""
-    k = int(np.nonzero((m[:-1] >= level) & (m[1:] < level))[0][0])
-    t1, t2, m1, m2 = temps[k], temps[k + 1], m[k], m[k + 1]
-    tc = t1 + (m1 - level) * (t2 - t1) / (m1 - m2)
-    d1 = (t2 - t1) * (level - m2) / (m1 - m2) ** 2
-    d2 = (t2 - t1) * (m1 - level) / (m1 - m2) ** 2
-    return tc, float(np.hypot(d1 * dm[k], d2 * dm[k + 1]))
+    mean, err = RC.block_mean(samples)
+    RC.assert_mc_curve(temps, mean, err)
 
 
 @pytest.mark.parametrize("nbins", [2, 4, 6, 10])

@@ -11,7 +11,7 @@ import refcases as RC
 from oracle import oracle as O
 from vampire_amd import system as S
 
-GOLDEN_CASES = ["sc6_uniaxial", "fcc3_cubic", "bcc4_rotated", "film_dmi"]
+GOLDEN_CASES = ["sc6_uniaxial", "fcc3_cubic", "bcc4_rotated", "film_dmi", "sc6_vectorial", "film_vectorial"]
 
 
 # ---------------------------------------------------------------- Philox known answers
@@ -247,6 +247,54 @@ def test_dmi_builder_equals_reference():
     np.testing.assert_array_equal(table[iid], f["exchange"][f["nbr_iid"]])
 
 
+@pytest.mark.parametrize("name,dims,pbc", [("sc6_vectorial", (6, 6, 6), (True, True, True)),
+                                            ("film_vectorial", (6, 6, 4), (True, True, False))])
+def test_vectorial_builder_equals_reference(name, dims, pbc):
+    """Three-value exchange-matrix decks: the builder's vectorial tables (Jxx, Jyy, Jzz per
+    material pair over mu_s, unroll_normalised.cpp:66-90) give every edge the reference's J."""
+    f = RC.load(name)
+    lat = S.build_lattice("sc", *dims, a=float(f["a"]), pbc=pbc)
+    assert np.array_equal(lat.nbr, f["nbr"])
+    spec = RC.CASES[name]
+    et, table, iid = S.exchange_tables(lat, f["type"], spec["mats"], np.array(spec["J"]), S.VECTORIAL)
+    assert et == S.VECTORIAL == int(f["exchange_type"])
+    np.testing.assert_array_equal(table[iid], f["exchange"][f["nbr_iid"]])
+
+
+def test_vectorial_with_equal_components_is_isotropic():
+    """Jxx = Jyy = Jzz reduces the vectorial branch (exchange_fields.cpp:85-116) to the isotropic
+    one (:52-78) exactly, for fields and trajectories."""
+    base = S.ferromagnet("fcc", 3)
+    vec = S.System(**{**base.__dict__, "exchange_type": S.VECTORIAL,
+                      "exchange": np.repeat(base.exchange, 3, axis=1)})
+    spins = S.random_spins(base.n, 4)
+    out = []
+    for s in (base, vec):
+        o = O.Oracle(s)
+        o.set_conditions(300.0, (0.1, 0.0, 0.3))
+        out.append((o.spin_fields(spins), o.llg_heun(spins, 5, seed=2)))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_vectorial_field_brute_force():
+    """Per-component field of an anisotropic ferromagnet by an explicit pair sum."""
+    s0 = S.ferromagnet("sc", 3)
+    J = np.array([1.5, -0.7, 2.25])
+    vec = S.System(**{**s0.__dict__, "exchange_type": S.VECTORIAL,
+                      "exchange": np.tile(J, (s0.exchange.shape[0], 1))})
+    spins = S.random_spins(s0.n, 8)
+    o = O.Oracle(vec)
+    o.set_conditions(0.0)
+    h = o.spin_fields(spins)
+    ref = np.zeros_like(spins)
+    for i in range(s0.n):
+        for e in range(s0.nbr_start[i], s0.nbr_end[i] + 1):
+            ref[:, i] += J * spins[:, s0.nbr[e]]
+    ku = s0.materials[0]["ku2"] * spins[2] * 2.0   # uniaxial along z of the Co material
+    ref[2] += ku
+    np.testing.assert_allclose(h, ref, rtol=1e-13, atol=1e-12)
+
+
 # ---------------------------------------------------------------- closed forms and invariants
 def _isolated(n, mat, dt_SI=1e-16):
     dt = dt_SI * S.GAMMA_SI
@@ -407,6 +455,53 @@ def test_metropolis_single_spin_boltzmann():
         th.append(np.arccos(np.clip(sp[2], -1, 1)))
     th = np.concatenate(th)
     hist, edges = np.histogram(th, bins=30, range=(0, np.pi), density=True)
+    c = 0.5 * (edges[1:] + edges[:-1])
+    p = np.sin(c) * np.exp(-Ku * np.sin(c) ** 2 / (1.3806503e-23 * T))
+    p /= np.trapezoid(p, c)
+    assert np.abs(hist - p).mean() < 0.03
+
+
+def test_random_site_mc_curve_matches_reference():
+    """or_mc_random_site restates montecarlo::mc_step (N trials at random sites per sweep, the
+    adaptive cone): its <|m|>(T) on SC 8^3 from +z (1000 sweeps of equilibration, then samples
+    every 5 sweeps to 6000, as the reference run) agrees with the reference's own mc_step curve
+    (its MT19937 stream, tests/golden/ref_mc_sc8.npz) within error bars, and so does the
+    <|m|> = 1/2 crossing.  A wrong acceptance sign, energy scale or temperature rescaling moves
+    the curve by far more than the bars."""
+    rows = RC.load("mc_sc8")["M_T"]
+    s = S.ferromagnet("sc", 8, Ku=0.0)
+    sweeps, equil, every = 6000, 1000, 5
+    means, errs = [], []
+    for T in rows[:, 0]:
+        o = O.Oracle(s)
+        o.set_conditions(float(T))
+        sp = np.zeros((3, s.n))
+        sp[2] = 1.0
+        sp, sigma, _, _ = o.mc_random_site(sp, equil, seed=31)
+        samples = []
+        for w in range((sweeps - equil) // every):
+            sp, sigma, _, _ = o.mc_random_site(sp, every, seed=31, sweep0=equil + every * w, sigma=sigma)
+            samples.append(np.linalg.norm(sp.sum(axis=1)) / s.n)
+        m, e = RC.block_mean(samples)
+        means.append(m)
+        errs.append(e)
+    RC.assert_mc_curve(rows[:, 0], np.array(means), np.array(errs))
+
+
+def test_random_site_single_spin_boltzmann():
+    """Isolated spins with uniaxial K under the random-site sweep sample the Boltzmann density of
+    the reference's Thermal test, as the checkerboard sweep does."""
+    Ku, T = 1e-23, 1.0
+    mat = S.Material(mu_s_muB=1.72, alpha=1.0, Ku=Ku)
+    s = _isolated(4000, mat)
+    o = O.Oracle(s)
+    o.set_conditions(T, thermal=False, applied=False)
+    sp, sigma, _, _ = o.mc_random_site(S.random_spins(s.n, 6), 60, seed=4)
+    th = []
+    for w in range(40):
+        sp, sigma, _, _ = o.mc_random_site(sp, 5, seed=4, sweep0=100 + 5 * w, sigma=sigma)
+        th.append(np.arccos(np.clip(sp[2], -1, 1)))
+    hist, edges = np.histogram(np.concatenate(th), bins=30, range=(0, np.pi), density=True)
     c = 0.5 * (edges[1:] + edges[:-1])
     p = np.sin(c) * np.exp(-Ku * np.sin(c) ** 2 / (1.3806503e-23 * T))
     p /= np.trapezoid(p, c)

@@ -92,6 +92,7 @@ class vmx_options(C.Structure):
         ("recompute_euler", C.c_int32),
         ("atom_order", C.c_int32),
         ("keep_external_field", C.c_int32),
+        ("fp64_normals", C.c_int32),
     ]
 
 

@@ -41,11 +41,15 @@ def material_struct(d: dict) -> L.vmx_material:
 
 
 class Context:
-    """A device context for one system (or a batch of replicas)."""
+    """A device context for one system (or a batch of replicas).
+
+    fp64_normals (FP32 only) draws each step's thermal normals with the FP64 generator, so an
+    FP32 run and an FP64 run of the same seed consume the same normals; a measurement mode, one
+    more kernel per step."""
 
     def __init__(self, system: System, precision: str = FP64, kernel: str = FUSED, block_size: int = 0,
                  seed: int = 0, device: int = -1, topology: str = "auto", recompute_euler: bool = False,
-                 atom_order: str = "auto", keep_external_field: bool = False):
+                 atom_order: str = "auto", keep_external_field: bool = False, fp64_normals: bool = False):
         self.lib = L.load()
         self.system = system
         self.n = system.n
@@ -71,7 +75,8 @@ class Context:
                             topology=TOPOLOGIES[topology],
                             recompute_euler=int(bool(recompute_euler)),
                             atom_order={"auto": 0, "caller": 1}[atom_order],
-                            keep_external_field=int(bool(keep_external_field)))
+                            keep_external_field=int(bool(keep_external_field)),
+                            fp64_normals=int(bool(fp64_normals)))
         h = C.c_void_p()
         L.check(self.lib.vmx_create(C.byref(sysc), C.byref(opt), C.byref(h)))
         self.h = h

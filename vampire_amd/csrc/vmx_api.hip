@@ -419,6 +419,10 @@ struct vmx_ctx {
     DevBuf mm, atom_rep;
     DevBuf staging;               // 3N doubles for host<->device spin copies and noise
     DevBuf noise_dev;             // 3N doubles
+    // FP32 contexts: draw the thermal normals with the FP64 kernels' Box-Muller into noise_dev
+    // (one more launch per step) and feed them to the FP32 step as injected noise, so that FP32
+    // and FP64 runs consume the same normals (to float rounding)
+    int fp64_normals = 0;
     std::vector<double> noise_host;   // one step of caller noise in internal order
     // a per-atom int array in internal order
     std::vector<int> internal_ints(const int32_t* a) const {
@@ -680,6 +684,17 @@ struct vmx_ctx {
         return precision == VMX_FP64 ? upload_conditions<double>() : upload_conditions<float>();
     }
 
+    // true when an FP32 step without caller noise takes its normals from the FP64 generator
+    template <typename R>
+    bool normals_from_fp64(const double* noise) const {
+        return fp64_normals && !noise && sizeof(R) == sizeof(float) && (fld & FLD_THERMAL);
+    }
+    // the FP64 kernels' normals of Philox step `st` into noise_dev ([3][n], internal order)
+    int draw_fp64_normals(uint64_t st) {
+        VMX_HIP(Ops<double>::thermal_noise(launch(), sys<double>(), st, noise_dev.as<double>()));
+        return VMX_OK;
+    }
+
     template <typename R>
     int heun_steps(int nsteps, const double* noise, double* kernel_ms = nullptr);
     template <typename R>
@@ -715,7 +730,8 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
     const SysDev<R> s = sys<R>(), sf = sys_fused<R>();
     const Launch L = launch(), Lf = launch_fused();
     const double* nd = nullptr;
-    if (noise) {
+    const bool normals64 = normals_from_fp64<R>(noise);
+    if (noise || normals64) {
         VMX_TRY(noise_dev.alloc(3 * (size_t)n * sizeof(double)));
         nd = noise_dev.as<double>();
     }
@@ -739,6 +755,8 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
             to_internal(noise + (size_t)k * 3 * n, noise_host.data(), 3);
             VMX_HIP(hipStreamSynchronize(stream));
             VMX_HIP(hipMemcpy(noise_dev.p, noise_host.data(), 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+        } else if (normals64) {
+            VMX_TRY(draw_fp64_normals(step + (uint64_t)k));
         }
         const int p_idx = 3 - s_idx - o_idx;
         int nk = 0;
@@ -799,7 +817,8 @@ int vmx_ctx::midpoint_steps(int nsteps, const double* noise) {
     const SysDev<R> s = sys<R>(), sf = sys_fused<R>();
     const Launch L = launch(), Lf = launch_fused();
     const double* nd = nullptr;
-    if (noise) {
+    const bool normals64 = normals_from_fp64<R>(noise);
+    if (noise || normals64) {
         VMX_TRY(noise_dev.alloc(3 * (size_t)n * sizeof(double)));
         nd = noise_dev.as<double>();
     }
@@ -810,6 +829,8 @@ int vmx_ctx::midpoint_steps(int nsteps, const double* noise) {
             to_internal(noise + (size_t)k * 3 * n, noise_host.data(), 3);
             VMX_HIP(hipStreamSynchronize(stream));
             VMX_HIP(hipMemcpy(noise_dev.p, noise_host.data(), 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+        } else if (normals64) {
+            VMX_TRY(draw_fp64_normals(step + (uint64_t)k));
         }
         const int p_idx = 3 - s_idx - o_idx;
         VMX_HIP(Ops<R>::mid_predictor(Lf, et, am(), sf, spins<R>(s_idx), spins<R>(p_idx), field<R>(), nd, (uint32_t)k));
@@ -1537,6 +1558,7 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
     }
     VMX_TRY(c->staging.alloc(3 * (size_t)n * sizeof(double)));
     c->store_euler = opt->recompute_euler ? 0 : 1;
+    c->fp64_normals = (opt->fp64_normals && c->precision == VMX_FP32) ? 1 : 0;
     c->store_field = opt->keep_external_field ? 1 : 0;
     if (c->kernel == VMX_KERNEL_FUSED && c->store_euler && c->precision == VMX_FP64)
         VMX_TRY(c->euler_buf.alloc(3 * (size_t)n * rs));
@@ -1626,6 +1648,8 @@ int vmx_get_spins(vmx_ctx* c, double* sx, double* sy, double* sz) {
 int vmx_set_time_step(vmx_ctx* c, double dt, double half_dt) {
     if (!c) return fail(VMX_ERR_ARG, "null context");
     if (!(dt > 0.0) || !std::isfinite(dt)) return fail(VMX_ERR_ARG, "dt must be positive and finite");
+    // a recorded graph holds the step's kernel arguments, dt among them
+    if (dt != c->dt || half_dt != c->half_dt) c->drop_graph();
     c->dt = dt;
     c->half_dt = half_dt;
     return VMX_OK;
@@ -1642,6 +1666,9 @@ int vmx_set_conditions(vmx_ctx* c, int32_t replica, double temperature, const do
         c->T_h[r] = temperature;
         for (int k = 0; k < 3; ++k) c->H_h[3 * r + k] = H[k];
     }
+    // temperatures and fields live in the device condition table that a graph reads at replay;
+    // the field flags are kernel arguments recorded in the graph
+    if ((field_flags & 0x7u) != c->fld) c->drop_graph();
     c->fld = field_flags & 0x7u;
     return c->upload_conditions_any();
 }
@@ -1671,6 +1698,9 @@ int vmx_llg_heun_graph(vmx_ctx* c, int32_t nsteps, int32_t nrepeat) {
     if (c->kernel != VMX_KERNEL_FUSED) return fail(VMX_ERR_STATE, "graphs are recorded from the fused kernels");
     if (c->dip.on)
         return fail(VMX_ERR_STATE, "graph replay does not refresh the dipole field; use vmx_llg_heun");
+    if (c->fp64_normals && (c->fld & FLD_THERMAL))
+        return fail(VMX_ERR_STATE, "graph replay draws the normals in the step kernels; fp64_normals needs "
+                                   "vmx_llg_heun");
     if (!(c->dt > 0.0)) return fail(VMX_ERR_STATE, "time step not set (vmx_set_time_step)");
     VMX_TRY(c->use_device());
     if (!c->graph_exec || c->graph_key != c->graph_key_of(nsteps)) {
@@ -1783,6 +1813,7 @@ int vmx_checkpoint_load(vmx_ctx* c, const void* buf, uint64_t bytes) {
     VMX_TRY(c->use_device());
     get(c->T_h.data(), 8 * (size_t)c->nrep);
     get(c->H_h.data(), 24 * (size_t)c->nrep);
+    if (((uint32_t)fld & 0x7u) != c->fld) c->drop_graph();
     c->fld = (uint32_t)fld & 0x7u;
     VMX_TRY(c->upload_conditions_any());
     std::vector<double> tmp(3 * (size_t)c->n);

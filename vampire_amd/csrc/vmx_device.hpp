@@ -159,12 +159,14 @@ __device__ __forceinline__ void box_muller3(const U4& u, double& g0, double& g1,
 }
 
 // FP32 variant on the hardware transcendental units: v_log_f32 is log2, v_sin/v_cos_f32 take
-// their argument in revolutions, so sin(2 pi u) is one instruction.
+// their argument in revolutions, so sin(2 pi u) is one instruction.  The uniforms are the FP64
+// ones, (k + 0.5) 2^-32, rounded to float (relative error 2^-24 at every k): small words keep
+// their full resolution, so the tail of the radius sqrt(-2 ln u1) follows the FP64 normals
+// instead of jumping in steps of 2^-24 in u1.
 __device__ __forceinline__ void box_muller3(const U4& u, float& g0, float& g1, float& g2) {
-    const float u1 = ((float)(u.v[0] >> 8) + 0.5f) * 5.9604644775390625e-08f;  // 24-bit
-    const float u2 = ((float)(u.v[1] >> 8) + 0.5f) * 5.9604644775390625e-08f;
-    const float u3 = ((float)(u.v[2] >> 8) + 0.5f) * 5.9604644775390625e-08f;
-    const float u4 = ((float)(u.v[3] >> 8) + 0.5f) * 5.9604644775390625e-08f;
+    const float s = 2.3283064365386963e-10f;  // 2^-32
+    const float u1 = ((float)u.v[0] + 0.5f) * s, u2 = ((float)u.v[1] + 0.5f) * s;
+    const float u3 = ((float)u.v[2] + 0.5f) * s, u4 = ((float)u.v[3] + 0.5f) * s;
     const float ln2x2 = 1.3862943611198906f;  // 2 ln 2
     const float r1 = __builtin_amdgcn_sqrtf(-ln2x2 * __builtin_amdgcn_logf(u1));
     const float r2 = __builtin_amdgcn_sqrtf(-ln2x2 * __builtin_amdgcn_logf(u3));

@@ -269,7 +269,10 @@ def exchange_tables(lat: Lattice, types: np.ndarray, mats: List[Material], J_SI:
     of dmi.cpp:46-221 added when dmi_SI is given."""
     nm = len(mats)
     mu_s = np.array([m.mu_s_SI for m in mats])
-    Jm = np.asarray(J_SI, dtype=np.float64) / mu_s[:, None]
+    J_SI = np.asarray(J_SI, dtype=np.float64)
+    # J_SI[mi][mj] or, for vectorial exchange, J_SI[mi][mj][k] (Jxx, Jyy, Jzz: the three values of
+    # material[i]:exchange-matrix[j], src/exchange/interface.cpp:86-95)
+    Jm = J_SI / mu_s.reshape((nm,) + (1,) * (J_SI.ndim - 1))
     row_atom = np.repeat(np.arange(lat.n), lat.nbr_end - lat.nbr_start + 1)
     mi = types[row_atom]
     mj = types[lat.nbr]
@@ -280,11 +283,14 @@ def exchange_tables(lat: Lattice, types: np.ndarray, mats: List[Material], J_SI:
         iid = (mi * nm + mj).astype(np.int32)
         return exchange_type, table, iid
     if exchange_type == VECTORIAL:
-        table = np.repeat(Jm.reshape(nm * nm, 1), 3, axis=1)
+        # unroll_normalised.cpp:66-90: template J (1 for a nearest neighbour) times Jij_matrix[k]
+        table = Jm.reshape(nm * nm, 3).copy() if Jm.ndim == 3 else np.repeat(Jm.reshape(nm * nm, 1), 3, axis=1)
         iid = (mi * nm + mj).astype(np.int32)
         return exchange_type, table, iid
     nnz = lat.nbr.shape[0]
     ten = np.zeros((nnz, 9))
+    if Jm.ndim == 3:
+        raise ValueError("tensorial exchange takes one J per material pair")
     jv = Jm[mi, mj]
     ten[:, 0] = 1.0 * jv
     ten[:, 4] = 1.0 * jv
