@@ -223,6 +223,75 @@ def configs_leg(c0_steps: int = 10000, c3_steps: int = 2000, c4_steps: int = 100
     return out
 
 
+def drift_leg(system, steps: int, c0_steps: int = 1000):
+    """FP32 against FP64 after `steps` Heun steps of the benchmark system from the same state (600 K,
+    spins +z, seed 7): with the same normals (the FP32 context draws the FP64 generator's normals,
+    fp64_normals) the difference is the FP32 arithmetic alone; with its own FP32 normals it also
+    holds the generators' difference.  max_abs_dm is the largest per-spin-component difference,
+    dM_over_Ms = |<m>_FP32 - <m>_FP64|.  The trajectories are chaotic, so per-spin differences
+    grow from any rounding: c0_floor measures, on the SC 20^3 workload at 300 K over c0_steps, the
+    same-normals FP32 difference beside what FP64 arithmetic reaches when its spins are rounded to
+    float after every step, and what one 2^-24 perturbation of the FP64 start grows into."""
+    from vampire_amd import system as S
+    from vampire_amd.context import FP32, FP64, FUSED, Context
+
+    def run(sys_, prec, f64n, T, start, k, seed, store32=False):
+        c = Context(sys_, precision=prec, kernel=FUSED, seed=seed, device=0, fp64_normals=f64n)
+        c.set_conditions(T)
+        c.set_spins(start)
+        if store32:
+            for _ in range(k):
+                c.llg_heun(1)
+                c.set_spins(c.get_spins().astype(np.float32).astype(np.float64))
+        else:
+            c.llg_heun(k)
+        out = c.get_spins()
+        c.close()
+        return out
+
+    def cmp(a, b):
+        return dict(max_abs_dm=float(np.abs(a - b).max()),
+                    dM_over_Ms=float(np.linalg.norm(a.mean(axis=1) - b.mean(axis=1))))
+
+    sp = np.zeros((3, system.n))
+    sp[2] = 1.0
+    r64 = run(system, FP64, False, 600.0, sp, steps, 7)
+    same = cmp(run(system, FP32, True, 600.0, sp, steps, 7), r64)
+    own = cmp(run(system, FP32, False, 600.0, sp, steps, 7), r64)
+    s0 = S.config_c0()
+    sp0 = S.random_spins(s0.n, 3)
+    rng = np.random.default_rng(1)
+    pert = sp0 * (1.0 + rng.choice([-1.0, 1.0], sp0.shape) * 2.0 ** -24)
+    pert /= np.linalg.norm(pert, axis=0)
+    c64 = run(s0, FP64, False, 300.0, sp0, c0_steps, 9)
+    floor = dict(workload=f"SC 20^3, 8000 atoms, 300 K, {c0_steps} steps",
+                 fp32_same_normals=cmp(run(s0, FP32, True, 300.0, sp0, c0_steps, 9), c64)["max_abs_dm"],
+                 fp64_stored_as_float=cmp(run(s0, FP64, False, 300.0, sp0.astype(np.float32).astype(np.float64),
+                                              c0_steps, 9, store32=True), c64)["max_abs_dm"],
+                 fp64_one_perturbation=cmp(run(s0, FP64, False, 300.0, pert, c0_steps, 9), c64)["max_abs_dm"])
+    return dict(steps=steps, same_normals=same, own_normals=own, target_max_abs_dm=1e-5, c0_floor=floor,
+                max_abs_dm=same["max_abs_dm"], dM_over_Ms=same["dM_over_Ms"])
+
+
+def host_cpu():
+    """(model name, threads the host gives this process) for the CPU baseline: the cores in the
+    process's affinity mask, capped by OMP_NUM_THREADS where the environment sets the host share."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        cores = min(cores, int(env))
+    return model, cores
+
+
 def cpu_sample(system, threads, seconds):
     """Oracle Heun steps on the host until the sample lasts `seconds`: (spin-steps/s, steps)."""
     one = cpu_baseline(system, 1, threads)
@@ -388,25 +457,8 @@ def main():
     if args.copy_calibration:
         copy_calibration()
 
-    # FP32 vs FP64 drift after drift-steps from the same state and Philox stream
-    drift = {}
-    if args.drift_steps:
-        outs = {}
-        for prec in (FP64, FP32):
-            from vampire_amd.context import Context
-            c = Context(system, precision=prec, kernel=FUSED, seed=7, device=0)
-            c.set_conditions(600.0)
-            sp = np.zeros((3, n))
-            sp[2] = 1.0
-            c.set_spins(sp)
-            c.llg_heun(args.drift_steps)
-            outs[prec] = c.get_spins()
-            c.close()
-        d = np.abs(outs[FP32] - outs[FP64])
-        m64 = outs[FP64].mean(axis=1)
-        m32 = outs[FP32].mean(axis=1)
-        drift = dict(steps=args.drift_steps, max_abs_dm=float(d.max()),
-                     dM_over_Ms=float(np.linalg.norm(m32 - m64)), mz_fp64=float(m64[2]), mz_fp32=float(m32[2]))
+    drift = drift_leg(system, args.drift_steps) if args.drift_steps else {}
+    if drift:
         print(f"# drift: {drift}", file=sys.stderr, flush=True)
 
     head = res.get("fp32_fused")
@@ -434,11 +486,13 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        nthreads = min(16, os.cpu_count() or 1)
+        model, nthreads = host_cpu()
         (v1, k1), (vN, kN) = (cpu_sample(system, 1, args.cpu_seconds),
                               cpu_sample(system, nthreads, args.cpu_seconds))
         port = dict(value=round(vN, 1), cores=nthreads, steps=kN, value_1thread=round(v1, 1), steps_1thread=k1,
-                    what="FP64 oracle (the reference's loops restated in C), same workload, OpenMP")
+                    cpu_model=model, host_cpus=os.cpu_count(),
+                    what="FP64 oracle (the reference's loops restated in C), same workload, OpenMP over every "
+                         "core the host gives this process")
         print(f"# cpu port: 1 thread {v1:.3e}, {nthreads} threads {vN:.3e} spin-steps/s", file=sys.stderr, flush=True)
         vr, why = None, "drop-in executable not built (make -C vampire_amd/shim)"
         ref_steps = max(4, int(round(args.cpu_seconds * 2.5)))
@@ -453,10 +507,10 @@ def main():
                               f"reference's own sim::LLG_Heun: VAMPIRE built from its sources with gpu:acceleration=off, "
                               f"one thread as vampire-serial; the rate is the difference of a 2-step and a "
                               f"{2 + ref_steps}-step run, which cancels the set-up",
-                       port=port, port_1thread_over_reference=round(v1 / vr, 3))
+                       cpu_model=model, port=port, port_1thread_over_reference=round(v1 / vr, 3))
             print(f"# cpu reference: {vr:.3e} spin-steps/s (1 thread)", file=sys.stderr, flush=True)
         else:
-            cpu = dict(value=port["value"], unit="spin-steps/s", cores=nthreads, kind="port",
+            cpu = dict(value=port["value"], unit="spin-steps/s", cores=nthreads, kind="port", cpu_model=model,
                        sample=f"{kN} Heun steps of the same 4M-atom FCC system (FP64 oracle, OpenMP over {nthreads} "
                               f"threads, >= {args.cpu_seconds:.0f} s); 1 thread: {k1} steps, {v1:.4g} spin-steps/s",
                        value_1thread=port["value_1thread"], reference_skipped=why)

@@ -199,6 +199,14 @@ int vmx_set_time_step(vmx_ctx* ctx, double dt, double half_dt);
 int vmx_set_conditions(vmx_ctx* ctx, int32_t replica, double temperature, const double H[3],
                        uint32_t field_flags);
 
+/* The thin-film demagnetising field of calculate_applied_fields (src/simulate/fields.cpp:219-239,
+ * enabled by sim:demagnetisation-factor, sim::ext_demag): a uniform field (T) added to the step's
+ * applied field after the global and local ones, and not to the applied-field energy
+ * (src/simulate/energy.cpp:94-98).  The reference forms it from the latest system magnetisation
+ * statistic; the caller passes the field itself.  Applies while VMX_FIELD_APPLIED is set; zero
+ * removes it.  replica = -1 sets every replica.  Not held by checkpoints. */
+int vmx_set_demag_field(vmx_ctx* ctx, int32_t replica, const double H[3]);
+
 /* Advances every replica by nsteps stochastic LLG-Heun steps (sim::LLG_Heun,
  * src/simulate/LLGHeun.cpp:160-282; gpu::llg_heun, src/gpu/llg_heun.cpp:23).
  * noise == NULL: thermal noise is generated on the device by Philox-4x32-10 keyed by

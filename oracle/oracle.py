@@ -49,7 +49,7 @@ class or_sys(C.Structure):
 
 class or_cond(C.Structure):
     _fields_ = [("temperature", C.c_double), ("H", C.c_double * 3), ("fld", C.c_uint), ("dt", C.c_double),
-                ("half_dt", C.c_double)]
+                ("half_dt", C.c_double), ("Hd", C.c_double * 3)]
 
 
 _lib = None
@@ -178,10 +178,12 @@ class Oracle:
         return tuple(k[x].reshape(3, -1).copy() for x in ("atom_B", "atom_Hd", "cell_mag", "cell_B", "cell_Hd"))
 
     def set_conditions(self, temperature: float, H=(0.0, 0.0, 0.0), thermal=True, applied=True,
-                       local_applied=False):
+                       local_applied=False, demag=(0.0, 0.0, 0.0)):
+        """demag: the thin-film demagnetising field added after the applied field (sim::ext_demag)."""
         self.cond.temperature = float(temperature)
         for k in range(3):
             self.cond.H[k] = float(H[k])
+            self.cond.Hd[k] = float(demag[k])
         self.cond.fld = (1 if thermal else 0) | (2 if applied else 0) | (4 if local_applied else 0)
 
     def spin_fields(self, spins: np.ndarray) -> np.ndarray:

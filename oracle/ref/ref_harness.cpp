@@ -2,8 +2,11 @@
 // oracle/ref/Makefile) to produce golden vectors for the oracle.  Development container only.
 //
 //   ref_harness <rundir> dump                       system after cs::create + sim::run set-up
-//   ref_harness <rundir> llg <nsteps> <every> [mid] sim::LLG_Heun (sim::LLG_Midpoint with `mid`) with
-//                                                   injected noise
+//   ref_harness <rundir> llg <nsteps> <every> [mid|demag]
+//                                                   sim::LLG_Heun (sim::LLG_Midpoint with `mid`) with
+//                                                   injected noise; `demag`: statistics updated before
+//                                                   every step, so the thin-film demag field follows the
+//                                                   spins, and that field written per step (demag.bin)
 //   ref_harness <rundir> mc <nsweeps> <T> <every>   montecarlo::mc_step, reference RNG
 //   ref_harness <rundir> dipole                     macrocell dipole set-up and fields (dipole.bin)
 //
@@ -241,6 +244,26 @@ int main(int argc, char** argv) {
         const int nsteps = std::atoi(argv[3]);
         const int every = std::atoi(argv[4]);
         const bool midpoint = argc > 5 && std::string(argv[5]) == "mid";
+        const bool demag = argc > 5 && std::string(argv[5]) == "demag";
+        if (demag && !sim::ext_demag) {
+            std::cerr << "the input does not set sim:demagnetisation-factor" << std::endl;
+            return 2;
+        }
+        // the field calculate_applied_fields adds (src/simulate/fields.cpp:219-239) for the current
+        // system magnetisation statistic, after bringing that statistic up to date
+        std::ofstream demag_file;
+        if (demag) demag_file.open("demag.bin", std::ios::binary);
+        auto demag_update = [&]() {
+            if (!demag) return;
+            stats::update(atoms::x_spin_array, atoms::y_spin_array, atoms::z_spin_array, atoms::m_spin_array,
+                          atoms::type_array, sim::temperature);
+            const std::vector<double> m_l = stats::system_magnetization.get_magnetization();
+            const double mu_0 = -4.0 * M_PI * 1.0e-7 /
+                                (cs::system_dimensions[0] * cs::system_dimensions[1] * cs::system_dimensions[2] * 1.0e-30);
+            const std::vector<double> HD = {mu_0 * sim::demag_factor[0] * m_l[0], mu_0 * sim::demag_factor[1] * m_l[1],
+                                            mu_0 * sim::demag_factor[2] * m_l[2]};
+            put(demag_file, HD);
+        };
         std::vector<double> noise = read_doubles("noise.bin");
         // one extra 3N block at the front feeds the field dump before the first step
         if (noise.size() != (size_t)(nsteps + 1) * 3 * n) {
@@ -249,12 +272,14 @@ int main(int argc, char** argv) {
         }
         g_noise = noise.data();
         g_noise_len = noise.size();
+        demag_update();
         {
             std::ofstream ff("fields.bin", std::ios::binary);
             dump_fields(ff);
         }
         std::ofstream tf("traj.bin", std::ios::binary);
         for (int s = 1; s <= nsteps; ++s) {
+            if (s > 1) demag_update();
             if (midpoint) sim::LLG_Midpoint();
             else sim::LLG_Heun();
             // sim::increment_time refreshes the dipole field after each step (sim.cpp:191-200)

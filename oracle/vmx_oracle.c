@@ -103,6 +103,7 @@ typedef struct {
     double H[3];                 /* sim::H_vec * sim::H_applied */
     unsigned fld;                /* 1 thermal, 2 applied, 4 local applied */
     double dt, half_dt;
+    double Hd[3];                /* thin-film demag field of calculate_applied_fields (sim::ext_demag) */
 } or_cond;
 
 /* ---- Philox-4x32-10 and Box-Muller ------------------------------------------------------ */
@@ -293,6 +294,14 @@ void or_external_fields(const or_sys* s, const or_cond* c, const double* noise, 
             ex[atom] += happ[3 * m + 0];
             ey[atom] += happ[3 * m + 1];
             ez[atom] += happ[3 * m + 2];
+        }
+        /* thin-film demag field, after the global and local fields (fields.cpp:219-239) */
+        if (c->Hd[0] != 0.0 || c->Hd[1] != 0.0 || c->Hd[2] != 0.0) {
+            for (int atom = 0; atom < n; ++atom) {
+                ex[atom] += c->Hd[0];
+                ey[atom] += c->Hd[1];
+                ez[atom] += c->Hd[2];
+            }
         }
     }
     if (s->dip) {                                               /* fields.cpp:291-310 */

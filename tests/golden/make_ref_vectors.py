@@ -115,12 +115,14 @@ def digest(*arrays):
 
 
 def case(name, crystal, L, a, materials, steps, snaps, T, H=0.0, Hdir=(0, 0, 1), pbc=(True, True, True),
-         seed=11, spin_seed=5, dmi_cutoff=None, Lz=None, slim=False, integrator="heun"):
+         seed=11, spin_seed=5, dmi_cutoff=None, Lz=None, slim=False, integrator="heun", extra=()):
     """slim: store the neighbour list and exchange table as a SHA-256 digest (plus the distinct J
     values) instead of the arrays; the builder must reproduce them.  integrator: "heun"
-    (sim::LLG_Heun) or "midpoint" (sim::LLG_Midpoint) for the trajectory."""
+    (sim::LLG_Heun), "midpoint" (sim::LLG_Midpoint) or "demag" (Heun with the statistics updated
+    before every step, so the thin-film demag field of a sim:demagnetisation-factor deck follows the
+    spins; that field is stored per step as demag_field)."""
     with tempfile.TemporaryDirectory(prefix="vref_") as d:
-        inp, mat = deck(crystal, L, a, materials, pbc, T, H, Hdir, dmi_cutoff, Lz)
+        inp, mat = deck(crystal, L, a, materials, pbc, T, H, Hdir, dmi_cutoff, Lz, extra)
         open(os.path.join(d, "input"), "w").write(inp)
         open(os.path.join(d, "test.mat"), "w").write(mat)
         run(d, "dump")
@@ -134,8 +136,11 @@ def case(name, crystal, L, a, materials, steps, snaps, T, H=0.0, Hdir=(0, 0, 1),
         # noise: block for the field dump at step "-1" is Philox step 0, then steps 1..nsteps
         blocks = [O.thermal_noise(seed, n, s) for s in range(steps + 1)]
         np.concatenate([b.reshape(-1) for b in blocks]).tofile(os.path.join(d, "noise.bin"))
-        run(d, "llg", steps, 1, *(["mid"] if integrator == "midpoint" else []))
+        run(d, "llg", steps, 1, *({"midpoint": ["mid"], "demag": ["demag"]}.get(integrator, [])))
         fields = np.fromfile(os.path.join(d, "fields.bin"), dtype=np.float64).reshape(6, n)
+        extra_arrays = {}
+        if integrator == "demag":
+            extra_arrays["demag_field"] = np.fromfile(os.path.join(d, "demag.bin"), dtype=np.float64).reshape(steps, 3)
         traj = np.fromfile(os.path.join(d, "traj.bin"), dtype=np.float64).reshape(steps, 3, n)
         stats_after = np.fromfile(os.path.join(d, "stats.bin"), dtype=np.float64)
     big = {k: v for k, v in sysd.items() if k != "spins0"}
@@ -148,7 +153,7 @@ def case(name, crystal, L, a, materials, steps, snaps, T, H=0.0, Hdir=(0, 0, 1),
         os.path.join(OUT, f"ref_{name}.npz"),
         crystal=crystal, L=L, a=a, T=T, H=H, Hdir=np.array(Hdir, dtype=float), pbc=np.array(pbc), integrator=integrator,
         noise_seed=seed, spins_in=spins, snap_steps=np.array(snaps),
-        snaps=traj[np.array(snaps) - 1], fields=fields, stats_after=stats_after,
+        snaps=traj[np.array(snaps) - 1], fields=fields, stats_after=stats_after, **extra_arrays,
         dt=meta["dt"], half_dt=meta["half_dt"], exchange_type=meta["exchange_type"],
         materials=np.array([[m[k] for k in ("alpha", "gamma_rel", "mu_s_SI", "one_oneplusalpha_sq",
                                             "alpha_oneplusalpha_sq", "H_th_sigma", "ku2", "mu_s_muB")]
@@ -304,6 +309,11 @@ def main(only=()):
     if want("film_vectorial"):
         case("film_vectorial", "sc", 6, 2.5, [va, vb], steps=100, snaps=[1, 100], T=200.0, H=1.0,
              Hdir=(0, 1, 1), pbc=(True, True, False), Lz=4)
+    # thin-film demagnetising field (sim:demagnetisation-factor, fields.cpp:219-239): an SC film
+    # open in z, statistics before every step so the field follows the system magnetisation
+    if want("film_demag"):
+        case("film_demag", "sc", 6, 3.54, [CO], steps=50, snaps=[1, 10, 50], T=300.0, H=0.5, Hdir=(1, 0, 0),
+             pbc=(True, True, False), Lz=3, integrator="demag", extra=["sim:demagnetisation-factor = 0,0.2,0.8"])
     # the LLG midpoint integrator (sim::LLG_Midpoint) on the SC uniaxial and two-material DMI decks
     if want("sc6_uniaxial_mid"):
         case("sc6_uniaxial_mid", "sc", 6, 3.54, [CO], steps=100, snaps=[1, 10, 100], T=300.0, H=0.5,

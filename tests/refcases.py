@@ -24,6 +24,8 @@ CASES = {
     "film_dmi": dict(mats=[S.Material(mu_s_muB=1.6, alpha=0.1), S.Material(mu_s_muB=7.6, alpha=0.1)], ani={}),
     "c0_sc20": dict(mats=[S.Material(**CO)], ani=dict(uniaxial=True)),
     # vectorial exchange (three-value exchange-matrix): one material, then two layered materials
+    # thin-film demagnetising field (sim:demagnetisation-factor = 0,0.2,0.8), film open in z
+    "film_demag": dict(mats=[S.Material(**CO)], ani=dict(uniaxial=True)),
     "sc6_vectorial": dict(mats=[S.Material(**CO)], ani=dict(uniaxial=True),
                           J=[[[6.064e-21, 5.2e-21, 7.3e-21]]]),
     "film_vectorial": dict(mats=[S.Material(mu_s_muB=1.6, alpha=0.1, Ku=2.0e-24, ku_vector=(0.0, 0.0, 1.0)),

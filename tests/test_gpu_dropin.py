@@ -237,3 +237,158 @@ def test_dropin_checkpoint_continue_resumes_noise_stream(gpu, exe, tmp_path):
         assert a.shape == (324,)
         assert np.array_equal(a, b), (acc, np.abs(a - b).max())
         assert np.abs(a - r).max() > 1e-3, acc
+
+
+FILM = """create:crystal-structure=sc
+create:periodic-boundaries-x
+create:periodic-boundaries-y
+dimensions:unit-cell-size = 3.54 !A
+dimensions:system-size-x = 21.23 !A
+dimensions:system-size-y = 21.23 !A
+dimensions:system-size-z = 14.15 !A
+"""
+
+TWO = """material:num-materials=2
+material[1]:material-name=A
+material[1]:damping-constant=0.1
+material[1]:exchange-matrix[1]=6.064e-21
+material[1]:exchange-matrix[2]=-2.0e-21
+material[1]:atomic-spin-moment=1.72 !muB
+material[1]:second-order-uniaxial-anisotropy-constant=4.644e-24
+material[1]:initial-spin-direction=random
+material[1]:maximum-height=0.5
+material[2]:material-name=B
+material[2]:damping-constant=0.05
+material[2]:exchange-matrix[1]=-2.0e-21
+material[2]:exchange-matrix[2]=5.0e-21
+material[2]:atomic-spin-moment=2.5 !muB
+material[2]:initial-spin-direction=random
+material[2]:minimum-height=0.5
+"""
+
+STATS_OUT = """output:height-magnetisation
+output:material-height-magnetisation
+output:material-magnetisation
+output:total-energy
+output:exchange-energy
+output:anisotropy-energy
+output:applied-field-energy
+output:mean-total-energy
+output:material-total-energy
+output:material-mean-total-energy
+output:mean-magnetisation-length
+"""
+
+
+def assert_columns_close(dev, cpu, rtol_scale, rtol_print=1e-5):
+    """The output file prints six significant digits, so two runs that agree to 1e-10 can still
+    print one unit apart in the last digit: each value must match within rtol_print of itself plus
+    rtol_scale of its column's largest value; failures name the columns."""
+    scale = np.abs(cpu).max(axis=0)
+    tol = rtol_print * np.abs(cpu) + rtol_scale * scale + 1e-300
+    diff = np.abs(dev - cpu)
+    bad = [(k, float(diff[:, k].max()), float(scale[k])) for k in range(cpu.shape[1]) if (diff[:, k] > tol[:, k]).any()]
+    assert not bad, f"columns (index, max difference, column scale) beyond tolerance: {bad}"
+
+
+def run_film(tmp, name, acc, extra="", env=None, T=0.0, steps=500, inc=50, H=0.5, Hdir="0.3,0,1", check=True):
+    d = tmp / name
+    d.mkdir()
+    kw = dict(T=T, inc=inc, steps=steps, H=H, Hdir=Hdir, acc=acc)
+    (d / "input").write_text(FILM + RUN.format(**kw) + extra)
+    (d / "Co.mat").write_text(TWO)
+    r = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=200,
+                       env=None if env is None else {**os.environ, **env})
+    if not check:
+        return r, d
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    rows = np.array([[float(v) for v in line.split()] for line in (d / "output").read_text().splitlines()
+                     if line.strip() and not line.startswith("#")])
+    raw = (d / "spins-00000000.data").read_bytes()
+    return rows, np.frombuffer(raw[8:], dtype=np.float64).copy()
+
+
+@pytest.mark.parametrize("stats", ["host", "device"])
+def test_dropin_statistics_equal_reference_cpu(gpu, exe, tmp_path, stats):
+    """Two-material layered film (open in z, antiparallel inter-layer coupling), T = 0: every
+    statistic in the output file (system, material, height and material-height magnetisation;
+    total, exchange, anisotropy, applied-field and material energies and their means) equals the
+    reference CPU run's, with statistics on the host (the backend's default, gpu:cpu-statistics)
+    and on the device (gpu:calculate-statistics-on-cpu=false: magnetisation and energy sums
+    reduced on the device, running means handed back through set_magnetization and
+    set_*_energy / update_mean_counter)."""
+    extra = STATS_OUT + ("gpu:calculate-statistics-on-cpu=false\n" if stats == "device" else "")
+    cpu, s_cpu = run_film(tmp_path, "cpu", "off", STATS_OUT)
+    dev, s_dev = run_film(tmp_path, "gpu", "on", extra)
+    assert cpu.shape == dev.shape and cpu.shape[0] == 10 and cpu.shape[1] > 30
+    assert np.abs(s_dev - s_cpu).max() <= 1e-10
+    assert_columns_close(dev, cpu, 1e-9)
+
+
+@pytest.mark.parametrize("stats", ["host", "device"])
+def test_dropin_thin_film_demag_equals_reference_cpu(gpu, exe, tmp_path, stats):
+    """sim:demagnetisation-factor (calculate_applied_fields, src/simulate/fields.cpp:219-239): the
+    field follows the system magnetisation statistic, updated every 10 steps.  Device path ==
+    reference CPU (<= 1e-10) with host and with device statistics; without the term the run ends
+    elsewhere, so the term is in the comparison."""
+    demag = "sim:demagnetisation-factor = 0.1,0.1,0.8\n"
+    dev_stats = "gpu:calculate-statistics-on-cpu=false\n" if stats == "device" else ""
+    kw = dict(steps=200, inc=10)
+    cpu, s_cpu = run_film(tmp_path, "cpu", "off", demag, **kw)
+    dev, s_dev = run_film(tmp_path, "gpu", "on", demag + dev_stats, **kw)
+    plain, s_plain = run_film(tmp_path, "plain", "off", "", **kw)
+    assert np.abs(s_dev - s_cpu).max() <= 1e-10
+    assert_columns_close(dev, cpu, 1e-9)
+    assert np.abs(s_plain - s_cpu).max() > 1e-3
+
+
+def test_dropin_single_precision_against_reference_cpu(gpu, exe, tmp_path):
+    """VMX_PRECISION=single selects the FP32 step through gpu:: (the reference backend's CUDA_SP
+    build choice, src/cuda/makefile:36-38, made at run time), with VMX_AUTOTUNE=1: 500 steps of the
+    film at T = 0 stay within 5e-5 per spin component of the reference CPU run (about a hundred
+    FP32 roundings, 6e-8 each; measured 1.2e-5), and the output's statistics within 5e-5
+    relative."""
+    cpu, s_cpu = run_film(tmp_path, "cpu", "off", STATS_OUT)
+    dev, s_dev = run_film(tmp_path, "gpu", "on", STATS_OUT, env={"VMX_PRECISION": "single", "VMX_AUTOTUNE": "1"})
+    err = np.abs(s_dev - s_cpu).max()
+    assert 1e-10 < err <= 5e-5, err           # FP32 (not the FP64 step) and close
+    assert_columns_close(dev, cpu, 5e-5)
+    r, _ = run_film(tmp_path, "bad", "on", env={"VMX_PRECISION": "half"}, check=False)
+    assert r.returncode != 0 and "VMX_PRECISION" in r.stderr
+
+
+def test_dropin_refuses_terms_the_device_does_not_compute(gpu, exe, tmp_path):
+    """A deck with material-specific temperatures (sim::local_temperature, fields.cpp:264) is
+    refused by gpu::initialize instead of running without the term."""
+    r, _ = run_film(tmp_path, "local", "on", "sim:temperature=300\n", check=False)
+    assert r.returncode == 0
+    mat = TWO.replace("material[2]:minimum-height=0.5", "material[2]:minimum-height=0.5\nmaterial[2]:temperature=600")
+    d = tmp_path / "refused"
+    d.mkdir()
+    (d / "input").write_text(FILM + RUN.format(T=300.0, inc=50, steps=100, H=0.5, Hdir="0,0,1", acc="on"))
+    (d / "Co.mat").write_text(mat)
+    r = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "does not compute material-specific temperatures" in r.stderr
+
+
+def test_dropin_continuous_checkpoints_hold_device_spins(gpu, exe, tmp_path):
+    """sim:save-checkpoint=continuous writes vampire0.chk from the host spin arrays at output steps
+    (src/vio/datalog.cpp:487, after gpu::stats::get); with device statistics nothing else copies
+    the spins back, so the drop-in synchronises them first.  A 250-step run at 300 K with output
+    every 50 steps and checkpoints every 100 leaves the step-200 checkpoint; resuming it with
+    sim:load-checkpoint=continue ends bit for bit where the uninterrupted run ends."""
+    extra = ("gpu:calculate-statistics-on-cpu=false\nsim:save-checkpoint=continuous\n"
+             "sim:save-checkpoint-rate=100\n")
+    _, a = run_film(tmp_path, "A", "on", extra, T=300.0, steps=250, inc=50)
+    d = tmp_path / "B"
+    d.mkdir()
+    (d / "vampire0.chk").write_bytes((tmp_path / "A" / "vampire0.chk").read_bytes())
+    kw = dict(T=300.0, inc=50, steps=250, H=0.5, Hdir="0.3,0,1", acc="on")
+    (d / "input").write_text(FILM + RUN.format(**kw) + "gpu:calculate-statistics-on-cpu=false\n"
+                             "sim:load-checkpoint=continue\n")
+    (d / "Co.mat").write_text(TWO)
+    r = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    b = np.frombuffer(sorted(d.glob("spins-*.data"))[-1].read_bytes()[8:], dtype=np.float64)
+    assert np.array_equal(a, b), np.abs(a - b).max()

@@ -308,3 +308,31 @@ def test_ragged_stencils_small_and_large_tables(gpu, dims, lds, kernel):
         c.set_spins(spins)
         c.llg_heun(20, noise=noise)
         assert np.abs(c.get_spins() - ref).max() <= 1e-10
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_thin_film_demag_matches_reference(gpu, kernel):
+    """The thin-film demag field (vmx_set_demag_field; calculate_applied_fields,
+    src/simulate/fields.cpp:219-239) set at every step to the field the reference formed from its
+    system magnetisation statistic: the device step reproduces the reference's trajectory
+    (<= 1e-10, FP64), fields included, for every kernel variant."""
+    f = RC.load("film_demag")
+    s = RC.system_of("film_demag")
+    T, H = RC.conditions("film_demag")
+    hd = f["demag_field"]
+    seed = int(f["noise_seed"])
+    snaps = [int(v) for v in f["snap_steps"]]
+    with _ctx(s, kernel=kernel) as c:
+        c.set_conditions(T, H)
+        c.set_spins(f["spins_in"])
+        c.set_demag_field(hd[0])
+        hs, he = c.fields(noise=O.thermal_noise(seed, s.n, 0))
+        scale = np.abs(f["fields"]).max()
+        assert np.abs(he - f["fields"][3:]).max() <= 2.3e-16 * scale
+        k = 0
+        for st in range(1, snaps[-1] + 1):
+            c.set_demag_field(hd[st - 1])
+            c.llg_heun(1, noise=O.thermal_noise(seed, s.n, st)[None])
+            if st == snaps[k]:
+                assert np.abs(c.get_spins() - f["snaps"][k]).max() <= 1e-10, st
+                k += 1

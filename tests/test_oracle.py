@@ -514,3 +514,31 @@ def test_checkerboard_colouring_is_proper():
         rows = np.repeat(np.arange(s.n), s.nbr_end - s.nbr_start + 1)
         assert (col[rows] != col[s.nbr]).all()
         assert sorted(lst.tolist()) == list(range(s.n))
+
+
+# ---------------------------------------------------------------- thin-film demagnetising field
+def test_demag_field_trajectory_equals_reference():
+    """sim:demagnetisation-factor: calculate_applied_fields adds -mu0 D m_l / V after the applied
+    field (src/simulate/fields.cpp:219-239).  Fed the field the reference formed at each step (its
+    system magnetisation statistic updated before every step), the oracle's fields and trajectory
+    are the reference's bit for bit; without the term they are not."""
+    f = RC.load("film_demag")
+    s = RC.system_of("film_demag")
+    T, H = RC.conditions("film_demag")
+    hd = f["demag_field"]
+    seed = int(f["noise_seed"])
+    o = O.Oracle(s)
+    o.set_conditions(T, H, demag=hd[0])
+    assert np.array_equal(o.spin_fields(f["spins_in"]), f["fields"][:3])
+    assert np.array_equal(o.external_fields(O.thermal_noise(seed, s.n, 0)), f["fields"][3:])
+    cur, k = f["spins_in"], 0
+    snaps = [int(v) for v in f["snap_steps"]]
+    for st in range(1, snaps[-1] + 1):
+        o.set_conditions(T, H, demag=hd[st - 1])
+        cur = o.llg_heun(cur, 1, noise=O.thermal_noise(seed, s.n, st)[None])
+        if st == snaps[k]:
+            assert np.array_equal(cur, f["snaps"][k]), st
+            k += 1
+    o.set_conditions(T, H)
+    plain = o.llg_heun(f["spins_in"], 1, noise=O.thermal_noise(seed, s.n, 1)[None])
+    assert np.abs(plain - f["snaps"][0]).max() > 1e-3

@@ -134,6 +134,7 @@ SIGNATURES = [
     ("vmx_get_spins", C.c_int, [_VP, _DP, _DP, _DP]),
     ("vmx_set_time_step", C.c_int, [_VP, C.c_double, C.c_double]),
     ("vmx_set_conditions", C.c_int, [_VP, C.c_int32, C.c_double, _DP, C.c_uint32]),
+    ("vmx_set_demag_field", C.c_int, [_VP, C.c_int32, _DP]),
     ("vmx_llg_heun", C.c_int, [_VP, C.c_int32, _DP]),
     ("vmx_llg_midpoint", C.c_int, [_VP, C.c_int32, _DP]),
     ("vmx_llg_heun_graph", C.c_int, [_VP, C.c_int32, C.c_int32]),

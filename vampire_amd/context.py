@@ -121,6 +121,12 @@ class Context:
         h = (C.c_double * 3)(*[float(v) for v in H])
         L.check(self.lib.vmx_set_conditions(self.h, replica, float(temperature), h, flags))
 
+    def set_demag_field(self, H: Sequence[float], replica: int = -1) -> None:
+        """Thin-film demagnetising field (T): added to the step's applied field, not to the
+        applied-field energy (calculate_applied_fields, src/simulate/fields.cpp:219-239)."""
+        h = (C.c_double * 3)(*[float(v) for v in H])
+        L.check(self.lib.vmx_set_demag_field(self.h, replica, h))
+
     def set_time_step(self, dt: float, half_dt: Optional[float] = None) -> None:
         L.check(self.lib.vmx_set_time_step(self.h, dt, 0.5 * dt if half_dt is None else half_dt))
 

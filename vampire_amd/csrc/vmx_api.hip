@@ -62,6 +62,14 @@ struct DevBuf {
     template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Copies a host vector into b (allocated to fit, at least one element).
+template <typename T>
+int upload(DevBuf& b, const std::vector<T>& v) {
+    VMX_TRY(b.alloc(std::max<size_t>(1, v.size()) * sizeof(T)));
+    if (!v.empty()) VMX_HIP(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return VMX_OK;
+}
+
 // Pinned staging ring for small host->device updates (conditions, sigma): a slot is reused only
 // after the copy that read it has completed.
 struct PinnedRing {
@@ -400,6 +408,11 @@ struct vmx_ctx {
     std::vector<int> rep_off_h;
     std::vector<double> T_h;      // per replica
     std::vector<double> H_h;      // per replica [3]
+    std::vector<double> Hd_h;     // per replica [3]: thin-film demag field, LLG applied field only
+    bool demag_on() const {
+        for (double v : Hd_h) if (v != 0.0) return true;
+        return false;
+    }
 
     // device state
     DevBuf row, col, jidx, jtab, type, mat, cond, blk, blk_fused, rep_off, rep_key, step_base;
@@ -475,7 +488,9 @@ struct vmx_ctx {
     bool coloured = false;
     int ncolours = 0;
     std::vector<int> colour_start;
-    DevBuf colour_list, mc_sigma, mc_rej, mc_tot_moves, mc_tot_rej;
+    static constexpr int kMcBlock = 256;   // list entries per Metropolis workgroup
+    std::vector<int> mc_blk_start;         // colour k's workgroups: mc_blk[mc_blk_start[k] ...]
+    DevBuf colour_list, mc_blk, mc_sigma, mc_rej, mc_tot_moves, mc_tot_rej;
 
     // graphs: (nsteps) -> executable
     hipGraphExec_t graph_exec = nullptr;
@@ -557,7 +572,7 @@ struct vmx_ctx {
         s.step_base = step_base.as<unsigned long long>();
         s.hdip = hdip_on ? hdip.as<R>() : nullptr;
         s.hd = dip.on ? dip.hd_atom.as<double>() : nullptr;
-        s.ani = ani; s.fld = fld;
+        s.ani = ani; s.fld = fld | (demag_on() ? FLD_DEMAG : 0u);
         s.dt = (R)dt; s.half_dt = (R)half_dt;
         return s;
     }
@@ -671,6 +686,7 @@ struct vmx_ctx {
                     double hl = h[k];
                     if (fld & FLD_LOCAL) hl = h[k] + a.applied_field_strength * a.applied_field_unit_vector[k];
                     c.happ[k] = (R)hl;
+                    c.hdemag[k] = (R)Hd_h[3 * (size_t)r + k];
                     c.hglob[k] = (R)h[k];
                 }
                 c.kBTBohr = (R)(9.27400915e-24 / (rescaled_temperature * 1.3806503e-23));
@@ -702,7 +718,9 @@ struct vmx_ctx {
     template <typename R>
     int capture_graph(int nsteps);
     template <typename R>
-    int stats_update();
+    int stats_update(double* fixed = nullptr);
+    int stats_grid(int nbins) const;
+    size_t stats_partial_bytes() const;
     template <typename R>
     int mc_sweeps(int nsweeps);
     int colour();
@@ -880,40 +898,47 @@ int vmx_ctx::capture_graph(int nsteps) {
     return VMX_OK;
 }
 
+// Workgroups of the partial-sum kernels.  Each thread walks its atoms with a grid stride, one
+// dependent memory round trip per atom, so masks with few bins get eight workgroups per CU
+// (2048); masks with many bins keep 512, which bounds the partials k_*_finish reduces.
+int vmx_ctx::stats_grid(int nbins) const {
+    return std::max(1, std::min((n + 255) / 256, nbins <= 16 ? 2048 : 512));
+}
+// Bytes of workgroup partials the largest registered statistic needs.
+size_t vmx_ctx::stats_partial_bytes() const {
+    size_t b = 0;
+    for (const MagStat* st : mags) b = std::max(b, (size_t)stats_grid(st->nbins) * 4 * st->nbins * sizeof(double));
+    for (const EnergyStat* st : energies)
+        b = std::max(b, (size_t)stats_grid(st->nbins) * 4 * st->nbins * sizeof(double));
+    return b;
+}
+
+// fixed: device memory of at least stats_partial_bytes() for the workgroup partials; null: the
+// context's own buffer, grown as needed.
 template <typename R>
-int vmx_ctx::stats_update() {
+int vmx_ctx::stats_update(double* fixed) {
     const Spin3<R> S = spins<R>(s_idx);
-    // Workgroups of the partial-sum kernels.  Each thread walks its atoms with a grid stride, one
-    // dependent memory round trip per atom, so masks with few bins get eight workgroups per CU
-    // (2048); masks with many bins keep 512, which bounds the partials k_*_finish reduces.
-    auto grid_for = [&](int nbins) { return std::max(1, std::min((n + 255) / 256, nbins <= 16 ? 2048 : 512)); };
+    double* part = fixed;
+    if (!part) {
+        VMX_TRY(partial.alloc(stats_partial_bytes()));
+        part = partial.as<double>();
+    }
     for (MagStat* st : mags) {
-        const int grid = grid_for(st->nbins);
-        VMX_TRY(partial.alloc((size_t)grid * 4 * st->nbins * sizeof(double)));
-        VMX_HIP(Ops<R>::mag_partial(grid, stream, n, S, st->mask.as<int>(), mm.as<double>(), st->nbins,
-                                    partial.as<double>()));
-        hipLaunchKernelGGL(k_mag_finish, dim3(1), dim3(256), 0, stream, grid, partial.as<double>(), st->nbins,
+        const int grid = stats_grid(st->nbins);
+        VMX_HIP(Ops<R>::mag_partial(grid, stream, n, S, st->mask.as<int>(), mm.as<double>(), st->nbins, part));
+        hipLaunchKernelGGL(k_mag_finish, dim3(1), dim3(256), 0, stream, grid, part, st->nbins,
                            st->cur.as<double>(), st->mean.as<double>(), st->empty_dev.as<int>(),
                            (int)st->empty.size(), st->counter.as<long long>());
         VMX_HIP(hipGetLastError());
     }
     if (!energies.empty()) {
         const SysDev<R> s = sys<R>();
-        if (nrep > 1 && !atom_rep.p) {
-            std::vector<int> ar(n);
-            for (int r = 0; r < nrep; ++r)
-                for (int a = rep_off_h[r]; a < rep_off_h[r + 1]; ++a) ar[a] = r;
-            VMX_TRY(atom_rep.alloc((size_t)n * sizeof(int)));
-            VMX_HIP(hipMemcpy(atom_rep.p, ar.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
-        }
         for (EnergyStat* st : energies) {
-            const int grid = grid_for(st->nbins);
-            VMX_TRY(partial.alloc((size_t)grid * 4 * st->nbins * sizeof(double)));
-            VMX_HIP(Ops<R>::energy_partial(grid, stream, et, am(),s, S, st->mask.as<int>(), mm.as<double>(),
-                                           nrep > 1 ? atom_rep.as<int>() : nullptr, st->nbins,
-                                           partial.as<double>()));
-            hipLaunchKernelGGL(k_energy_finish, dim3(1), dim3(256), 0, stream, grid, partial.as<double>(),
-                               st->nbins, st->cur.as<double>(), st->mean.as<double>(), st->empty_dev.as<int>(),
+            const int grid = stats_grid(st->nbins);
+            VMX_HIP(Ops<R>::energy_partial(grid, stream, et, am(), s, S, st->mask.as<int>(), mm.as<double>(),
+                                           nrep > 1 ? atom_rep.as<int>() : nullptr, st->nbins, part));
+            hipLaunchKernelGGL(k_energy_finish, dim3(1), dim3(256), 0, stream, grid, part, st->nbins,
+                               st->cur.as<double>(), st->mean.as<double>(), st->empty_dev.as<int>(),
                                (int)st->empty.size(), st->counter.as<long long>());
             VMX_HIP(hipGetLastError());
         }
@@ -955,19 +980,30 @@ int vmx_ctx::colour() {
         c[i] = k;
         nc = std::max(nc, k + 1);
     }
+    // Each colour lists its atoms replica by replica in storage order (moves of one colour commute,
+    // so the order is free: storage order keeps a wavefront's atoms, and their neighbours, close
+    // in memory), and each replica's run is cut into workgroups of 256 entries.
     std::vector<int> list;
+    std::vector<int4> tab;
     list.reserve(n);
     colour_start.assign(nc + 1, 0);
+    mc_blk_start.assign(nc + 1, 0);
     for (int k = 0; k < nc; ++k) {
         colour_start[k] = (int)list.size();
-        for (int o = 0; o < n; ++o) {
-            const int i = q > 1 ? pi[o] : o;
-            if (c[i] == k) list.push_back(i);
+        mc_blk_start[k] = (int)tab.size();
+        for (int r = 0; r < nrep; ++r) {
+            const int first = (int)list.size();
+            for (int i = rep_off_h[r]; i < rep_off_h[r + 1]; ++i)
+                if (c[i] == k) list.push_back(i);
+            const int last = (int)list.size();
+            for (int a = first; a < last; a += kMcBlock)
+                tab.push_back(make_int4(a, r, std::min(a + kMcBlock, last), rep_off_h[r]));
         }
     }
     colour_start[nc] = (int)list.size();
-    VMX_TRY(colour_list.alloc((size_t)n * sizeof(int)));
-    VMX_HIP(hipMemcpy(colour_list.p, list.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+    mc_blk_start[nc] = (int)tab.size();
+    VMX_TRY(upload(colour_list, list));
+    VMX_TRY(upload(mc_blk, tab));
     ncolours = nc;
     coloured = true;
     return VMX_OK;
@@ -980,8 +1016,8 @@ int vmx_ctx::mc_sweeps(int nsweeps) {
     const Spin3<R> S = spins<R>(s_idx);
     for (int w = 0; w < nsweeps; ++w) {
         for (int k = 0; k < ncolours; ++k) {
-            VMX_HIP(Ops<R>::mc_colour(stream, et, am(),s, S, colour_list.as<int>() + colour_start[k],
-                                      colour_start[k + 1] - colour_start[k], nrep, mc_sigma.as<double>(), sweep,
+            VMX_HIP(Ops<R>::mc_colour(stream, et, am(), s, S, colour_list.as<int>(), mc_blk.as<int4>() + mc_blk_start[k],
+                                      mc_blk_start[k + 1] - mc_blk_start[k], mc_sigma.as<double>(), sweep,
                                       mc_rej.as<double>()));
         }
         hipLaunchKernelGGL(k_mc_adapt, dim3((nrep + 255) / 256), dim3(256), 0, stream, nrep, rep_off.as<int>(),
@@ -1236,13 +1272,6 @@ bool build_stencils(int n, int zmax, const std::vector<int>& rowh, const std::ve
         pat[i] = (uint16_t)p;
     }
     return true;
-}
-
-template <typename T>
-int upload(DevBuf& b, const std::vector<T>& v) {
-    VMX_TRY(b.alloc(std::max<size_t>(1, v.size()) * sizeof(T)));
-    if (!v.empty()) VMX_HIP(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    return VMX_OK;
 }
 
 }  // namespace
@@ -1507,6 +1536,7 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
     c->rep_off_h = roff;
     c->T_h.assign(nrep, 0.0);
     c->H_h.assign(3 * (size_t)nrep, 0.0);
+    c->Hd_h.assign(3 * (size_t)nrep, 0.0);
     VMX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     VMX_HIP(hipEventCreate(&c->ev0));
     VMX_HIP(hipEventCreate(&c->ev1));
@@ -1535,6 +1565,10 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
     VMX_HIP(hipMemcpy(c->mm.p, mom_int.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice));
     VMX_TRY(c->rep_off.alloc((nrep + 1) * sizeof(int)));
     VMX_HIP(hipMemcpy(c->rep_off.p, roff.data(), (nrep + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if (nrep > 1) {   // replica of every atom (the energy statistics' applied field); replicas keep
+                      // their atom ranges in the internal order
+        VMX_TRY(upload(c->atom_rep, atom_rep_h));
+    }
     {
         std::vector<int> rep_m(nrep);
         for (int r = 0; r < nrep; ++r) rep_m[r] = (roff[r + 1] - roff[r]) / q;
@@ -1670,6 +1704,20 @@ int vmx_set_conditions(vmx_ctx* c, int32_t replica, double temperature, const do
     // the field flags are kernel arguments recorded in the graph
     if ((field_flags & 0x7u) != c->fld) c->drop_graph();
     c->fld = field_flags & 0x7u;
+    return c->upload_conditions_any();
+}
+
+int vmx_set_demag_field(vmx_ctx* c, int32_t replica, const double H[3]) {
+    if (!c || !H) return fail(VMX_ERR_ARG, "null argument");
+    if (replica < -1 || replica >= c->nrep) return fail(VMX_ERR_ARG, "replica %d outside [-1,%d)", replica, c->nrep);
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(H[k])) return fail(VMX_ERR_ARG, "demag field component %d is not finite", k);
+    VMX_TRY(c->use_device());
+    const bool was_on = c->demag_on();
+    const int r0 = replica < 0 ? 0 : replica, r1 = replica < 0 ? c->nrep : replica + 1;
+    for (int r = r0; r < r1; ++r)
+        for (int k = 0; k < 3; ++k) c->Hd_h[3 * (size_t)r + k] = H[k];
+    if (c->demag_on() != was_on) c->drop_graph();   // the term's flag is a recorded kernel argument
     return c->upload_conditions_any();
 }
 

@@ -46,7 +46,8 @@ struct Access {
 
 enum : uint32_t {
     ANI_UNI2 = 0x1u, ANI_CUB4 = 0x2u, ANI_CUB4R = 0x4u,
-    FLD_THERMAL = 0x1u, FLD_APPLIED = 0x2u, FLD_LOCAL = 0x4u
+    FLD_THERMAL = 0x1u, FLD_APPLIED = 0x2u, FLD_LOCAL = 0x4u,
+    FLD_DEMAG = 0x8u          // set by the library while a thin-film demag field is set
 };
 
 template <typename R>
@@ -66,6 +67,7 @@ struct Cond {                // per (replica, material); rewritten when conditio
     R happ[3];               // applied field of the LLG step: H or H + H_local (fields.cpp:171-210)
     R hglob[3];              // H of the applied-field energy (src/simulate/energy.cpp:94-98)
     R kBTBohr;               // mu_B / (kB T_rescaled)            (src/montecarlo/mc.cpp:45-53)
+    R hdemag[3];             // thin-film demag field, added after happ (fields.cpp:219-239)
 };
 
 template <typename R>
@@ -475,6 +477,9 @@ __device__ __forceinline__ void ext_from_normals(const SysDev<R>& s, const Cond<
     }
     if (s.fld & FLD_APPLIED) {
         ex += c.happ[0]; ey += c.happ[1]; ez += c.happ[2];
+        if (s.fld & FLD_DEMAG) {
+            ex += c.hdemag[0]; ey += c.hdemag[1]; ez += c.hdemag[2];
+        }
     }
     if (s.hdip) {
         ex += s.hdip[i]; ey += s.hdip[(size_t)s.n + i]; ez += s.hdip[2 * (size_t)s.n + i];
@@ -1101,39 +1106,56 @@ k_mag_partial(int n, Spin3<R> S, const int* __restrict__ mask, const double* __r
 // (src/anisotropy/energy.cpp:31-50 with uniaxial_second_order.cpp:104-121,
 // cubic_fourth_order.cpp:80-95, cubic_fourth_order_rotation.cpp:104-140) and applied field
 // (src/simulate/energy.cpp:94-98).
+// One neighbour's term of the exchange energy of spin (sx, sy, sz), subtracted from `energy`.
+template <typename R, int ET>
+__device__ __forceinline__ void exchange_energy_term(const R* J, R Sx, R Sy, R Sz, R sx, R sy, R sz, R& energy) {
+    if (ET == ET_ISO) {
+        energy -= J[0] * (Sx * sx + Sy * sy + Sz * sz);
+    } else if (ET == ET_VEC) {
+        energy -= (J[0] * Sx * sx + J[1] * Sy * sy + J[2] * Sz * sz);
+    } else {
+        energy -= (J[0] * Sx * sx + J[1] * Sy * sx + J[2] * Sz * sx +
+                   J[3] * Sx * sy + J[4] * Sy * sy + J[5] * Sz * sy +
+                   J[6] * Sx * sz + J[7] * Sy * sz + J[8] * Sz * sz);
+    }
+}
+
 template <typename R, int ET, int AM>
 __device__ __forceinline__ R exchange_energy(const SysDev<R>& s, const R* __restrict__ X,
                                              const R* __restrict__ Y, const R* __restrict__ Z,
                                              int i, R sx, R sy, R sz) {
     R energy = R(0);
     for_neighbours<R, ET, AM>(s, X, Y, Z, i, [&](const R Sx, const R Sy, const R Sz, const R* J) {
-        if (ET == ET_ISO) {
-            energy -= J[0] * (Sx * sx + Sy * sy + Sz * sz);
-        } else if (ET == ET_VEC) {
-            energy -= (J[0] * Sx * sx + J[1] * Sy * sy + J[2] * Sz * sz);
-        } else {
-            energy -= (J[0] * Sx * sx + J[1] * Sy * sx + J[2] * Sz * sx +
-                       J[3] * Sx * sy + J[4] * Sy * sy + J[5] * Sz * sy +
-                       J[6] * Sx * sz + J[7] * Sy * sz + J[8] * Sz * sz);
-        }
+        exchange_energy_term<R, ET>(J, Sx, Sy, Sz, sx, sy, sz, energy);
     });
     return energy;
 }
 
-template <typename R>
-__device__ __forceinline__ R anisotropy_energy(const SysDev<R>& s, int m, R sx, R sy, R sz) {
+// The same energy from a staged row (neighbour spins already in registers), in row order.
+template <typename R, int ET, int AM>
+__device__ __forceinline__ R row_exchange_energy(const SysDev<R>& s, const RowIdx<AM>& r, const RowSpins<R, AM>& v,
+                                                 const R* ju, R sx, R sy, R sz) {
     R energy = R(0);
-    if (!s.ani) return energy;
-    const MatConst<R>& mc = s.mat[m];
-    if (s.ani & ANI_UNI2) {
+    row_sum<R, ET, AM>(s, r, v, ju, [&](const R Sx, const R Sy, const R Sz, const R* J) {
+        exchange_energy_term<R, ET>(J, Sx, Sy, Sz, sx, sy, sz, energy);
+    });
+    return energy;
+}
+
+// Anisotropy energy with the material record mc (from LDS or memory).
+template <typename R>
+__device__ __forceinline__ R anisotropy_energy(uint32_t ani, const MatConst<R>& mc, R sx, R sy, R sz) {
+    R energy = R(0);
+    if (!ani) return energy;
+    if (ani & ANI_UNI2) {
         const R sdote = (sx * mc.ku[0] + sy * mc.ku[1] + sz * mc.ku[2]);
         energy += -mc.ku2 * (sdote * sdote);
     }
-    if (s.ani & ANI_CUB4) {
+    if (ani & ANI_CUB4) {
         const R sx4 = sx * sx * sx * sx, sy4 = sy * sy * sy * sy, sz4 = sz * sz * sz * sz;
         energy += R(-0.5) * mc.kc4 * (sx4 + sy4 + sz4);
     }
-    if (s.ani & ANI_CUB4R) {
+    if (ani & ANI_CUB4R) {
         const R d1 = sx * mc.e1[0] + sy * mc.e1[1] + sz * mc.e1[2];
         const R d2 = sx * mc.e2[0] + sy * mc.e2[1] + sz * mc.e2[2];
         const R d3 = sx * mc.e3[0] + sy * mc.e3[1] + sz * mc.e3[2];
@@ -1141,6 +1163,12 @@ __device__ __forceinline__ R anisotropy_energy(const SysDev<R>& s, int m, R sx, 
         energy += R(-0.5) * mc.kc4 * (d1f + d2f + d3f);
     }
     return energy;
+}
+
+template <typename R>
+__device__ __forceinline__ R anisotropy_energy(const SysDev<R>& s, int m, R sx, R sy, R sz) {
+    if (!s.ani) return R(0);
+    return anisotropy_energy(s.ani, s.mat[m], sx, sy, sz);
 }
 
 template <typename R>
@@ -1224,75 +1252,86 @@ k_energy_partial(SysDev<R> s, Spin3<R> S, const int* __restrict__ mask, const do
 // sees fixed neighbours and the updates of one launch commute.  Move and acceptance follow
 // montecarlo::mc_step (src/montecarlo/mc.cpp:59-108) with the adaptive cone of
 // mc_moves.cpp:71-87,158-162.
-template <typename R>
-__device__ __forceinline__ int replica_of(const SysDev<R>& s, int nrep, int i) {
-    int lo = 0, hi = nrep;                 // rep_off[lo] <= i < rep_off[hi]
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (s.rep_off[mid] <= i) lo = mid; else hi = mid;
-    }
-    return lo;
+
+// Acceptance of a move that raises the energy: exp(-dE mu_B/(kB T)) >= u, u = word / 2^32
+// (mc.cpp:93-99).  FP64 in double as the oracle; FP32 on the hardware exp2.
+__device__ __forceinline__ bool mc_accept(double x, uint32_t word) {
+    return exp(-x) >= (double)word * 2.3283064365386963e-10;
+}
+__device__ __forceinline__ bool mc_accept(float x, uint32_t word) {
+    return __expf(-x) >= (float)word * 2.3283064365386963e-10f;
 }
 
+// Workgroup b of a colour takes the list entries [blk.x, blk.z) of that colour, all of replica
+// blk.y (whose first atom is blk.w): the replica is uniform, its conditions and cone width are
+// read once, and the tables are staged in LDS as in the fused LLG kernels.  A thread forms its
+// move from Philox while its own loads are in flight, gathers its neighbours once, and forms the
+// energy before and after the move from the same registers, each in the oracle's order.
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(256)
-k_mc_colour(SysDev<R> s, Spin3<R> S, const int* __restrict__ list, int count, int nrep,
+k_mc_colour(SysDev<R> s, Spin3<R> S, const int* __restrict__ list, const int4* __restrict__ cblk, int ncblk,
             const double* __restrict__ sigma, uint64_t sweep, double* __restrict__ rejects) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    double rej = 0.0;
-    int myrep = -1;
-    if (t < count) {
-        const int i = list[t];
-        const int rep = replica_of(s, nrep, i);
-        myrep = rep;
-        const int local = caller_local(s, rep, i - s.rep_off[rep]);
-        const int m = s.type[i];
-        const Cond<R>& c = s.cond[rep * s.nmat + m];
-        const uint2 key = s.rep_key[rep];
-        const U4 um = philox4x32_10((uint32_t)local, (uint32_t)sweep, (uint32_t)(sweep >> 32),
-                                    STREAM_MC_MOVE, key.x, key.y);
-        R g0, g1, g2;
-        box_muller3(um, g0, g1, g2);
-        const R angle = (R)sigma[rep];
-        const R ox = S.x[i], oy = S.y[i], oz = S.z[i];
-        R nx = ox + g0 * angle, ny = oy + g1 * angle, nz = oz + g2 * angle;
-        const R r = inv_norm(nx, ny, nz);
-        nx *= r; ny *= r; nz *= r;
-        R e_old = (R(0) + exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, ox, oy, oz)
-                   + anisotropy_energy(s, m, ox, oy, oz)) + applied_energy(c, ox, oy, oz);
-        R e_new = (R(0) + exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, nx, ny, nz)
-                   + anisotropy_energy(s, m, nx, ny, nz)) + applied_energy(c, nx, ny, nz);
-        if (s.hd) {                        // last term of sim::calculate_spin_energy (energy.cpp:171)
-            e_old += magnetostatic_energy(s, i, ox, oy, oz);
-            e_new += magnetostatic_energy(s, i, nx, ny, nz);
-        }
-        const R de = (e_new - e_old) * s.mat[m].mu_s_SI * R(1.07828231e23);
-        bool accept = de < R(0);
-        if (!accept) {
-            const U4 ua = philox4x32_10((uint32_t)local, (uint32_t)sweep, (uint32_t)(sweep >> 32),
-                                        STREAM_MC_ACCEPT, key.x, key.y);
-            const double u = (double)ua.v[0] * 2.3283064365386963e-10;   // [0,1)
-            accept = exp((double)(-de * c.kBTBohr)) >= u;
-        }
-        if (accept) {
-            S.x[i] = nx; S.y[i] = ny; S.z[i] = nz;
-        } else {
-            rej = 1.0;
-        }
+    constexpr bool kStaged = Access<AM>::TP != TP_CSR;
+    const int lb = logical_block(s.xcd_remap);
+    if (lb >= ncblk) return;                       // whole workgroup
+    const int4 b = cblk[lb];
+    const int rep = b.y;
+    const int t = b.x + (int)threadIdx.x;
+    const bool ok = t < b.z;
+    const int i = list[ok ? t : b.x];
+    const int m = s.nmat == 1 ? 0 : (int)s.type[i];
+    const R ox = S.x[i], oy = S.y[i], oz = S.z[i];
+    int head = 0;
+    if (kStaged) head = row_head<R, AM>(s, i);
+    const int local = caller_local(s, rep, i - b.w);
+    const uint2 key = s.rep_key[rep];
+    const U4 um = philox4x32_10((uint32_t)local, (uint32_t)sweep, (uint32_t)(sweep >> 32), STREAM_MC_MOVE, key.x,
+                                key.y);
+    R g0, g1, g2;
+    box_muller3(um, g0, g1, g2);
+    const R angle = (R)sigma[rep];
+    stage_fused<R, AM>(s, rep);
+    const MatConst<R>& mc = FusedLds<R>::mat()[m];
+    const Cond<R>& c = FusedLds<R>::cond(s)[m];
+    R nx = ox + g0 * angle, ny = oy + g1 * angle, nz = oz + g2 * angle;
+    const R r = inv_norm(nx, ny, nz);
+    nx *= r; ny *= r; nz *= r;
+    R x_old, x_new;
+    if constexpr (kStaged) {
+        const SpinRsrc<R> rs(s, S.x, S.y, S.z);
+        const UniformJ<R, ET> ju(s, Access<AM>::JM);
+        RowIdx<AM> ri;
+        RowSpins<R, AM> rv;
+        fused_row_index(s, i, head, ri);
+        row_load(rs, ri, rv);
+        x_old = row_exchange_energy<R, ET, AM>(s, ri, rv, ju.v, ox, oy, oz);
+        x_new = row_exchange_energy<R, ET, AM>(s, ri, rv, ju.v, nx, ny, nz);
+    } else {
+        x_old = exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, ox, oy, oz);
+        x_new = exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, nx, ny, nz);
     }
-    // Rejections per replica.  Colour lists run in atom order, so a wavefront's lanes almost always
-    // share one replica: then one lane adds the wavefront's count, instead of one atomic per
-    // rejected move on a handful of contended addresses.  Counts are integers, exact in a double
-    // in any order.
-    const unsigned long long rejected = __ballot(rej != 0.0);
-    if (rejected == 0ull) return;
-    const int first = __ffsll(rejected) - 1;              // a lane that rejected
-    const int rep0 = __shfl(myrep, first);
-    if (__all(rej == 0.0 || myrep == rep0)) {
-        if ((int)__lane_id() == first) atomicAdd(&rejects[rep0], (double)__popcll(rejected));
-    } else if (rej != 0.0) {
-        atomicAdd(&rejects[myrep], rej);
+    // sim::calculate_spin_energy order: exchange, anisotropy, applied, magnetostatic (energy.cpp:147-174)
+    R e_old = (R(0) + x_old + anisotropy_energy(s.ani, mc, ox, oy, oz)) + applied_energy(c, ox, oy, oz);
+    R e_new = (R(0) + x_new + anisotropy_energy(s.ani, mc, nx, ny, nz)) + applied_energy(c, nx, ny, nz);
+    if (s.hd) {
+        e_old += magnetostatic_energy(s, i, ox, oy, oz);
+        e_new += magnetostatic_energy(s, i, nx, ny, nz);
     }
+    const R de = (e_new - e_old) * mc.mu_s_SI * R(1.07828231e23);
+    bool accept = de < R(0);
+    if (!accept) {
+        const U4 ua = philox4x32_10((uint32_t)local, (uint32_t)sweep, (uint32_t)(sweep >> 32), STREAM_MC_ACCEPT,
+                                    key.x, key.y);
+        accept = mc_accept(de * c.kBTBohr, ua.v[0]);
+    }
+    if (ok && accept) {
+        S.x[i] = nx; S.y[i] = ny; S.z[i] = nz;
+    }
+    // rejections of the replica: one atomic per wavefront (counts are integers, exact in a double
+    // in any order)
+    const unsigned long long rejected = __ballot(ok && !accept);
+    if (rejected != 0ull && (int)__lane_id() == __ffsll(rejected) - 1)
+        atomicAdd(&rejects[rep], (double)__popcll(rejected));
 }
 
 }  // namespace vmx

@@ -47,9 +47,10 @@ struct Ops {
     static hipError_t energy_partial(int grid, hipStream_t st, int et, int jm, const SysDev<R>& s, Spin3<R> S,
                                      const int* mask, const double* mm, const int* atom_rep, int nbins,
                                      double* partial);
-    // Monte Carlo: one colour
+    // Monte Carlo: one colour; cblk[ncblk] are the colour's workgroups {first list entry, replica,
+    // end list entry, replica's first atom}, 256 entries at most each
     static hipError_t mc_colour(hipStream_t st, int et, int jm, const SysDev<R>& s, Spin3<R> S, const int* list,
-                                int count, int nrep, const double* sigma, uint64_t sweep, double* rejects);
+                                const int4* cblk, int ncblk, const double* sigma, uint64_t sweep, double* rejects);
     // deterministic fitting step and its adjoint (isotropic uniform exchange); grids of 256
     static hipError_t fit_predictor(hipStream_t st, int am, const SysDev<R>& s, const FitPrm<R>& p, Spin3<R> h,
                                     Spin3<R> S, Spin3<R> P);

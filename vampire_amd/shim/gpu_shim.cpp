@@ -15,14 +15,19 @@
 //  * gpu::llg_heun reads sim::temperature, sim::H_vec * sim::H_applied and the Hamiltonian flags
 //    at every call (src/cuda/external_fields.cu:52-56) and takes one Heun step.
 //  * gpu::stats::get replaces the host statistics with the device ones: reset the averages, then
-//    set_magnetization(current, mean sums, counter) (src/cuda/statistics.cu:252-284).
+//    set_magnetization(current, mean sums, counter) (src/cuda/statistics.cu:252-284); the energy
+//    statistics, which the reference backend leaves to the CPU, go the same way through
+//    set_*_energy and update_mean_counter (src/statistics/energy.cpp:308-390).
+//  * terms the device step does not compute are refused in gpu::initialize rather than left out.
 //  * errors end the run through err::vexit(), as a failed CUDA call does (cuda_utils.hpp:32-43).
 #include <cmath>
+#include <cstdlib>
 #include <iostream>
 #include <string>
 #include <vector>
 
 #include "atoms.hpp"
+#include "create.hpp"
 #include "dipole.hpp"
 #include "montecarlo/internal.hpp"
 #include "errors.hpp"
@@ -34,6 +39,9 @@
 #include "stats.hpp"
 #include "vio.hpp"
 #include "anisotropy/internal.hpp"
+
+#include "simulate/internal.hpp"
+#include "spintorque/internal.hpp"
 
 #include "vmx.h"
 
@@ -47,6 +55,21 @@ struct MagHandle {
     int handle;
 };
 std::vector<MagHandle> g_mags;
+
+struct EnergyHandle {
+    stats::energy_statistic_t* stat;
+    int mask_size;
+    int handle;
+};
+std::vector<EnergyHandle> g_energies;   // energy statistics computed on the device
+
+// Thin-film demagnetising field (sim::ext_demag, src/simulate/fields.cpp:219-239): a uniform
+// field -mu0 D_k m_k / V from the latest system magnetisation statistic, added to the applied
+// field of the step (not to the applied-field energy).  g_sys_m holds the m_l the reference reads
+// with stats::system_magnetization.get_magnetization(); in device-statistics mode it is refreshed
+// from the device at every statistics update.
+double g_sys_m[3] = {0.0, 0.0, 0.0};
+double g_demag_sent[3] = {0.0, 0.0, 0.0};
 
 void check(int rc, const char* what) {
     if (rc != VMX_OK) {
@@ -64,10 +87,38 @@ uint32_t field_flags() {
     return f;
 }
 
+void push_demag() {
+    if (!sim::ext_demag) return;
+    const double mu_0 = -4.0 * M_PI * 1.0e-7 /
+                        (cs::system_dimensions[0] * cs::system_dimensions[1] * cs::system_dimensions[2] * 1.0e-30);
+    const double HD[3] = {mu_0 * sim::demag_factor[0] * g_sys_m[0], mu_0 * sim::demag_factor[1] * g_sys_m[1],
+                          mu_0 * sim::demag_factor[2] * g_sys_m[2]};
+    if (HD[0] == g_demag_sent[0] && HD[1] == g_demag_sent[1] && HD[2] == g_demag_sent[2]) return;
+    check(vmx_set_demag_field(g_ctx, -1, HD), "vmx_set_demag_field");
+    for (int k = 0; k < 3; ++k) g_demag_sent[k] = HD[k];
+}
+
 void push_conditions() {
     const double H[3] = {sim::H_vec[0] * sim::H_applied, sim::H_vec[1] * sim::H_applied,
                          sim::H_vec[2] * sim::H_applied};
+    push_demag();
     check(vmx_set_conditions(g_ctx, -1, sim::temperature, H, field_flags()), "vmx_set_conditions");
+}
+
+// m_l of the thin-film demag field from the host statistic (host-statistics mode, or before
+// the first device update)
+void host_system_m() {
+    if (!sim::ext_demag) return;
+    const std::vector<double> m = ::stats::system_magnetization.get_magnetization();
+    for (int k = 0; k < 3; ++k) g_sys_m[k] = m.size() > 2 ? m[k] : 0.0;
+}
+
+void refuse(bool cond, const char* what) {
+    if (!cond) return;
+    std::cerr << "Error: the GPU backend does not compute " << what << "; run this input with "
+              << "gpu:acceleration=off" << std::endl;
+    zlog << zTs() << "Error: the GPU backend does not compute " << what << std::endl;
+    err::vexit();
 }
 
 void add_magnetization(stats::magnetization_statistic_t& st, bool enabled) {
@@ -79,6 +130,38 @@ void add_magnetization(stats::magnetization_statistic_t& st, bool enabled) {
     int h = -1;
     check(vmx_stats_add_magnetization(g_ctx, mask_size, mask.data(), &h), "vmx_stats_add_magnetization");
     g_mags.push_back(MagHandle{&st, mask_size, h});
+}
+
+void add_energy(stats::energy_statistic_t& st, bool enabled) {
+    if (!enabled || !st.is_initialized()) return;
+    std::vector<int> mask;
+    std::vector<double> normalisation;
+    st.get_mask(mask, normalisation);
+    const int mask_size = (int)normalisation.size();
+    int h = -1;
+    check(vmx_stats_add_energy(g_ctx, mask_size, mask.data(), &h), "vmx_stats_add_energy");
+    g_energies.push_back(EnergyHandle{&st, mask_size, h});
+}
+
+// Host-derived statistics, from the host magnetisation and energy objects as the CPU path forms
+// them (src/statistics/statistics.cpp:45-55)
+void derived_statistics() {
+    if (::stats::calculate_system_specific_heat)
+        ::stats::system_specific_heat.calculate(::stats::system_energy.get_total_energy());
+    if (::stats::calculate_material_specific_heat)
+        ::stats::material_specific_heat.calculate(::stats::material_energy.get_total_energy());
+    if (::stats::calculate_material_standard_deviation)
+        ::stats::material_standard_deviation.update(::stats::system_magnetization.get_magnetization());
+    if (::stats::calculate_system_susceptibility)
+        ::stats::system_susceptibility.calculate(::stats::system_magnetization.get_magnetization());
+    if (::stats::calculate_material_susceptibility)
+        ::stats::material_susceptibility.calculate(::stats::material_magnetization.get_magnetization());
+}
+
+bool need_derived() {
+    return ::stats::calculate_system_specific_heat || ::stats::calculate_material_specific_heat ||
+           ::stats::calculate_material_standard_deviation || ::stats::calculate_system_susceptibility ||
+           ::stats::calculate_material_susceptibility;
 }
 
 }  // namespace
@@ -140,6 +223,15 @@ void initialize() {
                      "and bilinear exchange only" << std::endl;
         err::vexit();
     }
+    // terms of calculate_spin_fields / calculate_external_fields (src/simulate/fields.cpp:111-114,
+    // 137-166, 264) that the device step leaves out: a run with them would silently differ
+    refuse(sim::lagrange_multiplier, "the Lagrange-multiplier field (sim:constraint-angle / lagrange programs)");
+    refuse(sim::internal::enable_spin_torque_fields, "spin-torque fields");
+    refuse(st::internal::enabled, "the spin-transfer-torque field (spin-torque module)");
+    refuse(sim::enable_fmr, "the FMR field");
+    refuse(sim::program == 7, "the HAMR fields (sim:program=hamr-simulation)");
+    refuse(sim::program == 13, "localised thermal fields (sim:program=localised-temperature-pulse)");
+    refuse(sim::local_temperature, "material-specific temperatures (material:temperature)");
     vmx_system s{};
     s.num_atoms = n;
     s.num_materials = nmat;
@@ -159,7 +251,19 @@ void initialize() {
     s.replica_offset = nullptr;
 
     vmx_options o{};
+    // Arithmetic of the device step.  The reference backend picks it when it is compiled
+    // (CUDA_DP / CUDA_SP / CUDA_MP, src/cuda/makefile:36-38); the drop-in reads it at run time from
+    // VMX_PRECISION: double (default) or single / mixed (spins and fields in float, statistics and
+    // energies accumulated in double).
     o.precision = VMX_FP64;
+    if (const char* p = std::getenv("VMX_PRECISION")) {
+        const std::string v(p);
+        if (v == "single" || v == "mixed" || v == "fp32") o.precision = VMX_FP32;
+        else if (v != "double" && v != "fp64") {
+            std::cerr << "Error: VMX_PRECISION must be double, single or mixed (got " << v << ")" << std::endl;
+            err::vexit();
+        }
+    }
     o.kernel = VMX_KERNEL_FUSED;
     // gpu:num-threads becomes the workgroup size when it is one the kernels accept
     o.block_size = (gpu::num_threads >= 64 && gpu::num_threads <= 512 && gpu::num_threads % 64 == 0)
@@ -179,18 +283,41 @@ void initialize() {
     check(vmx_set_sweep(g_ctx, (int64_t)sim::time), "vmx_set_sweep");
     check(vmx_set_spins(g_ctx, atoms::x_spin_array.data(), atoms::y_spin_array.data(), atoms::z_spin_array.data()),
           "vmx_set_spins");
+    // VMX_AUTOTUNE=1: choose the workgroup size and XCD mapping by timing a few steps of this
+    // system (spins and step counter are restored afterwards)
+    if (const char* a = std::getenv("VMX_AUTOTUNE")) {
+        if (std::string(a) == "1") {
+            int chosen = 0;
+            check(vmx_autotune(g_ctx, 10, &chosen), "vmx_autotune");
+            zlog << zTs() << "GPU backend autotuned: " << chosen << " threads per workgroup" << std::endl;
+        }
+    }
+    if (sim::ext_demag && !::stats::system_magnetization.is_initialized()) {
+        std::cerr << "Error: the thin-film demagnetising field needs the system magnetisation statistic" << std::endl;
+        err::vexit();
+    }
+    g_demag_sent[0] = g_demag_sent[1] = g_demag_sent[2] = 0.0;
+    host_system_m();
     push_conditions();
     // dipole::initialize has computed the time-0 field on the host already (sim.cpp:270-295)
     transfer_dipole_fields_from_cpu_to_gpu();
 
     // statistics computed on the device unless the input asks for CPU statistics
     g_mags.clear();
+    g_energies.clear();
     if (!gpu::cpu_stats) {
         add_magnetization(::stats::system_magnetization, ::stats::calculate_system_magnetization);
         add_magnetization(::stats::material_magnetization, ::stats::calculate_material_magnetization);
         add_magnetization(::stats::height_magnetization, ::stats::calculate_height_magnetization);
         add_magnetization(::stats::material_height_magnetization,
                           ::stats::calculate_material_height_magnetization);
+        // energies on the device too, unless the host dipole solver's field is in H_ext: that field
+        // carries no demag field for the magnetostatic energy, so those runs form energies on the
+        // host from synchronised spins
+        if (!dipole::activated) {
+            add_energy(::stats::system_energy, ::stats::calculate_system_energy);
+            add_energy(::stats::material_energy, ::stats::calculate_material_energy);
+        }
     }
     gpu::initialized = true;
     zlog << zTs() << "GPU backend initialised: " << vmx_version() << ", " << n << " atoms" << std::endl;
@@ -255,10 +382,10 @@ void synchronise() { transfer_spin_positions_from_gpu_to_cpu(); }
 
 namespace stats {
 
-// Energy statistics are computed on the host from synchronised spins: energy_statistic_t takes
-// device values only without their update counter (hdr/stats.hpp:108-112), so its means could not
-// be formed from device sums.
+// Energies on the host from synchronised spins: only when the host dipole solver's field is in
+// H_ext (device energies refuse it, see gpu::initialize) or with host statistics.
 void host_energies() {
+    if (!g_energies.empty()) return;
     if (!::stats::calculate_system_energy && !::stats::calculate_material_energy) return;
     if (!gpu::cpu_stats) transfer_spin_positions_from_gpu_to_cpu();
     if (::stats::calculate_system_energy)
@@ -269,27 +396,11 @@ void host_energies() {
                                            atoms::m_spin_array, atoms::type_array, sim::temperature);
 }
 
-void update() {
-    if (!g_ctx) return;
-    if (gpu::cpu_stats) {
-        // statistics on the host from freshly copied spins (src/cuda/statistics.cu:34-48)
-        transfer_spin_positions_from_gpu_to_cpu();
-        if (::stats::calculate_system_magnetization)
-            ::stats::system_magnetization.calculate_magnetization(atoms::x_spin_array, atoms::y_spin_array,
-                                                                  atoms::z_spin_array, atoms::m_spin_array);
-        if (::stats::calculate_material_magnetization)
-            ::stats::material_magnetization.calculate_magnetization(atoms::x_spin_array, atoms::y_spin_array,
-                                                                    atoms::z_spin_array, atoms::m_spin_array);
-        host_energies();
-        return;
-    }
-    push_conditions();
-    check(vmx_stats_update(g_ctx), "vmx_stats_update");
-    host_energies();
-}
-
-void get() {
-    if (!g_ctx || gpu::cpu_stats) return;
+// The host statistic objects take the device's latest values and running sums: for each
+// magnetisation reset_magnetization_averages then set_magnetization(current, sums, counter)
+// (src/cuda/statistics.cu:252-284); for each energy reset_averages, set_*_energy(current, sums)
+// for the five terms and update_mean_counter(counter) (src/statistics/energy.cpp:308-390).
+void sync_from_device() {
     for (const MagHandle& h : g_mags) {
         std::vector<double> cur(4 * h.mask_size), mean(4 * h.mask_size);
         int64_t counter = 0;
@@ -298,11 +409,73 @@ void get() {
         h.stat->reset_magnetization_averages();
         h.stat->set_magnetization(cur, mean, (long)counter);
     }
+    for (const EnergyHandle& h : g_energies) {
+        const int ms = h.mask_size;
+        std::vector<double> cur(5 * (size_t)ms), mean(5 * (size_t)ms);
+        int64_t counter = 0;
+        check(vmx_stats_get_energy(g_ctx, h.handle, cur.data(), mean.data(), &counter), "vmx_stats_get_energy");
+        std::vector<double> c[5], m[5];
+        for (int t = 0; t < 5; ++t) {
+            c[t].assign(cur.begin() + (size_t)t * ms, cur.begin() + (size_t)(t + 1) * ms);
+            m[t].assign(mean.begin() + (size_t)t * ms, mean.begin() + (size_t)(t + 1) * ms);
+        }
+        h.stat->reset_averages();
+        h.stat->set_total_energy(c[0], m[0]);
+        h.stat->set_exchange_energy(c[1], m[1]);
+        h.stat->set_anisotropy_energy(c[2], m[2]);
+        h.stat->set_applied_field_energy(c[3], m[3]);
+        h.stat->set_magnetostatic_energy(c[4], m[4]);
+        h.stat->update_mean_counter((long)counter);
+    }
+}
+
+// stats::update under acceleration (src/statistics/statistics.cpp:35-37).  Host statistics: the
+// four magnetisation statistics and the energies from freshly copied spins, as
+// src/cuda/statistics.cu:34-48 does with cpu_stats.  Device statistics: one device update.  Either
+// way the statistics the CPU path derives from them (specific heat, susceptibility, standard
+// deviation; statistics.cpp:45-55) follow, and the thin-film demag field takes the new system
+// magnetisation.
+void update() {
+    if (!g_ctx) return;
+    if (gpu::cpu_stats) {
+        transfer_spin_positions_from_gpu_to_cpu();
+        host_energies();
+        const auto& X = atoms::x_spin_array;
+        const auto& Y = atoms::y_spin_array;
+        const auto& Z = atoms::z_spin_array;
+        const auto& M = atoms::m_spin_array;
+        if (::stats::calculate_system_magnetization)
+            ::stats::system_magnetization.calculate_magnetization(X, Y, Z, M);
+        if (::stats::calculate_material_magnetization)
+            ::stats::material_magnetization.calculate_magnetization(X, Y, Z, M);
+        if (::stats::calculate_height_magnetization)
+            ::stats::height_magnetization.calculate_magnetization(X, Y, Z, M);
+        if (::stats::calculate_material_height_magnetization)
+            ::stats::material_height_magnetization.calculate_magnetization(X, Y, Z, M);
+    } else {
+        push_conditions();
+        check(vmx_stats_update(g_ctx), "vmx_stats_update");
+        host_energies();
+        if (need_derived() || sim::ext_demag) sync_from_device();
+    }
+    derived_statistics();
+    host_system_m();
+}
+
+// Before output (src/vio/datalog.cpp:281).  With device statistics the host objects take the
+// device values.  A continuous checkpoint written in this output step (datalog.cpp:487) reads
+// the host spin arrays, so they are synchronised first.
+void get() {
+    if (!g_ctx) return;
+    if (!gpu::cpu_stats) sync_from_device();
+    if (sim::save_checkpoint_flag && sim::save_checkpoint_continuous_flag &&
+        sim::time % sim::save_checkpoint_rate == 0)
+        transfer_spin_positions_from_gpu_to_cpu();
 }
 
 // With host statistics the averages live in the host classes, so they are the ones cleared
-// (src/cuda/statistics.cu:115-121). The host energy averages are cleared in both modes, as the CPU
-// path does (src/statistics/statistics.cpp:74-77), since this backend accumulates them on the host.
+// (src/cuda/statistics.cu:115-121); with device statistics the device sums are.  Host energies
+// and the derived statistics are cleared as the CPU path clears them (statistics.cpp:74-93).
 void reset() {
     if (!g_ctx) return;
     if (gpu::cpu_stats) {
@@ -316,6 +489,11 @@ void reset() {
     }
     if (::stats::calculate_system_energy) ::stats::system_energy.reset_averages();
     if (::stats::calculate_material_energy) ::stats::material_energy.reset_averages();
+    if (::stats::calculate_material_standard_deviation) ::stats::material_standard_deviation.reset_averages();
+    if (::stats::calculate_system_specific_heat) ::stats::system_specific_heat.reset_averages();
+    if (::stats::calculate_material_specific_heat) ::stats::material_specific_heat.reset_averages();
+    if (::stats::calculate_system_susceptibility) ::stats::system_susceptibility.reset_averages();
+    if (::stats::calculate_material_susceptibility) ::stats::material_susceptibility.reset_averages();
 }
 
 }  // namespace stats
