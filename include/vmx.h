@@ -264,6 +264,16 @@ int vmx_stats_add_energy(vmx_ctx* ctx, int32_t mask_size, const int32_t* mask,
  * running means (stats::update, src/statistics/statistics.cpp:26-63). Accumulation is FP64. */
 int vmx_stats_update(vmx_ctx* ctx);
 
+/* vmx_stats_update without any allocation: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (HIP graph capture, serving loops, harnesses that forbid it).
+ * Same kernels and results; the per-workgroup partial sums go to a fixed 4 MiB device array of
+ * the library (one per GPU, kept until the process ends) instead of a buffer the library grows.
+ * A registered statistic of mask_size > 16 bins needs min(ceil(num_atoms/256), 512) x mask_size
+ * x 32 bytes of partials; beyond 4 MiB (only masks of more than 256 bins reach it) the call is
+ * refused with VMX_ERR_ARG before any work is queued.  Calls on one GPU share the
+ * array: calls that overlap on one GPU (several host threads or streams) race. */
+int vmx_stats_update_noalloc(vmx_ctx* ctx);
+
 /* Latest magnetisation [4*mask_size] (m_x,m_y,m_z unit vector, |M|/Ms per mask id: CPU
  * semantics of magnetization.cpp:157-168), running sum [4*mask_size] and update counter
  * (what gpu::stats::get hands to set_magnetization, magnetization.cpp:194-208). */

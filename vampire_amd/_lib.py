@@ -149,6 +149,7 @@ SIGNATURES = [
     ("vmx_stats_add_magnetization", C.c_int, [_VP, C.c_int32, _P(C.c_int32), _P(C.c_int32)]),
     ("vmx_stats_add_energy", C.c_int, [_VP, C.c_int32, _P(C.c_int32), _P(C.c_int32)]),
     ("vmx_stats_update", C.c_int, [_VP]),
+    ("vmx_stats_update_noalloc", C.c_int, [_VP]),
     ("vmx_stats_get_magnetization", C.c_int, [_VP, C.c_int32, _DP, _DP, _P(C.c_int64)]),
     ("vmx_stats_get_energy", C.c_int, [_VP, C.c_int32, _DP, _DP, _P(C.c_int64)]),
     ("vmx_stats_reset", C.c_int, [_VP]),

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -104,6 +105,34 @@ struct PinnedRing {
         return VMX_OK;
     }
 };
+
+// ---- scratch of the statistics update that allocates nothing -----------------------------------
+// vmx_stats_update_noalloc keeps its per-workgroup partial sums in this file-scope device array
+// instead of a buffer the library allocates.  Every GPU the library runs on holds its own copy
+// until the process ends; calls on one GPU share it (the last call's partials stay in it), so
+// calls that overlap on one GPU race.  Host code never passes the array's name to a kernel: its
+// address on a device comes from hipGetSymbolAddress, taken once per device when the first
+// context is created there (scratch_on_current_device).
+constexpr size_t kStatsScratchBytes = (size_t)4 << 20;
+static __device__ __align__(256) double scratch[kStatsScratchBytes / sizeof(double)];
+
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;   // device ordinal -> that device's copy of `scratch`
+
+int scratch_on_current_device(double** out) {
+    int dev = 0;
+    VMX_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        VMX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(VMX_ERR_HIP, "hipGetSymbolAddress returned no address for the statistics scratch");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return VMX_OK;
+}
 
 struct MagStat {
     int nbins;
@@ -721,6 +750,7 @@ struct vmx_ctx {
     int stats_update(double* fixed = nullptr);
     int stats_grid(int nbins) const;
     size_t stats_partial_bytes() const;
+    double* stats_scratch = nullptr;   // this device's copy of `scratch` (no-allocation update)
     template <typename R>
     int mc_sweeps(int nsweeps);
     int colour();
@@ -1569,6 +1599,7 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
                       // their atom ranges in the internal order
         VMX_TRY(upload(c->atom_rep, atom_rep_h));
     }
+    VMX_TRY(scratch_on_current_device(&c->stats_scratch));
     {
         std::vector<int> rep_m(nrep);
         for (int r = 0; r < nrep; ++r) rep_m[r] = (roff[r + 1] - roff[r]) / q;
@@ -1983,6 +2014,22 @@ int vmx_stats_update(vmx_ctx* c) {
                                    "use vmx_set_dipole or clear it");
     VMX_TRY(c->use_device());
     return c->precision == VMX_FP64 ? c->stats_update<double>() : c->stats_update<float>();
+}
+
+int vmx_stats_update_noalloc(vmx_ctx* c) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    if (c->hdip_on && !c->dip.on && !c->energies.empty())
+        return fail(VMX_ERR_STATE, "an uploaded dipole field has no demag field for the magnetostatic energy; "
+                                   "use vmx_set_dipole or clear it");
+    const size_t need = c->stats_partial_bytes();
+    if (need > kStatsScratchBytes)
+        return fail(VMX_ERR_ARG, "the registered statistics need %zu bytes of workgroup partials; the "
+                                 "no-allocation update holds at most %zu (fewer mask bins, or vmx_stats_update)",
+                    need, kStatsScratchBytes);
+    if (!c->stats_scratch) return fail(VMX_ERR_STATE, "no statistics scratch on this context's device");
+    VMX_TRY(c->use_device());
+    return c->precision == VMX_FP64 ? c->stats_update<double>(c->stats_scratch)
+                                    : c->stats_update<float>(c->stats_scratch);
 }
 
 int vmx_stats_get_magnetization(vmx_ctx* c, int32_t handle, double* magnetization, double* mean_sum,
