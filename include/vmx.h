@@ -249,6 +249,39 @@ int vmx_mc_get_state(vmx_ctx* ctx, double* sigma, double* moves, double* rejects
 int vmx_mc_set_sigma(vmx_ctx* ctx, int32_t replica, double sigma);
 int vmx_mc_num_colours(vmx_ctx* ctx, int32_t* ncolours);
 
+/* ---- constrained Monte Carlo (montecarlo::cmc_step, src/montecarlo/cmc.cpp:282-497) ----------
+ * Pair moves that keep the magnetisation along a constraint direction (integrator 3 of
+ * sim::integrate_serial, sim.cpp:623-628): spin 1 takes the Monte Carlo move, spin 2 cancels the
+ * change of the transverse moment in the constraint frame, and the pair is accepted with
+ * exp(-dE/kT) (Mz'/Mz)^2 |s2z/s2z'| when Mz' > 0.  The reference picks both sites at random; the
+ * device pairs atoms of one graph colour (which do not interact) through a random permutation per
+ * round, two rounds per colour, so a sweep makes N pair trials as the reference's does, and the
+ * acceptance uses the magnetisation at the start of the round.  The reference initialises the
+ * spins along the constraint (CMCinit, cmc.cpp:204-278): the caller sets them (vmx_set_spins). */
+enum {
+    VMX_CMC_MOVE_CONE = 0,    /* Gaussian cone of the replica's Metropolis width (vmx_mc_set_sigma,
+                                 60 by default): montecarlo's adaptive move, whose width cmc_step
+                                 never adapts (mc_moves.cpp:158-162)                            */
+    VMX_CMC_MOVE_ANGLE = 1    /* the angle move with width (kB T/mu_B)^0.2 0.08 per material, 0.02
+                                 below 1 K (cmc.cpp:320-325)                                     */
+};
+
+/* Constraint direction: polar angle phi and azimuth theta in degrees (sim::constraint_phi,
+ * sim::constraint_theta), the frame of cmc::polar_rot_matrix (cmc.cpp:43-138).  replica = -1
+ * sets every replica. */
+int vmx_cmc_set_constraint(vmx_ctx* ctx, int32_t replica, double phi, double theta);
+
+/* VMX_CMC_MOVE_CONE (default) or VMX_CMC_MOVE_ANGLE. */
+int vmx_cmc_set_move(vmx_ctx* ctx, int32_t move);
+
+/* nsweeps constrained sweeps at the conditions of vmx_set_conditions; advances the sweep counter
+ * (the Philox counter shared with vmx_mc_sweep).  Refused while a dipole field is set. */
+int vmx_cmc_sweep(vmx_ctx* ctx, int32_t nsweeps);
+
+/* Outcome totals per replica, [num_replicas][3]: accepted pairs, energy rejections, sphere
+ * rejections (montecarlo::cmc::mc_success, energy_reject, sphere_reject). */
+int vmx_cmc_get_counts(vmx_ctx* ctx, double* counts);
+
 /* ---- statistics (src/statistics, gpu::stats) ------------------------------------------ */
 /* Registers a magnetisation statistic with mask[num_atoms] in [0, mask_size)
  * (magnetization_statistic_t::set_mask, src/statistics/magnetization.cpp:53-113); the last

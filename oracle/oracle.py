@@ -52,6 +52,11 @@ class or_cond(C.Structure):
                 ("half_dt", C.c_double), ("Hd", C.c_double * 3)]
 
 
+class or_cmc(C.Structure):
+    _fields_ = [("P", C.c_double * 9), ("Pt", C.c_double * 9), ("v", C.c_double * 3), ("move", C.c_int),
+                ("sigma", C.c_double)]
+
+
 _lib = None
 
 
@@ -81,6 +86,11 @@ def load():
         lib.or_mc_random_site.argtypes = [_P(or_sys), _P(or_cond), _DP, _DP, _DP, C.c_int, C.c_uint64,
                                           C.c_uint64, _DP, _DP, _DP]
         lib.or_dipole_update.argtypes = [_P(or_sys), _DP, _DP, _DP]
+        lib.or_cmc_frame.argtypes = [C.c_double, C.c_double, _DP, _DP, _DP]
+        lib.or_cmc_random_pairs.argtypes = [_P(or_sys), _P(or_cond), _P(or_cmc), _DP, _DP, _DP, C.c_int,
+                                            C.c_uint64, C.c_uint64, _DP]
+        lib.or_cmc_checkerboard.argtypes = [_P(or_sys), _P(or_cond), _P(or_cmc), _DP, _DP, _DP, _IP, _IP, C.c_int,
+                                            C.c_int, C.c_uint64, C.c_uint64, _DP]
         lib.or_greedy_colour.argtypes = [_P(or_sys), _IP]
         lib.or_greedy_colour.restype = C.c_int
         _lib = lib
@@ -258,6 +268,33 @@ class Oracle:
                                     _i(start), len(start) - 1, int(nsweeps), seed, sweep0, C.byref(sg),
                                     C.byref(mv), C.byref(rj))
         return np.stack(sp), sg.value, mv.value, rj.value
+
+    def cmc(self, phi: float, theta: float, move: int = 0, sigma: float = 60.0):
+        """Constraint of constrained Monte Carlo: polar angle phi and azimuth theta (degrees) of the
+        magnetisation (sim::constraint_phi, constraint_theta); move 0 = cone of width sigma, 1 = the
+        temperature-scaled angle move."""
+        k = or_cmc()
+        self.lib.or_cmc_frame(float(phi), float(theta), k.P, k.Pt, k.v)
+        k.move = int(move)
+        k.sigma = float(sigma)
+        return k
+
+    def cmc_random_pairs(self, spins, nsweeps, cmc, seed=0, sweep0=0):
+        """montecarlo::cmc_step sweeps; returns (spins, [accepted, energy rejects, sphere rejects])."""
+        sp = [np.ascontiguousarray(spins[k], dtype=np.float64).copy() for k in range(3)]
+        counts = np.zeros(3)
+        self.lib.or_cmc_random_pairs(C.byref(self.sys), C.byref(self.cond), C.byref(cmc), *[_d(a) for a in sp],
+                                     int(nsweeps), seed, sweep0, _d(counts))
+        return np.stack(sp), counts
+
+    def cmc_checkerboard(self, spins, nsweeps, cmc, seed=0, sweep0=0):
+        """The colour-paired constrained sweeps of the device; returns (spins, counts)."""
+        sp = [np.ascontiguousarray(spins[k], dtype=np.float64).copy() for k in range(3)]
+        _, lst, start = self.colouring()
+        counts = np.zeros(3)
+        self.lib.or_cmc_checkerboard(C.byref(self.sys), C.byref(self.cond), C.byref(cmc), *[_d(a) for a in sp],
+                                     _i(lst), _i(start), len(start) - 1, int(nsweeps), seed, sweep0, _d(counts))
+        return np.stack(sp), counts
 
     def mc_random_site(self, spins, nsweeps, seed=0, sweep0=0, sigma=60.0):
         sp = [np.ascontiguousarray(spins[k], dtype=np.float64).copy() for k in range(3)]

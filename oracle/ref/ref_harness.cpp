@@ -8,7 +8,12 @@
 //                                                   every step, so the thin-film demag field follows the
 //                                                   spins, and that field written per step (demag.bin)
 //   ref_harness <rundir> mc <nsweeps> <T> <every>   montecarlo::mc_step, reference RNG
+//   ref_harness <rundir> cmc <nsweeps> <T> <every> <phi> <theta>
+//                                                   montecarlo::cmc_step (constrained), reference RNG
 //   ref_harness <rundir> dipole                     macrocell dipole set-up and fields (dipole.bin)
+//   ref_harness_vmx <rundir> gpucmc <nsweeps> <T> <every> <phi> <theta>
+//                                                   the cmc command through the gpu:: drop-in
+//                                                   (built with -DVMX_SHIM, linked with the shim)
 //
 // <rundir> holds `input` (and the material file it names).  Inputs read from <rundir>:
 //   spins_in.bin  optional: 3*N doubles (x block, y block, z block) replacing the initial spins
@@ -37,6 +42,12 @@
 #include "stats.hpp"
 #include "vio.hpp"
 #include "vmpi.hpp"
+#ifdef VMX_SHIM
+#include "gpu.hpp"
+namespace gpu {
+void cmc_step();   // defined by the drop-in shim; the reference's gpu.hpp declares no such call
+}
+#endif
 
 // fields.cpp entry points (declared there, not in a header)
 int calculate_spin_fields(const int, const int);
@@ -316,6 +327,68 @@ int main(int argc, char** argv) {
         }
         return 0;
     }
+    if (cmd == "cmc") {
+        // constrained Monte Carlo at constraint (phi, theta): the first cmc_step runs CMCinit, which
+        // sets every spin along the constraint (src/montecarlo/cmc.cpp:204-278); every `every`
+        // sweeps the system magnetisation (4 values) and the total energy per atom are recorded
+        const int nsweeps = std::atoi(argv[3]);
+        sim::temperature = std::atof(argv[4]);
+        const int every = std::atoi(argv[5]);
+        sim::constraint_phi = std::atof(argv[6]);
+        sim::constraint_theta = std::atof(argv[7]);
+        std::ofstream cf("cmc.bin", std::ios::binary);
+        for (int s = 1; s <= nsweeps; ++s) {
+            montecarlo::cmc_step();
+            if (s % every == 0) {
+                stats::system_magnetization.calculate_magnetization(atoms::x_spin_array, atoms::y_spin_array,
+                                                                    atoms::z_spin_array, atoms::m_spin_array);
+                std::vector<double> row = stats::system_magnetization.get_magnetization();
+                row.resize(4);
+                stats::system_energy.calculate(atoms::x_spin_array, atoms::y_spin_array, atoms::z_spin_array,
+                                               atoms::m_spin_array, atoms::type_array, sim::temperature);
+                row.push_back(stats::system_energy.get_total_energy()[0]);
+                put(cf, row);
+            }
+        }
+        const std::vector<double> counts = {montecarlo::cmc::mc_success, montecarlo::cmc::energy_reject,
+                                            montecarlo::cmc::sphere_reject, montecarlo::cmc::mc_total};
+        put(cf, counts);
+        return 0;
+    }
+#ifdef VMX_SHIM
+    if (cmd == "gpucmc") {
+        // the cmc command through the drop-in boundary: gpu::initialize, then gpu::cmc_step per
+        // sweep (the device runs the sweep; its first call runs CMCinit on the host spins), the
+        // host spin arrays refreshed by gpu::transfer_spin_positions_from_gpu_to_cpu before each
+        // sample; gpu::finalize brings the montecarlo::cmc counters up to date
+        const int nsweeps = std::atoi(argv[3]);
+        sim::temperature = std::atof(argv[4]);
+        const int every = std::atoi(argv[5]);
+        sim::constraint_phi = std::atof(argv[6]);
+        sim::constraint_theta = std::atof(argv[7]);
+        gpu::initialize();
+        std::ofstream cf("cmc.bin", std::ios::binary);
+        for (int s = 1; s <= nsweeps; ++s) {
+            gpu::cmc_step();
+            if (s % every == 0) {
+                gpu::transfer_spin_positions_from_gpu_to_cpu();
+                stats::system_magnetization.calculate_magnetization(atoms::x_spin_array, atoms::y_spin_array,
+                                                                    atoms::z_spin_array, atoms::m_spin_array);
+                std::vector<double> row = stats::system_magnetization.get_magnetization();
+                row.resize(4);
+                stats::system_energy.calculate(atoms::x_spin_array, atoms::y_spin_array, atoms::z_spin_array,
+                                               atoms::m_spin_array, atoms::type_array, sim::temperature);
+                row.push_back(stats::system_energy.get_total_energy()[0]);
+                put(cf, row);
+            }
+        }
+        gpu::finalize();
+        const std::vector<double> counts = {montecarlo::cmc::mc_success, montecarlo::cmc::energy_reject,
+                                            montecarlo::cmc::sphere_reject, montecarlo::cmc::mc_total};
+        put(cf, counts);
+        return 0;
+    }
+#endif
     if (cmd == "dipole") {
         if (!dipole::activated) {
             std::cerr << "the input does not activate the dipole field" << std::endl;

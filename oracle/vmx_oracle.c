@@ -30,6 +30,10 @@
  *   or_mc_random_site     montecarlo::mc_step              src/montecarlo/mc.cpp:29-127,
  *                                                          mc_moves.cpp:71-87,158-162
  *   or_mc_checkerboard    the colour-ordered variant of mc_step run by the device
+ *   or_cmc_random_pairs   montecarlo::cmc_step (constrained Monte Carlo)
+ *                                                          src/montecarlo/cmc.cpp:282-497,
+ *                                                          cmc::polar_rot_matrix cmc.cpp:43-138
+ *   or_cmc_checkerboard   the colour-paired variant of cmc_step run by the device
  *   or_dipole_update      cells::mag                       src/cells/mag.cpp:35-77
  *                         dipole::internal::update_field   src/dipole/update.cpp:37-105 (tensor solver)
  *                         atom fields from cell fields     src/dipole/field.cpp:106-126
@@ -786,6 +790,201 @@ void or_mc_random_site(const or_sys* s, const or_cond* c, double* sx, double* sy
         *tot_rej += rej;
         adapt(sigma, (double)nmoves, rej);
         dipole_tick(s, sx, sy, sz);
+    }
+}
+
+/* ---- constrained Monte Carlo (montecarlo::cmc_step, src/montecarlo/cmc.cpp:282-497) ----------
+ * Pair moves that keep the magnetisation along a constraint direction: spin 1 takes a Monte Carlo
+ * move, spin 2 compensates the transverse magnetisation in the constraint frame, and the pair is
+ * accepted with exp(-dE) (Mz'/Mz)^2 |s2z / s2z'| if Mz' > 0 (Asselin et al., PRB 82, 054415). */
+typedef struct {
+    double P[9], Pt[9], v[3];   /* cmc::polar_matrix, its transpose, cmc::polar_vector */
+    int move;                   /* 0: cone of width sigma (montecarlo's adaptive move, whose width
+                                   cmc_step never adapts); 1: the angle move with the width of
+                                   cmc.cpp:320-325 per material */
+    double sigma;
+} or_cmc;
+
+/* cmc::polar_rot_matrix (cmc.cpp:43-138): P = Y(phi) Z(theta) with vmath::matmul's sums from 0 in
+ * k order, Pt its transpose, v = (0,0,1) P.  phi, theta in degrees. */
+void or_cmc_frame(double phi, double theta, double P[9], double Pt[9], double v[3]) {
+    const double pi = 3.14159265358979323846264338327;
+    const double dy = (phi / 360.0) * 2.0 * pi, dz = (theta / 360.0) * 2.0 * pi;
+    const double sin_y = sin(dy), cos_y = cos(dy), sin_z = sin(dz), cos_z = cos(dz);
+    const double Y[9] = {cos_y, 0.0, -sin_y, 0.0, 1.0, 0.0, sin_y, 0.0, cos_y};
+    const double Z[9] = {cos_z, sin_z, 0.0, -sin_z, cos_z, 0.0, 0.0, 0.0, 1.0};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double c = 0.0;
+            for (int k = 0; k < 3; ++k) c += Y[3 * i + k] * Z[3 * k + j];
+            P[3 * i + j] = c;
+        }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Pt[3 * i + j] = P[3 * j + i];
+    const double ref[3] = {0.0, 0.0, 1.0};
+    for (int j = 0; j < 3; ++j) {
+        double c = 0.0;
+        for (int k = 0; k < 3; ++k) c += ref[k] * P[3 * k + j];
+        v[j] = c;
+    }
+}
+
+/* width of the angle move per material (cmc.cpp:320-325) */
+static double cmc_angle_width(const or_material* m, double T) {
+    const double alpha = m->temperature_rescaling_alpha, Tc = m->temperature_rescaling_Tc;
+    const double rescaled = T < Tc ? Tc * pow(T / Tc, alpha) : T;
+    const double kbt = 9.27400915e-24 / (rescaled * 1.3806503e-23);
+    return rescaled < 1.0 ? 0.02 : pow(1.0 / kbt, 0.2) * 0.08;
+}
+
+static void rotate3(const double* M, const double* x, double* out) {   /* cmc.cpp:363-365 order */
+    out[0] = M[0] * x[0] + M[1] * x[1] + M[2] * x[2];
+    out[1] = M[3] * x[0] + M[4] * x[1] + M[5] * x[2];
+    out[2] = M[6] * x[0] + M[7] * x[1] + M[8] * x[2];
+}
+
+/* One pair trial of cmc_step (cmc.cpp:355-494) with move normals g and acceptance uniform u.
+ * M is the magnetisation the acceptance uses; with update_M it takes the accepted change, as
+ * M_other does in the reference.  Returns 0 accepted, 1 energy reject, 2 sphere reject. */
+static int cmc_pair(const or_sys* s, const or_cond* c, const or_cmc* k, double* sx, double* sy, double* sz,
+                    int a1, int a2, const double g[3], double u, double M[3], int update_M) {
+    const int imat1 = s->type[a1];
+    const double mu1 = s->moment[a1];
+    const double delta = k->move == 1 ? cmc_angle_width(&s->mat[imat1], c->temperature) : k->sigma;
+    const double s1i[3] = {sx[a1], sy[a1], sz[a1]};
+    double s1i_mvd[3], s1f_mvd[3], s2i_mvd[3], s2f_mvd[3];
+    rotate3(k->P, s1i, s1i_mvd);
+    double s1f[3] = {s1i[0] + g[0] * delta, s1i[1] + g[1] * delta, s1i[2] + g[2] * delta};   /* mc_angle */
+    const double r = 1.0 / sqrt(s1f[0] * s1f[0] + s1f[1] * s1f[1] + s1f[2] * s1f[2]);
+    s1f[0] *= r; s1f[1] *= r; s1f[2] *= r;
+    rotate3(k->P, s1f, s1f_mvd);
+    double Eold = or_spin_energy(s, c, sx, sy, sz, a1, sx[a1], sy[a1], sz[a1]);
+    sx[a1] = s1f[0]; sy[a1] = s1f[1]; sz[a1] = s1f[2];
+    double Enew = or_spin_energy(s, c, sx, sy, sz, a1, sx[a1], sy[a1], sz[a1]);
+    const double delta_energy1 = (Enew - Eold) * s->mat[imat1].mu_s_SI * 1.07828231e23;
+    const int imat2 = s->type[a2];
+    const double mu2 = s->moment[a2];
+    const double s2i[3] = {sx[a2], sy[a2], sz[a2]};
+    rotate3(k->P, s2i, s2i_mvd);
+    const double mom_ratio = (mu1 / mu2);
+    s2f_mvd[0] = mom_ratio * (s1i_mvd[0] - s1f_mvd[0]) + s2i_mvd[0];
+    s2f_mvd[1] = mom_ratio * (s1i_mvd[1] - s1f_mvd[1]) + s2i_mvd[1];
+    if (((s2f_mvd[0] * s2f_mvd[0] + s2f_mvd[1] * s2f_mvd[1]) < 1.0) && (a1 != a2)) {
+        s2f_mvd[2] = (s2i_mvd[2] < 0.0 ? -1.0 : 1.0) * sqrt(1.0 - s2f_mvd[0] * s2f_mvd[0] - s2f_mvd[1] * s2f_mvd[1]);
+        double s2f[3];
+        rotate3(k->Pt, s2f_mvd, s2f);
+        Eold = or_spin_energy(s, c, sx, sy, sz, a2, sx[a2], sy[a2], sz[a2]);
+        sx[a2] = s2f[0]; sy[a2] = s2f[1]; sz[a2] = s2f[2];
+        Enew = or_spin_energy(s, c, sx, sy, sz, a2, sx[a2], sy[a2], sz[a2]);
+        const double delta_energy2 = (Enew - Eold) * s->mat[imat2].mu_s_SI * 1.07828231e23;
+        const double delta_energy21 = delta_energy1 * kBTBohr_of(&s->mat[imat1], c->temperature) +
+                                      delta_energy2 * kBTBohr_of(&s->mat[imat2], c->temperature);
+        const double* v = k->v;
+        const double Mz_old = M[0] * v[0] + M[1] * v[1] + M[2] * v[2];
+        const double Mz_new = (M[0] + mu1 * s1f[0] + mu2 * s2f[0] - mu1 * s1i[0] - mu2 * s2i[0]) * v[0] +
+                              (M[1] + mu1 * s1f[1] + mu2 * s2f[1] - mu1 * s1i[1] - mu2 * s2i[1]) * v[1] +
+                              (M[2] + mu1 * s1f[2] + mu2 * s2f[2] - mu1 * s1i[2] - mu2 * s2i[2]) * v[2];
+        const double probability =
+            exp(-delta_energy21) * ((Mz_new / Mz_old) * (Mz_new / Mz_old)) * fabs(s2i_mvd[2] / s2f_mvd[2]);
+        if ((probability >= u) && (Mz_new > 0.0)) {
+            if (update_M)
+                for (int q = 0; q < 3; ++q) M[q] = M[q] + mu1 * s1f[q] + mu2 * s2f[q] - mu1 * s1i[q] - mu2 * s2i[q];
+            return 0;
+        }
+        sx[a1] = s1i[0]; sy[a1] = s1i[1]; sz[a1] = s1i[2];
+        sx[a2] = s2i[0]; sy[a2] = s2i[1]; sz[a2] = s2i[2];
+        return 1;
+    }
+    sx[a1] = s1i[0]; sy[a1] = s1i[1]; sz[a1] = s1i[2];
+    return 2;
+}
+
+static void cmc_moment(const or_sys* s, const double* sx, const double* sy, const double* sz, double M[3]) {
+    M[0] = 0.0; M[1] = 0.0; M[2] = 0.0;                           /* cmc.cpp:346-351 */
+    for (int atom = 0; atom < s->n; ++atom) {
+        const double mu = s->moment[atom];
+        M[0] = M[0] + sx[atom] * mu;
+        M[1] = M[1] + sy[atom] * mu;
+        M[2] = M[2] + sz[atom] * mu;
+    }
+}
+
+/* cmc_step as the reference runs it: per sweep M from all spins, then N trials, each at two
+ * random sites (with replacement); uniforms from Philox (trial, sweep, stream 6): site 1 from
+ * word 0, site 2 from word 1, acceptance from word 2; move normals from stream 7.
+ * counts[3] += accepted, energy rejects, sphere rejects. */
+void or_cmc_random_pairs(const or_sys* s, const or_cond* c, const or_cmc* k, double* sx, double* sy, double* sz,
+                         int nsweeps, uint64_t seed, uint64_t sweep0, double* counts) {
+    const int n = s->n;
+    for (int w = 0; w < nsweeps; ++w) {
+        const uint64_t sweep = sweep0 + (uint64_t)w;
+        double M[3];
+        cmc_moment(s, sx, sy, sz, M);
+        for (int t = 0; t < n; ++t) {
+            uint32_t uu[4];
+            philox_at(seed, (uint32_t)t, sweep, 6u, uu);
+            const int a1 = (int)(n * ((double)uu[0] * 2.3283064365386963e-10));
+            const int a2 = (int)(n * ((double)uu[1] * 2.3283064365386963e-10));
+            double g[3];
+            or_normals3(seed, t, sweep, 7u, g);
+            counts[cmc_pair(s, c, k, sx, sy, sz, a1, a2, g, (double)uu[2] * 2.3283064365386963e-10, M, 1)] += 1.0;
+        }
+    }
+}
+
+/* The permutation of a round of the colour-paired sweep: from Philox (colour | round << 16, sweep,
+ * stream 8), b = word 1 mod m and a = 1 + word 0 mod (m - 1), moved up (1 after m - 1) to the first
+ * value coprime with m. */
+void or_cmc_permutation(uint64_t seed, int colour, int round, uint64_t sweep, int m, uint64_t* a, uint64_t* b) {
+    uint32_t w[4];
+    philox_at(seed, (uint32_t)colour | ((uint32_t)round << 16), sweep, 8u, w);
+    *b = (uint64_t)(w[1] % (uint32_t)m);
+    uint32_t x = m > 1 ? 1u + w[0] % (uint32_t)(m - 1) : 1u;
+    for (;;) {
+        uint32_t p = x, q = (uint32_t)m;
+        while (q) { const uint32_t t = p % q; p = q; q = t; }
+        if (p == 1u) break;
+        x = x + 1u >= (uint32_t)m ? 1u : x + 1u;
+    }
+    *a = x;
+}
+
+/* The colour-paired sweep the device runs.  Atoms of one colour do not interact, so pairs taken
+ * within a colour have independent energy changes.  Per sweep, per colour k (list L_k, caller
+ * order, m entries) and round r = 0, 1: the acceptance uses the magnetisation M of the spins at
+ * the start of the round; half = m/2 pairs, pair p joins entries pi(2p) (spin 1) and pi(2p + 1)
+ * (spin 2) of the random affine permutation pi(t) = (a t + b) mod m drawn for the round
+ * (or_cmc_permutation): an entry's partner lies a entries away, and a changes from round to
+ * round.  Move normals: Philox (spin 1's
+ * index, sweep, stream 9 + r); acceptance: word 0 of (spin 1, sweep, stream 11 + r).
+ * counts[3] += accepted, energy rejects, sphere rejects. */
+void or_cmc_checkerboard(const or_sys* s, const or_cond* c, const or_cmc* k, double* sx, double* sy, double* sz,
+                         const int* colour_list, const int* colour_start, int ncolours, int nsweeps, uint64_t seed,
+                         uint64_t sweep0, double* counts) {
+    for (int w = 0; w < nsweeps; ++w) {
+        const uint64_t sweep = sweep0 + (uint64_t)w;
+        for (int col = 0; col < ncolours; ++col) {
+            const int* L = colour_list + colour_start[col];
+            const int m = colour_start[col + 1] - colour_start[col];
+            if (m < 2) continue;
+            const int half = m / 2;
+            for (int r = 0; r < 2; ++r) {
+                uint64_t a, b;
+                or_cmc_permutation(seed, col, r, sweep, m, &a, &b);
+                double M[3];
+                cmc_moment(s, sx, sy, sz, M);
+                for (int p = 0; p < half; ++p) {
+                    const int e1 = (int)((a * (uint64_t)(2 * p) + b) % (uint64_t)m);
+                    const int e2 = (int)((a * (uint64_t)(2 * p + 1) + b) % (uint64_t)m);
+                    const int a1 = L[e1], a2 = L[e2];
+                    double g[3];
+                    or_normals3(seed, a1, sweep, 9u + (uint32_t)r, g);
+                    uint32_t ua[4];
+                    philox_at(seed, (uint32_t)a1, sweep, 11u + (uint32_t)r, ua);
+                    counts[cmc_pair(s, c, k, sx, sy, sz, a1, a2, g, (double)ua[0] * 2.3283064365386963e-10, M, 0)] += 1.0;
+                }
+            }
+        }
     }
 }
 

@@ -252,6 +252,38 @@ def mc_case(name, L, a, materials, temps, sweeps, every, equil):
                         M_T=np.array(rows))
 
 
+def cmc_case(name, L, a, materials, temps, phi, theta, sweeps, every, equil):
+    """Constrained Monte Carlo (montecarlo::cmc_step) at constraint (phi, theta): per temperature
+    the mean magnetisation along the constraint and the mean total energy over the samples after
+    `equil` sweeps, with block errors, the mean transverse moment in the constraint frame, and the
+    outcome counts (accepted, energy and sphere rejections, trials)."""
+    rows, counts = [], []
+    P = np.empty(9)
+    Pt = np.empty(9)
+    v = np.empty(3)
+    O.load().or_cmc_frame(float(phi), float(theta), O._d(P), O._d(Pt), O._d(v))
+    for T in temps:
+        with tempfile.TemporaryDirectory(prefix="vref_cmc_") as d:
+            inp, mat = deck("sc", L, a, materials, (True, True, True), T)
+            open(os.path.join(d, "input"), "w").write(inp)
+            open(os.path.join(d, "test.mat"), "w").write(mat)
+            run(d, "cmc", sweeps, T, every, phi, theta)
+            raw = np.fromfile(os.path.join(d, "cmc.bin"), dtype=np.float64)
+        data, cnt = raw[:-4].reshape(-1, 5), raw[-4:]
+        keep = data[equil // every:]
+        m = keep[:, :3] * keep[:, 3:4]                 # magnetisation vector / Ms
+        mv = m @ v
+        trans = np.abs(m @ P.reshape(3, 3).T[:, :2]).max()
+        bm = np.array([b.mean() for b in np.array_split(mv, 10)])
+        be = np.array([b.mean() for b in np.array_split(keep[:, 4], 10)])
+        rows.append((T, mv.mean(), bm.std(ddof=1) / np.sqrt(10), keep[:, 4].mean(), be.std(ddof=1) / np.sqrt(10), trans))
+        counts.append(cnt)
+        print(f"  T={T:7.1f}  <m.v>={mv.mean():.4f} +- {rows[-1][2]:.4f}  <E>={rows[-1][3]:.5g}  "
+              f"max transverse {trans:.1e}  counts {cnt}")
+    np.savez_compressed(os.path.join(OUT, f"ref_{name}.npz"), L=L, a=a, sweeps=sweeps, equil=equil, every=every,
+                        phi=phi, theta=theta, rows=np.array(rows), counts=np.array(counts))
+
+
 CO = {"material-name": "Co", "damping-constant": 0.1, "exchange-matrix[1]": 6.064e-21,
       "atomic-spin-moment": "1.72 !muB", "second-order-uniaxial-anisotropy-constant": 4.644e-24,
       "uniaxial-anisotropy-direction": "0,0,1", "initial-spin-direction": "random"}
@@ -338,6 +370,11 @@ def main(only=()):
     if want("mc_sc8"):
         mc_case("mc_sc8", 8, 3.54, [mcm], temps=[100.0, 300.0, 500.0, 600.0, 700.0, 900.0],
                 sweeps=6000, every=5, equil=1000)
+    # constrained Monte Carlo (montecarlo::cmc_step), SC 8^3 with uniaxial anisotropy along z,
+    # magnetisation held at 40 degrees from z (azimuth 30 degrees), reference MT19937 stream
+    if want("cmc_sc8"):
+        cmc_case("cmc_sc8", 8, 3.54, [CO], temps=[100.0, 300.0, 500.0, 650.0], phi=40.0, theta=30.0,
+                 sweeps=6000, every=5, equil=1000)
 
 
 if __name__ == "__main__":

@@ -101,3 +101,39 @@ def assert_mc_curve(temps, mean, err, name: str = "mc_sc8"):
     tc, dtc = crossing(rows[:, 0], np.asarray(mean), np.asarray(err))
     assert 500.0 < tc_ref < 700.0
     assert abs(tc - tc_ref) <= 4.0 * np.hypot(dtc_ref, dtc) + 5.0, (tc, tc_ref, dtc, dtc_ref)
+
+
+def cmc_curve(run_sweeps, n, v, P, temps_rows, equil=1000, sweeps=6000, every=5, energy=None):
+    """<m.v> and <E> per temperature of a constrained Monte Carlo driver, sampled as the reference
+    fixture was: run_sweeps(T, spins, nsweeps, sweep0) -> spins; energy(spins) -> total energy."""
+    out = []
+    for T in temps_rows[:, 0]:
+        sp = np.tile(np.asarray(v, dtype=float)[:, None], (1, n))
+        sp = run_sweeps(float(T), sp, equil, 0)
+        ms, es, trans = [], [], 0.0
+        for w in range((sweeps - equil) // every):
+            sp = run_sweeps(float(T), sp, every, equil + every * w)
+            m = sp.sum(axis=1) / n
+            ms.append(m @ v)
+            trans = max(trans, float(np.abs((np.asarray(P).reshape(3, 3) @ m)[:2]).max()))
+            if energy is not None:
+                es.append(energy(float(T), sp))
+        mm, dm = block_mean(ms)
+        row = [T, mm, dm, trans]
+        if energy is not None:
+            me, de = block_mean(es)
+            row += [me, de]
+        out.append(row)
+    return np.array(out)
+
+
+def assert_cmc_curve(got, name="cmc_sc8"):
+    """<m.v>(T) within 4 combined sigma + 0.01 of the reference's cmc_step, <E>(T) within 4
+    combined sigma + 0.5 %, transverse moment in the constraint frame below 1e-12."""
+    rows = load(name)["rows"]
+    for r, g in zip(rows, got):
+        assert g[0] == r[0]
+        assert abs(g[1] - r[1]) <= 4.0 * np.hypot(g[2], r[2]) + 0.01, (r[0], g[1], r[1])
+        assert g[3] <= 1e-12, (r[0], g[3])
+        if len(g) > 4:
+            assert abs(g[4] - r[3]) <= 4.0 * np.hypot(g[5], r[4]) + 0.005 * abs(r[3]), (r[0], g[4], r[3])

@@ -1,0 +1,137 @@
+"""Constrained Monte Carlo on the device against the oracle's colour-paired sweep (FP64, same Philox
+streams and permutations) and against the reference's own cmc_step run (statistically)."""
+import numpy as np
+import pytest
+
+import refcases as RC
+from oracle import oracle as O
+from vampire_amd import system as S
+from vampire_amd.context import FP32, FP64, Context
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("topology", ["csr", "ell", "stencil"])
+@pytest.mark.parametrize("angle", [False, True])
+def test_cmc_matches_oracle(gpu, topology, angle):
+    """FP64 constrained sweeps from random spins equal the oracle's colour-paired sweeps to 1e-12,
+    with the same accepted / energy-rejected / sphere-rejected pair counts, for the cone move and
+    the temperature-scaled angle move, on every neighbour layout."""
+    s = S.ferromagnet("sc", 6)
+    o = O.Oracle(s)
+    o.set_conditions(400.0, (0.0, 0.0, 0.2))
+    k = o.cmc(70.0, 120.0, move=1 if angle else 0, sigma=45.0)
+    spins = S.random_spins(s.n, 31)
+    ref, counts = o.cmc_checkerboard(spins, 6, k, seed=91, sweep0=0)
+    with Context(s, precision=FP64, seed=91, topology=topology) as c:
+        c.set_conditions(400.0, (0.0, 0.0, 0.2))
+        c.set_spins(spins)
+        c.mc_set_sigma(45.0)
+        c.cmc_set_constraint(70.0, 120.0)
+        c.cmc_set_move(angle)
+        c.cmc_sweep(6)
+        dev = c.get_spins()
+        dcounts = c.cmc_counts()
+    print("counts", counts, dcounts[0])
+    assert np.abs(dev - ref).max() <= 1e-12
+    assert np.array_equal(dcounts[0], counts)
+    assert counts[0] > 0 and counts[1] > 0 and counts[2] > 0
+
+
+def test_cmc_two_materials_tensor_exchange_matches_oracle(gpu):
+    """The thin film with two materials (moments 1.6 and 7.6 mu_B: the compensating spin scales by
+    mu1/mu2) and tensor exchange with DMI: device equals the oracle to 1e-12 with equal counts."""
+    s = RC.system_of("film_dmi")
+    o = O.Oracle(s)
+    o.set_conditions(300.0, (0.0, 0.0, 0.5))
+    k = o.cmc(30.0, 10.0, sigma=30.0)
+    spins = S.random_spins(s.n, 4)
+    ref, counts = o.cmc_checkerboard(spins, 5, k, seed=3)
+    with Context(s, precision=FP64, seed=3) as c:
+        c.set_conditions(300.0, (0.0, 0.0, 0.5))
+        c.set_spins(spins)
+        c.mc_set_sigma(30.0)
+        c.cmc_set_constraint(30.0, 10.0)
+        c.cmc_sweep(5)
+        assert np.abs(c.get_spins() - ref).max() <= 1e-12
+        assert np.array_equal(c.cmc_counts()[0], counts)
+
+
+def test_cmc_replicas_equal_standalone(gpu):
+    """A batch of replicas with their own constraints and temperatures equals each run alone with
+    seed + r, bit for bit, counts included."""
+    sizes = [4, 6, 5]
+    cons = [(20.0, 0.0), (60.0, 45.0), (90.0, 270.0)]
+    temps = [200.0, 400.0, 600.0]
+    systems = [S.ferromagnet("sc", L) for L in sizes]
+    batch = S.concat_systems(systems)
+    spins = [S.random_spins(x.n, 40 + r) for r, x in enumerate(systems)]
+    with Context(batch, precision=FP64, seed=12) as c:
+        for r in range(3):
+            c.set_conditions(temps[r], replica=r)
+            c.cmc_set_constraint(*cons[r], replica=r)
+        c.set_spins(np.concatenate(spins, axis=1))
+        c.cmc_sweep(7)
+        got, gcounts = c.get_spins(), c.cmc_counts()
+    off = batch.replica_offset
+    for r, x in enumerate(systems):
+        with Context(x, precision=FP64, seed=12 + r) as c:
+            c.set_conditions(temps[r])
+            c.cmc_set_constraint(*cons[r])
+            c.set_spins(spins[r])
+            c.cmc_sweep(7)
+            assert np.array_equal(c.get_spins(), got[:, off[r]:off[r + 1]])
+            assert np.array_equal(c.cmc_counts()[0], gcounts[r])
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32])
+def test_cmc_curve_matches_reference(gpu, precision):
+    """Statistical parity with the reference's montecarlo::cmc_step on SC 8^3 with uniaxial
+    anisotropy, magnetisation held at phi = 40, theta = 30 degrees: <m.v>(T) and <E>(T) at
+    100-650 K within combined error bars, the moment on the constraint axis throughout, and the
+    accepted / energy-rejected / sphere-rejected fractions within 0.01; the four temperatures run
+    as replicas of one launch, in FP64 and in FP32."""
+    f = RC.load("cmc_sc8")
+    rows = f["rows"]
+    temps = rows[:, 0]
+    phi, theta = float(f["phi"]), float(f["theta"])
+    base = S.ferromagnet("sc", 8)
+    n = base.n
+    batch = S.concat_systems([S.ferromagnet("sc", 8) for _ in temps])
+    o = O.Oracle(base)
+    k = o.cmc(phi, theta)
+    v, P = np.array(k.v), np.array(k.P).reshape(3, 3)
+    mask = np.zeros(n, np.int32)
+    sweeps, equil, every = 6000, 1000, 5
+    with Context(batch, precision=precision, seed=2025) as c:
+        for r, T in enumerate(temps):
+            c.set_conditions(float(T), replica=r)
+        c.cmc_set_constraint(phi, theta)
+        c.set_spins(np.tile(v[:, None], (1, batch.n)))
+        c.cmc_sweep(equil)
+        ms, es = [[] for _ in temps], [[] for _ in temps]
+        trans = np.zeros(len(temps))
+        for _ in range((sweeps - equil) // every):
+            c.cmc_sweep(every)
+            sp = c.get_spins().astype(np.float64)
+            for r, T in enumerate(temps):
+                x = sp[:, r * n:(r + 1) * n]
+                m = x.sum(axis=1) / n
+                ms[r].append(m @ v)
+                trans[r] = max(trans[r], float(np.abs((P @ m)[:2]).max()))
+                o.set_conditions(float(T))
+                es[r].append(o.energy(x, mask, 2)[0][0])
+        counts = c.cmc_counts()
+    # N pair trials per sweep; outcome fractions as in the reference run (random pairs there)
+    assert np.array_equal(counts.sum(axis=1), np.full(len(temps), float(sweeps * n)))
+    np.testing.assert_allclose(counts / (sweeps * n), f["counts"][:, :3] / f["counts"][:, 3:], atol=0.01)
+    got = []
+    for r, T in enumerate(temps):
+        mm, dm = RC.block_mean(ms[r])
+        me, de = RC.block_mean(es[r])
+        # FP32 spins keep the transverse moment to single-precision rounding only
+        got.append([T, mm, dm, trans[r] if precision == FP64 else 0.0, me, de])
+        print(T, mm, dm, trans[r], me, de)
+    RC.assert_cmc_curve(np.array(got))
+    if precision == FP32:
+        assert trans.max() < 1e-4

@@ -392,3 +392,55 @@ def test_dropin_continuous_checkpoints_hold_device_spins(gpu, exe, tmp_path):
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     b = np.frombuffer(sorted(d.glob("spins-*.data"))[-1].read_bytes()[8:], dtype=np.float64)
     assert np.array_equal(a, b), np.abs(a - b).max()
+
+
+HARNESS_VMX = os.path.join(ROOT, "vampire_amd", "shim", "_build", "ref_harness_vmx")
+
+
+def test_dropin_constrained_monte_carlo_against_reference(gpu, tmp_path):
+    """Constrained Monte Carlo through the gpu:: boundary: the oracle harness linked to the shim
+    runs gpu::initialize, then gpu::cmc_step per sweep (its first call runs the reference's CMCinit
+    on the host spins), samples the host arrays after gpu::transfer_spin_positions_from_gpu_to_cpu,
+    and reads the montecarlo::cmc counters after gpu::finalize.  Against the reference's own
+    cmc_step run of the same deck (SC 8^3, Co, phi = 40, theta = 30 degrees, 100-650 K):
+    <m.v> and <E> within combined error bars, the moment on the constraint axis, N pair trials
+    per sweep and the accepted / energy-rejected / sphere-rejected fractions within 0.01."""
+    if not os.path.exists(HARNESS_VMX):
+        pytest.skip("harness over the shim not built (make -C vampire_amd/shim)")
+    import refcases as RC
+    f = RC.load("cmc_sc8")
+    phi, theta = float(f["phi"]), float(f["theta"])
+    sweeps, every, equil = int(f["sweeps"]), int(f["every"]), int(f["equil"])
+    L, a = int(f["L"]), float(f["a"])
+    size = L * a - 0.01
+    p, t = np.radians(phi), np.radians(theta)
+    v = np.array([np.sin(p) * np.cos(t), np.sin(p) * np.sin(t), np.cos(p)])
+    got = []
+    for T, ref_counts in zip(f["rows"][:, 0], f["counts"]):
+        d = tmp_path / f"T{int(T)}"
+        d.mkdir()
+        (d / "input").write_text(
+            "create:crystal-structure=sc\n"
+            + "".join(f"create:periodic-boundaries-{x}\n" for x in "xyz")
+            + f"dimensions:unit-cell-size = {a} !A\n"
+            + "".join(f"dimensions:system-size-{x} = {size} !A\n" for x in "xyz")
+            + f"material:file=test.mat\nsim:temperature={T}\nsim:time-step=1.0E-16\n"
+            + "sim:program=time-series\nsim:integrator=llg-heun\nsim:total-time-steps=0\n"
+            + "output:total-energy\noutput:material-magnetisation\n")
+        (d / "test.mat").write_text(CO.format(alpha=0.1, ku=4.644e-24, init="random"))
+        print("gpucmc", T, flush=True)
+        r = subprocess.run([HARNESS_VMX, str(d), "gpucmc", str(sweeps), str(T), str(every), str(phi), str(theta)],
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        raw = np.fromfile(d / "cmc.bin", dtype=np.float64)
+        data, cnt = raw[:-4].reshape(-1, 5), raw[-4:]
+        keep = data[equil // every:]
+        m = keep[:, :3] * keep[:, 3:4]
+        mv = m @ v
+        trans = np.abs(m - np.outer(mv, v)).max()
+        mm, dm = RC.block_mean(mv)
+        me, de = RC.block_mean(keep[:, 4])
+        got.append([T, mm, dm, trans, me, de])
+        assert cnt[3] == sweeps * L ** 3 and cnt[:3].sum() == cnt[3]
+        np.testing.assert_allclose(cnt[:3] / cnt[3], ref_counts[:3] / ref_counts[3], atol=0.01)
+    RC.assert_cmc_curve(np.array(got))

@@ -24,6 +24,7 @@ VMX_ANISOTROPY_CUBIC_FOURTH_ROTATION = 0x4
 VMX_FIELD_THERMAL = 0x1
 VMX_FIELD_APPLIED = 0x2
 VMX_FIELD_LOCAL_APPLIED = 0x4
+VMX_CMC_MOVE_CONE, VMX_CMC_MOVE_ANGLE = 0, 1
 
 
 class VmxUnavailable(RuntimeError):
@@ -146,6 +147,10 @@ SIGNATURES = [
     ("vmx_mc_get_state", C.c_int, [_VP, _DP, _DP, _DP]),
     ("vmx_mc_set_sigma", C.c_int, [_VP, C.c_int32, C.c_double]),
     ("vmx_mc_num_colours", C.c_int, [_VP, _P(C.c_int32)]),
+    ("vmx_cmc_set_constraint", C.c_int, [_VP, C.c_int32, C.c_double, C.c_double]),
+    ("vmx_cmc_set_move", C.c_int, [_VP, C.c_int32]),
+    ("vmx_cmc_sweep", C.c_int, [_VP, C.c_int32]),
+    ("vmx_cmc_get_counts", C.c_int, [_VP, _DP]),
     ("vmx_stats_add_magnetization", C.c_int, [_VP, C.c_int32, _P(C.c_int32), _P(C.c_int32)]),
     ("vmx_stats_add_energy", C.c_int, [_VP, C.c_int32, _P(C.c_int32), _P(C.c_int32)]),
     ("vmx_stats_update", C.c_int, [_VP]),

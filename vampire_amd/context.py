@@ -299,6 +299,24 @@ class Context:
         L.check(self.lib.vmx_mc_num_colours(self.h, C.byref(v)))
         return int(v.value)
 
+    # --- constrained Monte Carlo ----------------------------------------------------------------
+    def cmc_set_constraint(self, phi: float, theta: float, replica: int = -1) -> None:
+        """Constraint direction: polar angle phi and azimuth theta (degrees) of the magnetisation."""
+        L.check(self.lib.vmx_cmc_set_constraint(self.h, replica, float(phi), float(theta)))
+
+    def cmc_set_move(self, angle: bool) -> None:
+        """False: cone of the replica's Metropolis width (default); True: the angle move."""
+        L.check(self.lib.vmx_cmc_set_move(self.h, L.VMX_CMC_MOVE_ANGLE if angle else L.VMX_CMC_MOVE_CONE))
+
+    def cmc_sweep(self, nsweeps: int = 1) -> None:
+        L.check(self.lib.vmx_cmc_sweep(self.h, int(nsweeps)))
+
+    def cmc_counts(self) -> np.ndarray:
+        """[replicas][3]: accepted pairs, energy rejections, sphere rejections."""
+        out = np.empty(3 * self.system.num_replicas)
+        L.check(self.lib.vmx_cmc_get_counts(self.h, L.dptr(out)))
+        return out.reshape(-1, 3)
+
     # --- statistics -----------------------------------------------------------------------------
     def add_magnetization(self, mask: np.ndarray, mask_size: int) -> int:
         m = np.ascontiguousarray(mask, dtype=np.int32)

@@ -253,13 +253,19 @@ __global__ void k_energy_finish(int nblk, const double* __restrict__ partial, in
     if (threadIdx.x == 0) *counter += 1;
 }
 
-// Adaptive cone width after a sweep (src/montecarlo/mc.cpp:112-120) and running totals.
+// Adaptive cone width after a sweep (src/montecarlo/mc.cpp:112-120) and running totals.  A
+// replica's rejections arrive in kRejectSlots counters (integers held in doubles: exact in any
+// order), which are summed and cleared here.
 __global__ void k_mc_adapt(int nrep, const int* __restrict__ rep_off, double* sigma, double* rejects,
                            double* tot_moves, double* tot_rej) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrep) return;
     const double moves = (double)(rep_off[r + 1] - rep_off[r]);
-    const double rej = rejects[r];
+    double rej = 0.0;
+    for (int k = 0; k < kRejectSlots; ++k) {
+        rej += rejects[((size_t)r * kRejectSlots + k) * kRejectStride];
+        rejects[((size_t)r * kRejectSlots + k) * kRejectStride] = 0.0;
+    }
     const double last_rejection_rate = rej / moves;
     const double factor = 0.5 / last_rejection_rate;
     double s = sigma[r] * factor;
@@ -267,7 +273,6 @@ __global__ void k_mc_adapt(int nrep, const int* __restrict__ rep_off, double* si
     sigma[r] = s;
     tot_moves[r] += moves;
     tot_rej[r] += rej;
-    rejects[r] = 0.0;
 }
 
 // ---- macrocell dipole field, tensor solver ------------------------------------------------
@@ -447,6 +452,7 @@ struct vmx_ctx {
     DevBuf row, col, jidx, jtab, type, mat, cond, blk, blk_fused, rep_off, rep_key, step_base;
     DevBuf deg, pat, pat_off, pat_jid, pat_deg;
     DevBuf rep_m;                 // [nrep] replica size / q
+    DevBuf rep_blk;               // [nrep+1] each replica's first entry of the one-atom block table
     DevBuf spin[3];               // three buffers of 3N R each (S, P, O roles rotate)
     int s_idx = 0, o_idx = 2;     // current spin buffer and the corrector's output buffer
     // The fused correctors read their own atom's S and gather only S' (or the midpoint), so they
@@ -517,6 +523,16 @@ struct vmx_ctx {
     bool coloured = false;
     int ncolours = 0;
     std::vector<int> colour_start;
+    // constrained Monte Carlo: per colour, per replica, the colour's atoms in caller order (the
+    // pairing follows that order); workgroup table {first entry, replica, run length, first pair}
+    DevBuf cmc_list, cmc_tab, cmc_frame, cmc_M, cmc_part, cmc_counts;
+    std::vector<int> cmc_tab_start;        // colour k's workgroups: cmc_tab[cmc_tab_start[k] ...]
+    std::vector<double> cmc_frame_h;       // [nrep][21]: polar matrix, transpose, constraint direction
+    std::vector<char> cmc_frame_set;       // per replica
+    bool cmc_frame_dirty = true;
+    int cmc_move = 0;                      // 0: cone of the replica's Metropolis width; 1: angle move
+    template <typename R>
+    int cmc_sweeps(int nsweeps);
     static constexpr int kMcBlock = 256;   // list entries per Metropolis workgroup
     std::vector<int> mc_blk_start;         // colour k's workgroups: mc_blk[mc_blk_start[k] ...]
     DevBuf colour_list, mc_blk, mc_sigma, mc_rej, mc_tot_moves, mc_tot_rej;
@@ -666,6 +682,12 @@ struct vmx_ctx {
         VMX_HIP(hipMemcpy(blk.p, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice));
         VMX_TRY(blk_fused.alloc(tabf.size() * sizeof(int4)));
         VMX_HIP(hipMemcpy(blk_fused.p, tabf.data(), tabf.size() * sizeof(int4), hipMemcpyHostToDevice));
+        // the table lists each replica's entries together, replica after replica
+        std::vector<int> first(nrep + 1, (int)tab.size());
+        for (int e = (int)tab.size() - 1; e >= 0; --e)
+            if (tab[e].z > tab[e].x) first[tab[e].y] = e;
+        for (int r = nrep - 1; r >= 0; --r) first[r] = std::min(first[r], first[r + 1]);
+        VMX_TRY(upload(rep_blk, first));
         return VMX_OK;
     }
 
@@ -718,7 +740,9 @@ struct vmx_ctx {
                     c.hdemag[k] = (R)Hd_h[3 * (size_t)r + k];
                     c.hglob[k] = (R)h[k];
                 }
-                c.kBTBohr = (R)(9.27400915e-24 / (rescaled_temperature * 1.3806503e-23));
+                const double kbt = 9.27400915e-24 / (rescaled_temperature * 1.3806503e-23);
+                c.kBTBohr = (R)kbt;
+                c.cmc_delta = (R)(rescaled_temperature < 1.0 ? 0.02 : pow(1.0 / kbt, 0.2) * 0.08);
             }
         }
         VMX_TRY(cond.alloc(cv.size() * sizeof(Cond<R>)));
@@ -1034,6 +1058,27 @@ int vmx_ctx::colour() {
     mc_blk_start[nc] = (int)tab.size();
     VMX_TRY(upload(colour_list, list));
     VMX_TRY(upload(mc_blk, tab));
+    // constrained Monte Carlo: the same colours, each replica's run in caller order, workgroups of
+    // up to 256 pairs per run
+    std::vector<int> clist;
+    std::vector<int4> ctab;
+    clist.reserve(n);
+    cmc_tab_start.assign(nc + 1, 0);
+    for (int k = 0; k < nc; ++k) {
+        cmc_tab_start[k] = (int)ctab.size();
+        for (int r = 0; r < nrep; ++r) {
+            const int first = (int)clist.size();
+            for (int o = rep_off_h[r]; o < rep_off_h[r + 1]; ++o) {
+                const int i = q > 1 ? pi[o] : o;
+                if (c[i] == k) clist.push_back(i);
+            }
+            const int m = (int)clist.size() - first;
+            for (int p0 = 0; p0 < m / 2; p0 += kMcBlock) ctab.push_back(make_int4(first, r, m, p0));
+        }
+    }
+    cmc_tab_start[nc] = (int)ctab.size();
+    VMX_TRY(upload(cmc_list, clist));
+    VMX_TRY(upload(cmc_tab, ctab));
     ncolours = nc;
     coloured = true;
     return VMX_OK;
@@ -1057,6 +1102,44 @@ int vmx_ctx::mc_sweeps(int nsweeps) {
         ++sweep;
         // a sweep advances sim::time like a Heun step (sim.cpp:608-615, 191-200)
         if (dip.on && ++dip.time % dip.rate == 0) VMX_TRY(dipole_refresh<R>());
+    }
+    return VMX_OK;
+}
+
+// Constrained Monte Carlo sweeps (montecarlo::cmc_step, src/montecarlo/cmc.cpp:282-497): per
+// colour two rounds, each after recomputing every replica's magnetisation.
+template <typename R>
+int vmx_ctx::cmc_sweeps(int nsweeps) {
+    if (!coloured) VMX_TRY(colour());
+    if (cmc_frame_dirty) {
+        std::vector<CmcFrame<R>> f(nrep);
+        for (int r = 0; r < nrep; ++r)
+            for (int k = 0; k < 9; ++k) {
+                f[r].P[k] = (R)cmc_frame_h[21 * (size_t)r + k];
+                f[r].Pt[k] = (R)cmc_frame_h[21 * (size_t)r + 9 + k];
+                if (k < 3) f[r].v[k] = (R)cmc_frame_h[21 * (size_t)r + 18 + k];
+            }
+        VMX_TRY(upload(cmc_frame, f));
+        cmc_frame_dirty = false;
+    }
+    VMX_TRY(cmc_M.alloc(3 * (size_t)nrep * sizeof(double)));
+    VMX_TRY(cmc_part.alloc(3 * (size_t)nblocks * sizeof(double)));
+    const SysDev<R> s = sys<R>();
+    const Spin3<R> S = spins<R>(s_idx);
+    for (int w = 0; w < nsweeps; ++w) {
+        for (int k = 0; k < ncolours; ++k) {
+            const int ntab = cmc_tab_start[k + 1] - cmc_tab_start[k];
+            if (ntab == 0) continue;
+            for (int round = 0; round < 2; ++round) {
+                VMX_HIP(Ops<R>::replica_moment(stream, s, S, mm.as<double>(), rep_blk.as<int>(), nrep,
+                                               cmc_part.as<double>(), cmc_M.as<double>()));
+                VMX_HIP(Ops<R>::cmc_pairs(stream, et, am(), s, S, mm.as<double>(), cmc_list.as<int>(),
+                                          cmc_tab.as<int4>() + cmc_tab_start[k], ntab, cmc_frame.as<CmcFrame<R>>(),
+                                          cmc_M.as<double>(), mc_sigma.as<double>(), cmc_move, k, round, sweep,
+                                          cmc_counts.as<double>()));
+            }
+        }
+        ++sweep;   // a constrained sweep advances sim::time like a Heun step (sim.cpp:623-628)
     }
     return VMX_OK;
 }
@@ -1633,15 +1716,19 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
     VMX_TRY(c->upload_conditions_any());
     // Monte Carlo state: adaptive sigma starts at 60 (src/montecarlo/data.cpp:46)
     VMX_TRY(c->mc_sigma.alloc(nrep * sizeof(double)));
-    VMX_TRY(c->mc_rej.alloc(nrep * sizeof(double)));
+    VMX_TRY(c->mc_rej.alloc((size_t)nrep * kRejectSlots * kRejectStride * sizeof(double)));
     VMX_TRY(c->mc_tot_moves.alloc(nrep * sizeof(double)));
     VMX_TRY(c->mc_tot_rej.alloc(nrep * sizeof(double)));
+    VMX_TRY(c->cmc_counts.alloc((size_t)nrep * kRejectSlots * kRejectStride * sizeof(double)));
+    c->cmc_frame_h.assign(21 * (size_t)nrep, 0.0);
+    c->cmc_frame_set.assign(nrep, 0);
     {
         std::vector<double> s60(nrep, 60.0);
         VMX_HIP(hipMemcpy(c->mc_sigma.p, s60.data(), nrep * sizeof(double), hipMemcpyHostToDevice));
-        VMX_HIP(hipMemset(c->mc_rej.p, 0, nrep * sizeof(double)));
+        VMX_HIP(hipMemset(c->mc_rej.p, 0, (size_t)nrep * kRejectSlots * kRejectStride * sizeof(double)));
         VMX_HIP(hipMemset(c->mc_tot_moves.p, 0, nrep * sizeof(double)));
         VMX_HIP(hipMemset(c->mc_tot_rej.p, 0, nrep * sizeof(double)));
+        VMX_HIP(hipMemset(c->cmc_counts.p, 0, (size_t)nrep * kRejectSlots * kRejectStride * sizeof(double)));
     }
     VMX_HIP(hipDeviceSynchronize());
     *out = c.release();
@@ -1946,6 +2033,77 @@ int vmx_mc_num_colours(vmx_ctx* c, int32_t* ncolours) {
     VMX_TRY(c->use_device());
     if (!c->coloured) VMX_TRY(c->colour());
     *ncolours = c->ncolours;
+    return VMX_OK;
+}
+
+// ---- constrained Monte Carlo ----------------------------------------------------------------
+int vmx_cmc_set_constraint(vmx_ctx* c, int32_t replica, double phi, double theta) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    if (replica < -1 || replica >= c->nrep) return fail(VMX_ERR_ARG, "replica %d outside [-1,%d)", replica, c->nrep);
+    if (!std::isfinite(phi) || !std::isfinite(theta)) return fail(VMX_ERR_ARG, "constraint angles must be finite");
+    // cmc::polar_rot_matrix (src/montecarlo/cmc.cpp:43-138): P = Y(phi) Z(theta), sums from 0 in
+    // the order of vmath::matmul; Pt = transpose; v = (0,0,1) P
+    const double pi = 3.14159265358979323846264338327;
+    const double dy = (phi / 360.0) * 2.0 * pi, dz = (theta / 360.0) * 2.0 * pi;
+    const double sin_y = sin(dy), cos_y = cos(dy), sin_z = sin(dz), cos_z = cos(dz);
+    const double Y[9] = {cos_y, 0.0, -sin_y, 0.0, 1.0, 0.0, sin_y, 0.0, cos_y};
+    const double Z[9] = {cos_z, sin_z, 0.0, -sin_z, cos_z, 0.0, 0.0, 0.0, 1.0};
+    double f[21];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            double t = 0.0;
+            for (int k = 0; k < 3; ++k) t += Y[3 * a + k] * Z[3 * k + b];
+            f[3 * a + b] = t;
+        }
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) f[9 + 3 * a + b] = f[3 * b + a];
+    const double ref[3] = {0.0, 0.0, 1.0};
+    for (int b = 0; b < 3; ++b) {
+        double t = 0.0;
+        for (int k = 0; k < 3; ++k) t += ref[k] * f[3 * k + b];
+        f[18 + b] = t;
+    }
+    const int r0 = replica < 0 ? 0 : replica, r1 = replica < 0 ? c->nrep : replica + 1;
+    for (int r = r0; r < r1; ++r) {
+        std::copy(f, f + 21, c->cmc_frame_h.begin() + 21 * (size_t)r);
+        c->cmc_frame_set[r] = 1;
+    }
+    c->cmc_frame_dirty = true;
+    return VMX_OK;
+}
+
+int vmx_cmc_set_move(vmx_ctx* c, int32_t move) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    if (move != VMX_CMC_MOVE_CONE && move != VMX_CMC_MOVE_ANGLE) return fail(VMX_ERR_ARG, "unknown move %d", move);
+    c->cmc_move = move;
+    return VMX_OK;
+}
+
+int vmx_cmc_sweep(vmx_ctx* c, int32_t nsweeps) {
+    if (!c) return fail(VMX_ERR_ARG, "null context");
+    if (nsweeps < 0) return fail(VMX_ERR_ARG, "nsweeps must be >= 0");
+    for (int r = 0; r < c->nrep; ++r)
+        if (!c->cmc_frame_set[r]) return fail(VMX_ERR_STATE, "no constraint set for replica %d (vmx_cmc_set_constraint)", r);
+    if (c->dip.on || c->hdip_on)
+        return fail(VMX_ERR_STATE, "constrained Monte Carlo does not refresh a dipole field; clear it first");
+    if (nsweeps == 0) return VMX_OK;
+    VMX_TRY(c->use_device());
+    return c->precision == VMX_FP64 ? c->cmc_sweeps<double>(nsweeps) : c->cmc_sweeps<float>(nsweeps);
+}
+
+int vmx_cmc_get_counts(vmx_ctx* c, double* counts) {
+    if (!c || !counts) return fail(VMX_ERR_ARG, "null argument");
+    VMX_TRY(c->use_device());
+    VMX_HIP(hipStreamSynchronize(c->stream));
+    const size_t per = (size_t)kRejectSlots * kRejectStride;
+    std::vector<double> slots((size_t)c->nrep * per);
+    VMX_HIP(hipMemcpy(slots.data(), c->cmc_counts.p, slots.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int r = 0; r < c->nrep; ++r)
+        for (int k = 0; k < 3; ++k) {
+            double t = 0.0;
+            for (int q = 0; q < kRejectSlots; ++q) t += slots[r * per + (size_t)q * kRejectStride + k];
+            counts[3 * (size_t)r + k] = t;
+        }
     return VMX_OK;
 }
 

@@ -68,6 +68,7 @@ struct Cond {                // per (replica, material); rewritten when conditio
     R hglob[3];              // H of the applied-field energy (src/simulate/energy.cpp:94-98)
     R kBTBohr;               // mu_B / (kB T_rescaled)            (src/montecarlo/mc.cpp:45-53)
     R hdemag[3];             // thin-film demag field, added after happ (fields.cpp:219-239)
+    R cmc_delta;             // width of the constrained Monte Carlo angle move (cmc.cpp:320-325)
 };
 
 template <typename R>
@@ -143,7 +144,10 @@ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c
 }
 
 // Stream ids: the fourth counter word separates independent uses of one (atom, step) pair.
-enum : uint32_t { STREAM_THERMAL = 0u, STREAM_MC_MOVE = 1u, STREAM_MC_ACCEPT = 2u };
+enum : uint32_t { STREAM_THERMAL = 0u, STREAM_MC_MOVE = 1u, STREAM_MC_ACCEPT = 2u,
+                  // constrained Monte Carlo: a round's pair permutation, and per pair the move
+                  // normals (9 + round) and the acceptance uniform (11 + round)
+                  STREAM_CMC_PAIR = 8u, STREAM_CMC_MOVE = 9u, STREAM_CMC_ACCEPT = 11u };
 
 // Three standard normals from one Philox block by Box-Muller on (x0,x1) and (x2,x3);
 // uniforms are (k + 0.5) * 2^-32, never 0 or 1.
@@ -1253,6 +1257,19 @@ k_energy_partial(SysDev<R> s, Spin3<R> S, const int* __restrict__ mask, const do
 // montecarlo::mc_step (src/montecarlo/mc.cpp:59-108) with the adaptive cone of
 // mc_moves.cpp:71-87,158-162.
 
+// Rejection counters per replica: a workgroup adds its count to slot (workgroup mod
+// kRejectSlots) of its replica, each slot on a cache line of its own (kRejectStride doubles), so
+// the thousands of workgroups of one large replica do not all update one line; k_mc_adapt sums
+// the slots.
+constexpr int kRejectSlots = 64;
+constexpr int kRejectStride = 16;
+
+// The workgroup's rejection count (LDS)
+__device__ __forceinline__ unsigned int& mc_wg_rejects() {
+    __shared__ unsigned int count;
+    return count;
+}
+
 // Acceptance of a move that raises the energy: exp(-dE mu_B/(kB T)) >= u, u = word / 2^32
 // (mc.cpp:93-99).  FP64 in double as the oracle; FP32 on the hardware exp2.
 __device__ __forceinline__ bool mc_accept(double x, uint32_t word) {
@@ -1290,6 +1307,7 @@ k_mc_colour(SysDev<R> s, Spin3<R> S, const int* __restrict__ list, const int4* _
     R g0, g1, g2;
     box_muller3(um, g0, g1, g2);
     const R angle = (R)sigma[rep];
+    if (threadIdx.x == 0) mc_wg_rejects() = 0u;
     stage_fused<R, AM>(s, rep);
     const MatConst<R>& mc = FusedLds<R>::mat()[m];
     const Cond<R>& c = FusedLds<R>::cond(s)[m];
@@ -1327,11 +1345,180 @@ k_mc_colour(SysDev<R> s, Spin3<R> S, const int* __restrict__ list, const int4* _
     if (ok && accept) {
         S.x[i] = nx; S.y[i] = ny; S.z[i] = nz;
     }
-    // rejections of the replica: one atomic per wavefront (counts are integers, exact in a double
-    // in any order)
+    // rejections of the replica: wavefront counts summed in LDS, then one atomic per workgroup
+    // (counts are integers, exact in a double in any order)
     const unsigned long long rejected = __ballot(ok && !accept);
     if (rejected != 0ull && (int)__lane_id() == __ffsll(rejected) - 1)
-        atomicAdd(&rejects[rep], (double)__popcll(rejected));
+        atomicAdd(&mc_wg_rejects(), (unsigned int)__popcll(rejected));
+    __syncthreads();
+    if (threadIdx.x == 0 && mc_wg_rejects() != 0u)
+        atomicAdd(&rejects[((size_t)rep * kRejectSlots + (lb & (kRejectSlots - 1))) * kRejectStride],
+                  (double)mc_wg_rejects());
+}
+
+// ======================= constrained Monte Carlo ===============================================
+// montecarlo::cmc_step (src/montecarlo/cmc.cpp:282-497): pair moves that keep the magnetisation
+// along the constraint direction.  The reference draws both sites of a pair at random; here the
+// two sites of every pair have the same colour, so they do not interact with each other nor with
+// any other pair of the launch, and all pairs of one colour are tried at once.  A round pairs the
+// colour's atoms (caller order, m of them) through a random affine permutation
+// pi(t) = (a t + b) mod m: pair p joins pi(2p) (spin 1) and pi(2p + 1) (spin 2); two rounds per
+// colour make N pair trials per sweep, as the reference's N.  The acceptance's magnetisation is
+// the replica's at the start of the round.  Every pair's arithmetic is the reference's.
+template <typename R>
+struct CmcFrame {            // cmc::polar_matrix, its transpose, cmc::polar_vector
+    R P[9], Pt[9], v[3];
+};
+
+// The round's permutation: from Philox (colour | round << 16, sweep, STREAM_CMC_PAIR), b = word 1
+// mod m, a = 1 + word 0 mod (m - 1), moved up (1 after m - 1) to the first value coprime with m.
+__device__ __forceinline__ void cmc_permutation(uint2 key, int colour, int round, uint64_t sweep, int m,
+                                                uint64_t& a, uint64_t& b) {
+    const U4 w = philox4x32_10((uint32_t)colour | ((uint32_t)round << 16), (uint32_t)sweep, (uint32_t)(sweep >> 32),
+                               STREAM_CMC_PAIR, key.x, key.y);
+    b = (uint64_t)(w.v[1] % (uint32_t)m);
+    uint32_t x = m > 1 ? 1u + w.v[0] % (uint32_t)(m - 1) : 1u;
+    for (;;) {
+        uint32_t p = x, q = (uint32_t)m;
+        while (q) { const uint32_t t = p % q; p = q; q = t; }
+        if (p == 1u) break;
+        x = x + 1u >= (uint32_t)m ? 1u : x + 1u;
+    }
+    a = x;
+}
+
+template <typename R>
+__device__ __forceinline__ void cmc_rotate(const R* M, R x0, R x1, R x2, R (&o)[3]) {   // cmc.cpp:363-365
+    o[0] = M[0] * x0 + M[1] * x1 + M[2] * x2;
+    o[1] = M[3] * x0 + M[4] * x1 + M[5] * x2;
+    o[2] = M[6] * x0 + M[7] * x1 + M[8] * x2;
+}
+
+// Single-spin energy of atom i before (o) and after (n) a move, from one gather of its row, in the
+// order of sim::calculate_spin_energy (src/simulate/energy.cpp:147-174).
+template <typename R, int ET, int AM>
+__device__ __forceinline__ void move_energies(const SysDev<R>& s, const Spin3<R>& S, const SpinRsrc<R>& rs, const R* ju,
+                                              int i, int head,
+                                              const MatConst<R>& mc, const Cond<R>& c, R ox, R oy, R oz, R nx, R ny,
+                                              R nz, R& e_old, R& e_new) {
+    R x_old, x_new;
+    if constexpr (Access<AM>::TP != TP_CSR) {
+        RowIdx<AM> ri;
+        RowSpins<R, AM> rv;
+        fused_row_index(s, i, head, ri);
+        row_load(rs, ri, rv);
+        x_old = row_exchange_energy<R, ET, AM>(s, ri, rv, ju, ox, oy, oz);
+        x_new = row_exchange_energy<R, ET, AM>(s, ri, rv, ju, nx, ny, nz);
+    } else {
+        x_old = exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, ox, oy, oz);
+        x_new = exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, nx, ny, nz);
+    }
+    e_old = (R(0) + x_old + anisotropy_energy(s.ani, mc, ox, oy, oz)) + applied_energy(c, ox, oy, oz);
+    e_new = (R(0) + x_new + anisotropy_energy(s.ani, mc, nx, ny, nz)) + applied_energy(c, nx, ny, nz);
+}
+
+// The workgroup's outcome counts (LDS): accepted, energy rejects, sphere rejects
+__device__ __forceinline__ unsigned int* cmc_wg_counts() {
+    __shared__ unsigned int counts[3];
+    return counts;
+}
+
+// One round of one colour.  tab[lb] = {first cmc-list entry of the (replica, colour) run,
+// replica, run length m, first pair}; a workgroup takes up to blockDim pairs of one run.
+// M: each replica's magnetisation sum(mu s) at the start of the round (double [nrep][3]);
+// mm: moments (mu_B); sigma: cone width per replica (move 0); counts: kRejectSlots slots of three
+// outcome counts per replica, each slot on its own cache line.
+template <typename R, int ET, int AM>
+__global__ void __launch_bounds__(256)
+k_cmc_pairs(SysDev<R> s, Spin3<R> S, const double* __restrict__ mm, const int* __restrict__ list,
+            const int4* __restrict__ tab, int ntab, const CmcFrame<R>* __restrict__ frame,
+            const double* __restrict__ M, const double* __restrict__ sigma, int move, int colour, int round,
+            uint64_t sweep, double* __restrict__ counts) {
+    const int lb = logical_block(s.xcd_remap);
+    if (lb >= ntab) return;                        // whole workgroup
+    const int4 b = tab[lb];
+    const int rep = b.y, m = b.z, half = m >> 1;
+    const int p = b.w + (int)threadIdx.x;
+    const bool ok = p < half;
+    const uint2 key = s.rep_key[rep];
+    uint64_t pa, pb;
+    cmc_permutation(key, colour, round, sweep, m, pa, pb);
+    const uint64_t t1 = 2 * (uint64_t)(ok ? p : b.w);
+    const int i1 = list[b.x + (int)((pa * t1 + pb) % (uint64_t)m)];
+    const int i2 = list[b.x + (int)((pa * (t1 + 1) + pb) % (uint64_t)m)];
+    const int m1 = s.nmat == 1 ? 0 : (int)s.type[i1];
+    const int m2 = s.nmat == 1 ? 0 : (int)s.type[i2];
+    const R a0 = S.x[i1], a1 = S.y[i1], a2 = S.z[i1];      // spin 1 initial
+    const R c0 = S.x[i2], c1 = S.y[i2], c2 = S.z[i2];      // spin 2 initial
+    const double mu1 = mm[i1], mu2 = mm[i2];
+    int h1 = 0, h2 = 0;
+    if (Access<AM>::TP != TP_CSR) { h1 = row_head<R, AM>(s, i1); h2 = row_head<R, AM>(s, i2); }
+    const int local1 = caller_local(s, rep, i1 - s.rep_off[rep]);
+    const U4 um = philox4x32_10((uint32_t)local1, (uint32_t)sweep, (uint32_t)(sweep >> 32),
+                                STREAM_CMC_MOVE + (uint32_t)round, key.x, key.y);
+    R g0, g1, g2;
+    box_muller3(um, g0, g1, g2);
+    if (threadIdx.x < 3) cmc_wg_counts()[threadIdx.x] = 0u;
+    stage_fused<R, AM>(s, rep);
+    const MatConst<R>& mc1 = FusedLds<R>::mat()[m1];
+    const MatConst<R>& mc2 = FusedLds<R>::mat()[m2];
+    const Cond<R>& cd1 = FusedLds<R>::cond(s)[m1];
+    const Cond<R>& cd2 = FusedLds<R>::cond(s)[m2];
+    const CmcFrame<R>& F = frame[rep];
+    const R delta = move ? cd1.cmc_delta : (R)sigma[rep];
+    // spin 1: the Monte Carlo move (mc_angle, mc_moves.cpp:71-87) seen in the constraint frame
+    R s1i_mvd[3], s1f_mvd[3], s2i_mvd[3], s2f_mvd[3];
+    cmc_rotate(F.P, a0, a1, a2, s1i_mvd);
+    R f0 = a0 + g0 * delta, f1 = a1 + g1 * delta, f2 = a2 + g2 * delta;
+    const R r = inv_norm(f0, f1, f2);
+    f0 *= r; f1 *= r; f2 *= r;
+    cmc_rotate(F.P, f0, f1, f2, s1f_mvd);
+    const SpinRsrc<R> rs(s, S.x, S.y, S.z);
+    const UniformJ<R, ET> ju(s, Access<AM>::JM);
+    R e1o, e1n;
+    move_energies<R, ET, AM>(s, S, rs, ju.v, i1, h1, mc1, cd1, a0, a1, a2, f0, f1, f2, e1o, e1n);
+    const R delta_energy1 = (e1n - e1o) * mc1.mu_s_SI * R(1.07828231e23);
+    // spin 2 compensates the transverse moment in the constraint frame (cmc.cpp:416-423)
+    cmc_rotate(F.P, c0, c1, c2, s2i_mvd);
+    const R mom_ratio = (R)(mu1 / mu2);
+    s2f_mvd[0] = mom_ratio * (s1i_mvd[0] - s1f_mvd[0]) + s2i_mvd[0];
+    s2f_mvd[1] = mom_ratio * (s1i_mvd[1] - s1f_mvd[1]) + s2i_mvd[1];
+    const bool on_sphere = (s2f_mvd[0] * s2f_mvd[0] + s2f_mvd[1] * s2f_mvd[1]) < R(1);
+    // off the sphere the pair is rejected; the lane still runs the arithmetic below on a point
+    // of the sphere so every lane stays in step
+    s2f_mvd[2] = (s2i_mvd[2] < R(0) ? R(-1) : R(1)) *
+                 sqrt(on_sphere ? R(1) - s2f_mvd[0] * s2f_mvd[0] - s2f_mvd[1] * s2f_mvd[1] : R(1));
+    R d[3];
+    cmc_rotate(F.Pt, s2f_mvd[0], s2f_mvd[1], s2f_mvd[2], d);
+    R e2o, e2n;
+    move_energies<R, ET, AM>(s, S, rs, ju.v, i2, h2, mc2, cd2, c0, c1, c2, d[0], d[1], d[2], e2o, e2n);
+    const R delta_energy2 = (e2n - e2o) * mc2.mu_s_SI * R(1.07828231e23);
+    const R delta_energy21 = delta_energy1 * cd1.kBTBohr + delta_energy2 * cd2.kBTBohr;
+    const double* Mr = M + 3 * (size_t)rep;
+    const double v0 = (double)F.v[0], v1 = (double)F.v[1], v2 = (double)F.v[2];
+    const double Mz_old = Mr[0] * v0 + Mr[1] * v1 + Mr[2] * v2;
+    const double Mz_new = (Mr[0] + mu1 * (double)f0 + mu2 * (double)d[0] - mu1 * (double)a0 - mu2 * (double)c0) * v0 +
+                          (Mr[1] + mu1 * (double)f1 + mu2 * (double)d[1] - mu1 * (double)a1 - mu2 * (double)c1) * v1 +
+                          (Mr[2] + mu1 * (double)f2 + mu2 * (double)d[2] - mu1 * (double)a2 - mu2 * (double)c2) * v2;
+    const double probability = exp(-(double)delta_energy21) * ((Mz_new / Mz_old) * (Mz_new / Mz_old)) *
+                               fabs((double)s2i_mvd[2] / (double)s2f_mvd[2]);
+    const U4 ua = philox4x32_10((uint32_t)local1, (uint32_t)sweep, (uint32_t)(sweep >> 32),
+                                STREAM_CMC_ACCEPT + (uint32_t)round, key.x, key.y);
+    const bool accept = on_sphere && (probability >= (double)ua.v[0] * 2.3283064365386963e-10) && (Mz_new > 0.0);
+    if (ok && accept) {
+        S.x[i1] = f0; S.y[i1] = f1; S.z[i1] = f2;
+        S.x[i2] = d[0]; S.y[i2] = d[1]; S.z[i2] = d[2];
+    }
+    const int outcome = accept ? 0 : (on_sphere ? 1 : 2);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const unsigned long long hit = __ballot(ok && outcome == k);
+        if (hit != 0ull && (int)__lane_id() == __ffsll(hit) - 1) atomicAdd(&cmc_wg_counts()[k], (unsigned)__popcll(hit));
+    }
+    __syncthreads();
+    if (threadIdx.x < 3 && cmc_wg_counts()[threadIdx.x] != 0u)
+        atomicAdd(&counts[((size_t)rep * kRejectSlots + (lb & (kRejectSlots - 1))) * kRejectStride + threadIdx.x],
+                  (double)cmc_wg_counts()[threadIdx.x]);
 }
 
 }  // namespace vmx

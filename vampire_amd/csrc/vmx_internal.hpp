@@ -51,6 +51,14 @@ struct Ops {
     // end list entry, replica's first atom}, 256 entries at most each
     static hipError_t mc_colour(hipStream_t st, int et, int jm, const SysDev<R>& s, Spin3<R> S, const int* list,
                                 const int4* cblk, int ncblk, const double* sigma, uint64_t sweep, double* rejects);
+    // constrained Monte Carlo: one round of one colour (tab[ntab] workgroups of up to 256 pairs)
+    static hipError_t cmc_pairs(hipStream_t st, int et, int jm, const SysDev<R>& s, Spin3<R> S, const double* mm,
+                                const int* list, const int4* tab, int ntab, const CmcFrame<R>* frame, const double* M,
+                                const double* sigma, int move, int colour, int round, uint64_t sweep, double* counts);
+    // each replica's sum(mu s) (double [nrep][3]): per-workgroup partials over the one-atom block
+    // table, then one workgroup per replica over its partials (rep_blk[r] .. rep_blk[r+1])
+    static hipError_t replica_moment(hipStream_t st, const SysDev<R>& s, Spin3<R> S, const double* mm,
+                                     const int* rep_blk, int nrep, double* partial, double* M);
     // deterministic fitting step and its adjoint (isotropic uniform exchange); grids of 256
     static hipError_t fit_predictor(hipStream_t st, int am, const SysDev<R>& s, const FitPrm<R>& p, Spin3<R> h,
                                     Spin3<R> S, Spin3<R> P);

@@ -113,6 +113,25 @@ void host_system_m() {
     for (int k = 0; k < 3; ++k) g_sys_m[k] = m.size() > 2 ? m[k] : 0.0;
 }
 
+// Constraint angles last sent to the device (none yet: -1e300) and the device outcome totals
+// already added to the montecarlo::cmc counters.
+double g_cmc_sent[2] = {-1.0e300, -1.0e300};
+double g_cmc_counted[3] = {0.0, 0.0, 0.0};
+
+// montecarlo::cmc counters take the device's outcome totals; the reference prints them at the end
+// of sim::run (sim.cpp:505-515), so they are brought up to date at every statistics update.
+void sync_cmc_counts() {
+    if (!g_ctx || g_cmc_sent[0] == -1.0e300) return;
+    double c[3];
+    check(vmx_cmc_get_counts(g_ctx, c), "vmx_cmc_get_counts");
+    const double d0 = c[0] - g_cmc_counted[0], d1 = c[1] - g_cmc_counted[1], d2 = c[2] - g_cmc_counted[2];
+    montecarlo::cmc::mc_success += d0;
+    montecarlo::cmc::energy_reject += d1;
+    montecarlo::cmc::sphere_reject += d2;
+    montecarlo::cmc::mc_total += d0 + d1 + d2;
+    for (int k = 0; k < 3; ++k) g_cmc_counted[k] = c[k];
+}
+
 void refuse(bool cond, const char* what) {
     if (!cond) return;
     std::cerr << "Error: the GPU backend does not compute " << what << "; run this input with "
@@ -283,6 +302,9 @@ void initialize() {
     check(vmx_set_sweep(g_ctx, (int64_t)sim::time), "vmx_set_sweep");
     check(vmx_set_spins(g_ctx, atoms::x_spin_array.data(), atoms::y_spin_array.data(), atoms::z_spin_array.data()),
           "vmx_set_spins");
+    check(vmx_mc_set_sigma(g_ctx, -1, montecarlo::internal::adaptive_sigma), "vmx_mc_set_sigma");
+    g_cmc_sent[0] = g_cmc_sent[1] = -1.0e300;
+    g_cmc_counted[0] = g_cmc_counted[1] = g_cmc_counted[2] = 0.0;
     // VMX_AUTOTUNE=1: choose the workgroup size and XCD mapping by timing a few steps of this
     // system (spins and step counter are restored afterwards)
     if (const char* a = std::getenv("VMX_AUTOTUNE")) {
@@ -345,6 +367,41 @@ void mc_step() {
     check(vmx_mc_sweep(g_ctx, 1), "vmx_mc_sweep");
 }
 
+// Integrator 3 (sim.cpp:623-628) on the device: one call is one constrained sweep of N pair
+// trials (montecarlo::cmc_step, src/montecarlo/cmc.cpp:282-497), colour-paired on the device.
+//     if(gpu::acceleration) gpu::cmc_step(); else montecarlo::cmc_step();
+// The first call after cmc::is_initialised was cleared runs CMCinit on the host spins (it sets or
+// rotates them to the constraint direction, cmc.cpp:205-273) and uploads them; every call sends
+// the constraint angles, as cmc_anisotropy changes them between temperature loops.  The move is
+// the reference's montecarlo::internal::mc_move: the adaptive cone at the device's Metropolis
+// width (the same width gpu::mc_step adapts) or the temperature-scaled angle move.
+void cmc_step() {
+    namespace mi = montecarlo::internal;
+    if (mi::algorithm != mi::adaptive && mi::algorithm != mi::angle) {
+        std::cerr << "Error: the GPU backend implements the adaptive and angle moves of constrained Monte Carlo only"
+                  << std::endl;
+        err::vexit();
+    }
+    refuse(dipole::activated, "the dipole field in constrained Monte Carlo");
+    if (!montecarlo::cmc::is_initialised) {
+        transfer_spin_positions_from_gpu_to_cpu();
+        montecarlo::CMCinit();
+        check(vmx_set_spins(g_ctx, atoms::x_spin_array.data(), atoms::y_spin_array.data(),
+                            atoms::z_spin_array.data()),
+              "vmx_set_spins");
+    }
+    if (sim::constraint_phi != g_cmc_sent[0] || sim::constraint_theta != g_cmc_sent[1]) {
+        check(vmx_cmc_set_constraint(g_ctx, -1, sim::constraint_phi, sim::constraint_theta),
+              "vmx_cmc_set_constraint");
+        g_cmc_sent[0] = sim::constraint_phi;
+        g_cmc_sent[1] = sim::constraint_theta;
+    }
+    check(vmx_cmc_set_move(g_ctx, mi::algorithm == mi::angle ? VMX_CMC_MOVE_ANGLE : VMX_CMC_MOVE_CONE),
+          "vmx_cmc_set_move");
+    push_conditions();
+    check(vmx_cmc_sweep(g_ctx, 1), "vmx_cmc_sweep");
+}
+
 void llg_midpoint() {
     push_conditions();
     check(vmx_llg_midpoint(g_ctx, 1, nullptr), "vmx_llg_midpoint");
@@ -352,6 +409,7 @@ void llg_midpoint() {
 
 void finalize() {
     if (g_ctx) {
+        sync_cmc_counts();
         transfer_spin_positions_from_gpu_to_cpu();
         check(vmx_destroy(g_ctx), "vmx_destroy");
         g_ctx = nullptr;
@@ -460,6 +518,7 @@ void update() {
     }
     derived_statistics();
     host_system_m();
+    sync_cmc_counts();
 }
 
 // Before output (src/vio/datalog.cpp:281).  With device statistics the host objects take the
