@@ -143,6 +143,34 @@ def test_c2_full_batch_replicas_equal_standalone(gpu):
 
 
 @pytest.mark.timeout(300)
+def test_c2_full_batch_metropolis_replicas_equal_standalone(gpu):
+    """The 256-replica batch under FP32 checkerboard Metropolis (the launches of a sweep mix the
+    replicas' colours to even out their sizes): after 5 batched sweeps, the smallest, the largest,
+    a middle replica and the first odd and even lattices equal the same replica swept alone with
+    key seed + r, bit for bit."""
+    systems, temps = S.config_c2()
+    batch = S.concat_systems(systems)
+    spins = S.random_spins(batch.n, 13)
+    with Context(batch, precision=FP32, seed=200) as c:
+        for r, T in enumerate(temps):
+            c.set_conditions(T, replica=r)
+        c.set_spins(spins)
+        c.mc_sweep(5)
+        got = c.get_spins()
+    off = batch.replica_offset
+    sizes = np.diff(off)
+    side = np.rint(np.cbrt(sizes)).astype(int)
+    picks = {int(np.argmin(sizes)), int(np.argmax(sizes)), len(systems) // 2,
+             int(np.argmax(side % 2 == 1)), int(np.argmax(side % 2 == 0))}
+    for r in sorted(picks):
+        with Context(systems[r], precision=FP32, seed=200 + r) as c:
+            c.set_conditions(temps[r])
+            c.set_spins(spins[:, off[r]:off[r + 1]])
+            c.mc_sweep(5)
+            assert np.array_equal(c.get_spins(), got[:, off[r]:off[r + 1]]), r
+
+
+@pytest.mark.timeout(300)
 def test_c3_full_size_adjoint_matches_finite_differences(gpu):
     """BCC 32^3 (65,536 atoms), T = 0, Hz = 1 T, 30 degree tilt, 2000 steps recorded every 100 with
     checkpoints every 100: the adjoint gradient of a trajectory loss in J, Ku and alpha equals

@@ -96,6 +96,66 @@ def test_mc_checkerboard_matches_oracle(gpu, topology):
 
 
 @pytest.mark.parametrize("precision", [FP64, FP32])
+@pytest.mark.parametrize("case", ["sc_batch", "fcc", "film_dmi", "bcc_rotated"])
+def test_mc_colour_order_equals_storage_order(gpu, case, precision):
+    """Calls of several sweeps run with the spins in colour order (each replica's atoms colour by
+    colour); single sweeps run in storage order.  Eight single sweeps and one call of eight give
+    the same spins, cone widths and rejection totals bit for bit."""
+    if case == "sc_batch":
+        s = S.concat_systems([S.ferromagnet("sc", L) for L in (5, 4, 7, 6)])
+    elif case == "fcc":
+        s = S.ferromagnet("fcc", 4)
+    elif case == "film_dmi":
+        s = RC.system_of("film_dmi")
+    else:
+        s = RC.system_of("bcc4_rotated")
+    spins = S.random_spins(s.n, 8)
+    out = []
+    for calls in ([1] * 8, [8]):
+        with Context(s, precision=precision, seed=5, topology="stencil") as c:
+            for r in range(s.num_replicas):
+                c.set_conditions(250.0 + 100.0 * r, (0.0, 0.0, 0.3), replica=r)
+            c.set_spins(spins)
+            for k in calls:
+                c.mc_sweep(k)
+            out.append((c.get_spins(), c.mc_state()))
+    assert np.array_equal(out[0][0], out[1][0])
+    for a, b in zip(out[0][1], out[1][1]):
+        assert np.array_equal(a, b)
+
+
+def test_mc_replicas_equal_standalone(gpu):
+    """A batch of even and odd periodic lattices (two colours and four): the sweep's launches carry
+    different replicas' colours side by side, each replica's colours in order, so every replica
+    equals the same lattice swept alone with seed + r, bit for bit, cone widths and rejection
+    totals included."""
+    sizes = [5, 4, 6, 3, 7, 8]
+    temps = [300.0, 500.0, 200.0, 700.0, 400.0, 600.0]
+    systems = [S.ferromagnet("sc", L) for L in sizes]
+    batch = S.concat_systems(systems)
+    spins = [S.random_spins(x.n, 60 + r) for r, x in enumerate(systems)]
+    with Context(batch, precision=FP64, seed=33) as c:
+        for r, T in enumerate(temps):
+            c.set_conditions(T, replica=r)
+        c.set_spins(np.concatenate(spins, axis=1))
+        c.mc_sweep(6)
+        got = c.get_spins()
+        gsig, gmov, grej = c.mc_state()
+    off = batch.replica_offset
+    ncol = []
+    for r, x in enumerate(systems):
+        with Context(x, precision=FP64, seed=33 + r) as c:
+            c.set_conditions(temps[r])
+            c.set_spins(spins[r])
+            c.mc_sweep(6)
+            ncol.append(c.mc_num_colours())
+            assert np.array_equal(c.get_spins(), got[:, off[r]:off[r + 1]]), r
+            sig, mov, rej = c.mc_state()
+            assert sig[0] == gsig[r] and mov[0] == gmov[r] and rej[0] == grej[r], r
+    assert min(ncol) == 2 and max(ncol) > 2
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32])
 def test_mc_curve_matches_reference(gpu, precision):
     """Statistical parity: <|m|>(T) of checkerboard Metropolis on SC 8^3 against the reference
     random-site montecarlo::mc_step (its own MT19937 stream) within combined error bars

@@ -255,17 +255,19 @@ __global__ void k_energy_finish(int nblk, const double* __restrict__ partial, in
 
 // Adaptive cone width after a sweep (src/montecarlo/mc.cpp:112-120) and running totals.  A
 // replica's rejections arrive in kRejectSlots counters (integers held in doubles: exact in any
-// order), which are summed and cleared here.
-__global__ void k_mc_adapt(int nrep, const int* __restrict__ rep_off, double* sigma, double* rejects,
-                           double* tot_moves, double* tot_rej) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+// order), which are summed and cleared here: one wavefront per replica, a slot per lane.
+static_assert(kRejectSlots == 64, "one rejection slot per lane of a wavefront");
+__global__ void __launch_bounds__(256) k_mc_adapt(int nrep, const int* __restrict__ rep_off, double* sigma,
+                                                  double* rejects, double* tot_moves, double* tot_rej) {
+    const int r = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);   // uniform in a wavefront
     if (r >= nrep) return;
+    const int lane = (int)(threadIdx.x & 63u);
+    double* slot = rejects + ((size_t)r * kRejectSlots + lane) * kRejectStride;
+    double rej = *slot;
+    *slot = 0.0;
+    for (int off = 32; off > 0; off >>= 1) rej += __shfl_xor(rej, off);
+    if (lane != 0) return;
     const double moves = (double)(rep_off[r + 1] - rep_off[r]);
-    double rej = 0.0;
-    for (int k = 0; k < kRejectSlots; ++k) {
-        rej += rejects[((size_t)r * kRejectSlots + k) * kRejectStride];
-        rejects[((size_t)r * kRejectSlots + k) * kRejectStride] = 0.0;
-    }
     const double last_rejection_rate = rej / moves;
     const double factor = 0.5 / last_rejection_rate;
     double s = sigma[r] * factor;
@@ -533,9 +535,22 @@ struct vmx_ctx {
     int cmc_move = 0;                      // 0: cone of the replica's Metropolis width; 1: angle move
     template <typename R>
     int cmc_sweeps(int nsweeps);
-    static constexpr int kMcBlock = 256;   // list entries per Metropolis workgroup
-    std::vector<int> mc_blk_start;         // colour k's workgroups: mc_blk[mc_blk_start[k] ...]
-    DevBuf colour_list, mc_blk, mc_sigma, mc_rej, mc_tot_moves, mc_tot_rej;
+    int colour_order(const std::vector<int>& c, const std::vector<int>& lau, int nc);
+    template <typename R>
+    SysDev<R> sys_colour_order() const;
+    static constexpr int kMcOrderedMinSweeps = 4;   // calls with fewer sweeps stay in storage order
+    static constexpr int kMcBlock = 256;   // pairs per constrained Monte Carlo workgroup
+    std::vector<int> mc_blk_start;         // launch l's workgroups: mc_blk[mc_blk_start[l] ...]
+    DevBuf mc_atom, mc_pat, mc_blk, mc_sigma, mc_rej, mc_tot_moves, mc_tot_rej;
+    // Metropolis in colour order (stencil rows only): each replica's atoms colour by colour,
+    // storage order within a colour.  mco_perm[p]: storage index of colour-order atom p; the
+    // stencils, material ids and caller indices (Philox counters) of the colour order; the sweep's
+    // workgroups {first, replica, end, 0} per launch; spins copied in for a call's sweeps.
+    bool mco_ok = false;
+    int mco_npat = 0, mco_full_rows = 0;
+    std::vector<int> mco_blk_start;
+    DevBuf mco_perm, mco_lidx, mco_type, mco_pat, mco_pat_off, mco_pat_jid, mco_pat_deg, mco_blk, mco_spin;
+    std::vector<uint16_t> idx16_;   // host per-edge J ids of the stencil layout (JM_IDX16)
 
     // graphs: (nsteps) -> executable
     hipGraphExec_t graph_exec = nullptr;
@@ -1038,26 +1053,112 @@ int vmx_ctx::colour() {
     // so the order is free: storage order keeps a wavefront's atoms, and their neighbours, close
     // in memory), and each replica's run is cut into workgroups of 256 entries.
     std::vector<int> list;
-    std::vector<int4> tab;
     list.reserve(n);
     colour_start.assign(nc + 1, 0);
-    mc_blk_start.assign(nc + 1, 0);
+    std::vector<int> run_first((size_t)nc * nrep), run_last((size_t)nc * nrep);
     for (int k = 0; k < nc; ++k) {
         colour_start[k] = (int)list.size();
-        mc_blk_start[k] = (int)tab.size();
         for (int r = 0; r < nrep; ++r) {
-            const int first = (int)list.size();
+            run_first[(size_t)k * nrep + r] = (int)list.size();
             for (int i = rep_off_h[r]; i < rep_off_h[r + 1]; ++i)
                 if (c[i] == k) list.push_back(i);
-            const int last = (int)list.size();
-            for (int a = first; a < last; a += kMcBlock)
-                tab.push_back(make_int4(a, r, std::min(a + kMcBlock, last), rep_off_h[r]));
+            run_last[(size_t)k * nrep + r] = (int)list.size();
         }
     }
     colour_start[nc] = (int)list.size();
+    // A sweep is nc launches.  Replicas do not interact, so only each replica's own colour order
+    // matters: replica r with k_r < nc colours may put its colours in any increasing set of
+    // launches.  Replicas with nc colours take launch k for colour k; the others, largest first,
+    // take the increasing set that keeps the largest launch smallest (then the loads most even),
+    // so that no launch is left with only the few atoms of the last colours of odd lattices.
+    std::vector<int> lau((size_t)nc * nrep);
+    {
+        std::vector<double> load(nc, 0.0);
+        std::vector<int> kr(nrep, 0);
+        for (int r = 0; r < nrep; ++r)
+            for (int k = 0; k < nc; ++k)
+                if (run_last[(size_t)k * nrep + r] > run_first[(size_t)k * nrep + r]) kr[r] = k + 1;
+        auto size = [&](int k, int r) {
+            return (double)(run_last[(size_t)k * nrep + r] - run_first[(size_t)k * nrep + r]);
+        };
+        auto sets = [&](int k) {   // number of increasing launch sets of size k
+            double b = 1.0;
+            for (int j = 0; j < k; ++j) b = b * (nc - j) / (j + 1);
+            return b;
+        };
+        std::vector<int> order;
+        for (int r = 0; r < nrep; ++r) {
+            if (kr[r] == nc || sets(kr[r]) > 4096.0) {
+                for (int k = 0; k < nc; ++k) { lau[(size_t)k * nrep + r] = k; load[k] += size(k, r); }
+            } else {
+                order.push_back(r);
+            }
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return rep_off_h[a + 1] - rep_off_h[a] > rep_off_h[b + 1] - rep_off_h[b];
+        });
+        std::vector<int> pick(nc), best(nc);
+        for (int r : order) {
+            const int k = kr[r];
+            double best_max = 1e300, best_sq = 1e300;
+            std::vector<double> trial(nc);
+            // increasing launch sets of size k, first in lexicographic order
+            for (int j = 0; j < k; ++j) pick[j] = j;
+            for (;;) {
+                double mx = 0.0, sq = 0.0;
+                trial = load;
+                for (int j = 0; j < k; ++j) trial[pick[j]] += size(j, r);
+                for (double v : trial) { mx = std::max(mx, v); sq += v * v; }
+                if (mx < best_max || (mx == best_max && sq < best_sq)) {
+                    best_max = mx; best_sq = sq;
+                    std::copy(pick.begin(), pick.begin() + k, best.begin());
+                }
+                int j = k - 1;
+                while (j >= 0 && pick[j] == nc - k + j) --j;
+                if (j < 0) break;
+                ++pick[j];
+                for (int q2 = j + 1; q2 < k; ++q2) pick[q2] = pick[q2 - 1] + 1;
+            }
+            for (int j = 0; j < k; ++j) { lau[(size_t)j * nrep + r] = best[j]; load[best[j]] += size(j, r); }
+        }
+    }
+    // A launch holds at most one colour of each replica; its workgroups go replica by replica, so
+    // the costly short runs (the last colours of odd lattices, scattered atoms with few cache
+    // hits) are spread over the launch and over the XCDs' contiguous ranges of workgroups.
+    // Workgroup w of the sweep owns entries [w * mcb, (w + 1) * mcb) of two flat arrays: the atom
+    // (~first atom of the run past the run's end) and, with stencil rows, its stencil id; mc_blk[w]
+    // = {replica, its first atom}.  A thread's first loads depend on nothing but its position.
+    const int mcb = 256 * (precision == VMX_FP64 ? McAtoms<double>::value : McAtoms<float>::value);
+    std::vector<uint16_t> pat_h;
+    if (tp == TP_PAT) {
+        pat_h.resize(n);
+        VMX_HIP(hipMemcpy(pat_h.data(), pat.p, (size_t)n * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    }
+    std::vector<int> atom_flat;
+    std::vector<uint16_t> pat_flat;
+    std::vector<int2> tab;
+    mc_blk_start.assign(nc + 1, 0);
+    for (int l = 0; l < nc; ++l) {
+        mc_blk_start[l] = (int)tab.size();
+        for (int r = 0; r < nrep; ++r)
+            for (int k = 0; k < nc; ++k) {
+                if (lau[(size_t)k * nrep + r] != l) continue;
+                const int first = run_first[(size_t)k * nrep + r], last = run_last[(size_t)k * nrep + r];
+                for (int a = first; a < last; a += mcb) {
+                    tab.push_back(make_int2(r, rep_off_h[r]));
+                    for (int e = a; e < a + mcb; ++e) {
+                        const int i = e < last ? list[e] : list[a];
+                        atom_flat.push_back(e < last ? i : ~i);
+                        if (tp == TP_PAT) pat_flat.push_back(pat_h[i]);
+                    }
+                }
+            }
+    }
     mc_blk_start[nc] = (int)tab.size();
-    VMX_TRY(upload(colour_list, list));
+    VMX_TRY(upload(mc_atom, atom_flat));
+    if (tp == TP_PAT) VMX_TRY(upload(mc_pat, pat_flat));
     VMX_TRY(upload(mc_blk, tab));
+    if (tp == TP_PAT) VMX_TRY(colour_order(c, lau, nc));
     // constrained Monte Carlo: the same colours, each replica's run in caller order, workgroups of
     // up to 256 pairs per run
     std::vector<int> clist;
@@ -1084,18 +1185,132 @@ int vmx_ctx::colour() {
     return VMX_OK;
 }
 
+namespace {
+bool build_stencils(int n, int zmax, const std::vector<int>& rowh, const std::vector<int>& colh,
+                    const std::vector<uint16_t>& jid, std::vector<uint16_t>& pat, std::vector<int>& off,
+                    std::vector<uint16_t>& pjid, std::vector<uint8_t>& pdeg, int max_stencils);
+}
+
+// The colour order of the Metropolis sweep (stencil rows): the permutation, its stencils (rows
+// keep their neighbour order, so every energy sums in the same order), material ids, caller
+// indices and per-launch workgroups.  A table that does not fit the stencil limit leaves the
+// storage-order sweep in use.
+int vmx_ctx::colour_order(const std::vector<int>& c, const std::vector<int>& lau, int nc) {
+    mco_ok = false;
+    std::vector<int> perm(n), invp(n), first((size_t)nc * nrep), last((size_t)nc * nrep);
+    int p = 0;
+    for (int r = 0; r < nrep; ++r)
+        for (int k = 0; k < nc; ++k) {
+            first[(size_t)k * nrep + r] = p;
+            for (int i = rep_off_h[r]; i < rep_off_h[r + 1]; ++i)
+                if (c[i] == k) { perm[p] = i; invp[i] = p; ++p; }
+            last[(size_t)k * nrep + r] = p;
+        }
+    std::vector<int> rowc(n + 1), colc((size_t)nnz);
+    std::vector<uint16_t> jidc(jm == JM_IDX16 ? (size_t)nnz : 0);
+    rowc[0] = 0;
+    for (int q2 = 0; q2 < n; ++q2) {
+        const int i = perm[q2], b = rowh_[i], d = rowh_[i + 1] - b;
+        rowc[q2 + 1] = rowc[q2] + d;
+        for (int k = 0; k < d; ++k) {
+            colc[rowc[q2] + k] = invp[colh_[b + k]];
+            if (jm == JM_IDX16) jidc[rowc[q2] + k] = idx16_[b + k];
+        }
+    }
+    std::vector<uint16_t> patc, pjid;
+    std::vector<int> off;
+    std::vector<uint8_t> pdeg;
+    // the colour order of lattices with an odd period has irregular rows near the seams: up to the
+    // full 16-bit range of stencil ids (the table stays in L2: 24-48 bytes per stencil)
+    if (!build_stencils(n, zmax, rowc, colc, jidc, patc, off, pjid, pdeg, 65536)) return VMX_OK;
+    // material ids in colour order; caller index within the replica (the inverse of the
+    // basis-major storage order, as caller_local computes it on the device)
+    std::vector<uint8_t> type_s(n), type_c(n);
+    VMX_HIP(hipMemcpy(type_s.data(), type.p, (size_t)n, hipMemcpyDeviceToHost));
+    std::vector<int> lidx(n);
+    for (int r = 0; r < nrep; ++r) {
+        const int m = (rep_off_h[r + 1] - rep_off_h[r]) / std::max(q, 1);
+        for (int q2 = rep_off_h[r]; q2 < rep_off_h[r + 1]; ++q2) {
+            const int i = perm[q2], loc = i - rep_off_h[r];
+            lidx[q2] = q <= 1 ? loc : (loc % m) * q + loc / m;
+            type_c[q2] = type_s[i];
+        }
+    }
+    const int mcb = 256 * (precision == VMX_FP64 ? McAtoms<double>::value : McAtoms<float>::value);
+    std::vector<int4> tab;
+    mco_blk_start.assign(nc + 1, 0);
+    for (int l = 0; l < nc; ++l) {
+        mco_blk_start[l] = (int)tab.size();
+        for (int r = 0; r < nrep; ++r)
+            for (int k = 0; k < nc; ++k) {
+                if (lau[(size_t)k * nrep + r] != l) continue;
+                const int a0 = first[(size_t)k * nrep + r], a1 = last[(size_t)k * nrep + r];
+                for (int a = a0; a < a1; a += mcb) tab.push_back(make_int4(a, r, std::min(a + mcb, a1), 0));
+            }
+    }
+    mco_blk_start[nc] = (int)tab.size();
+    VMX_TRY(upload(mco_perm, perm));
+    VMX_TRY(upload(mco_lidx, lidx));
+    VMX_TRY(upload(mco_type, type_c));
+    VMX_TRY(upload(mco_pat, patc));
+    VMX_TRY(upload(mco_pat_off, off));
+    VMX_TRY(upload(mco_pat_jid, pjid));
+    VMX_TRY(upload(mco_pat_deg, pdeg));
+    VMX_TRY(upload(mco_blk, tab));
+    VMX_TRY(mco_spin.alloc(3 * (size_t)n * rsize()));
+    mco_npat = (int)pdeg.size();
+    mco_full_rows = std::all_of(pdeg.begin(), pdeg.end(), [this](uint8_t d) { return d == zmax; }) ? 1 : 0;
+    mco_ok = true;
+    return VMX_OK;
+}
+
+// The system description of the colour order: the storage order's with the colour order's
+// stencils and material ids.
+template <typename R>
+SysDev<R> vmx_ctx::sys_colour_order() const {
+    SysDev<R> s = sys<R>();
+    s.npat = mco_npat;
+    s.full_rows = mco_full_rows;
+    s.pat_lds = ((size_t)mco_npat * zmax * sizeof(int) <= 16384 && fused_lds_bytes<R>(nmat, mco_npat, zmax, 1) <= 65536)
+                    ? 1 : 0;
+    s.pat = mco_pat.as<uint16_t>();
+    s.pat_off = mco_pat_off.as<int>();
+    s.pat_jid = mco_pat_jid.as<uint16_t>();
+    s.pat_deg = mco_pat_deg.as<uint8_t>();
+    s.type = mco_type.as<uint8_t>();
+    return s;
+}
+
 template <typename R>
 int vmx_ctx::mc_sweeps(int nsweeps) {
     if (!coloured) VMX_TRY(colour());
     const SysDev<R> s = sys<R>();
     const Spin3<R> S = spins<R>(s_idx);
+    // Calls of several sweeps without the dipole field run in colour order: the spins go there
+    // once and come back at the end.  Both orders give the same results bit for bit (the same
+    // moves, counters and arithmetic; only the memory layout differs).
+    const bool ordered = mco_ok && !dip.on && nsweeps >= kMcOrderedMinSweeps;
+    const SysDev<R> so = ordered ? sys_colour_order<R>() : s;
+    Spin3<R> C{};
+    if (ordered) {
+        R* base = mco_spin.as<R>();
+        C = Spin3<R>{base, base + n, base + 2 * (size_t)n};
+        VMX_HIP(Ops<R>::mc_permute(stream, n, mco_perm.as<int>(), S, C, true));
+    }
     for (int w = 0; w < nsweeps; ++w) {
-        for (int k = 0; k < ncolours; ++k) {
-            VMX_HIP(Ops<R>::mc_colour(stream, et, am(), s, S, colour_list.as<int>(), mc_blk.as<int4>() + mc_blk_start[k],
-                                      mc_blk_start[k + 1] - mc_blk_start[k], mc_sigma.as<double>(), sweep,
+        for (int k = 0; k < ncolours; ++k) {   // launch k (each replica's colours in order)
+            if (ordered) {
+                VMX_HIP(Ops<R>::mc_colour_ordered(stream, et, am(), so, C, mco_lidx.as<int>(), mco_blk.as<int4>(),
+                                                  mco_blk_start[k], mco_blk_start[k + 1] - mco_blk_start[k],
+                                                  mc_sigma.as<double>(), sweep, mc_rej.as<double>()));
+                continue;
+            }
+            VMX_HIP(Ops<R>::mc_colour(stream, et, am(), s, S, mc_atom.as<int>(), mc_pat.as<uint16_t>(),
+                                      mc_blk.as<int2>(), mc_blk_start[k], mc_blk_start[k + 1] - mc_blk_start[k],
+                                      mc_sigma.as<double>(), sweep,
                                       mc_rej.as<double>()));
         }
-        hipLaunchKernelGGL(k_mc_adapt, dim3((nrep + 255) / 256), dim3(256), 0, stream, nrep, rep_off.as<int>(),
+        hipLaunchKernelGGL(k_mc_adapt, dim3((nrep + 3) / 4), dim3(256), 0, stream, nrep, rep_off.as<int>(),
                            mc_sigma.as<double>(), mc_rej.as<double>(), mc_tot_moves.as<double>(),
                            mc_tot_rej.as<double>());
         VMX_HIP(hipGetLastError());
@@ -1103,6 +1318,7 @@ int vmx_ctx::mc_sweeps(int nsweeps) {
         // a sweep advances sim::time like a Heun step (sim.cpp:608-615, 191-200)
         if (dip.on && ++dip.time % dip.rate == 0) VMX_TRY(dipole_refresh<R>());
     }
+    if (ordered) VMX_HIP(Ops<R>::mc_permute(stream, n, mco_perm.as<int>(), C, S, false));
     return VMX_OK;
 }
 
@@ -1351,7 +1567,7 @@ constexpr int kMaxStencils = 4096;
 
 bool build_stencils(int n, int zmax, const std::vector<int>& rowh, const std::vector<int>& colh,
                     const std::vector<uint16_t>& jid, std::vector<uint16_t>& pat, std::vector<int>& off,
-                    std::vector<uint16_t>& pjid, std::vector<uint8_t>& pdeg) {
+                    std::vector<uint16_t>& pjid, std::vector<uint8_t>& pdeg, int max_stencils) {
     std::unordered_map<std::string, int> ids;
     pat.resize(n);
     std::string key;
@@ -1370,7 +1586,7 @@ bool build_stencils(int n, int zmax, const std::vector<int>& rowh, const std::ve
         int p;
         if (it == ids.end()) {
             p = (int)ids.size();
-            if (p >= kMaxStencils) return false;
+            if (p >= max_stencils) return false;
             ids.emplace(key, p);
             off.resize((size_t)(p + 1) * zmax, 0);
             pjid.resize((size_t)(p + 1) * zmax, 0);
@@ -1445,7 +1661,7 @@ static int build_topology(vmx_ctx* c, int request, const std::vector<int>& rowh,
         std::vector<int> off;
         std::vector<uint8_t> pdeg;
         if (build_stencils(n, W, rowh, colh, c->jm == JM_IDX16 ? idx16 : std::vector<uint16_t>(), pat, off, pjid,
-                           pdeg)) {
+                           pdeg, kMaxStencils)) {
             c->tp = TP_PAT;
             c->wsel = wsel;
             c->zmax = W;
@@ -1661,6 +1877,7 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
 
     const size_t rs = c->rsize();
     VMX_TRY(build_topology(c.get(), opt->topology, rowh, colh, idx16, idx32));
+    if (c->tp == TP_PAT && c->jm == JM_IDX16) c->idx16_ = idx16;
     c->rowh_.swap(rowh);
     c->colh_.swap(colh);
     VMX_TRY(c->jtab.alloc(table.size() * rs));

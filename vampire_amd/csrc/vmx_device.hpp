@@ -1279,81 +1279,178 @@ __device__ __forceinline__ bool mc_accept(float x, uint32_t word) {
     return __expf(-x) >= (float)word * 2.3283064365386963e-10f;
 }
 
-// Workgroup b of a colour takes the list entries [blk.x, blk.z) of that colour, all of replica
-// blk.y (whose first atom is blk.w): the replica is uniform, its conditions and cone width are
-// read once, and the tables are staged in LDS as in the fused LLG kernels.  A thread forms its
-// move from Philox while its own loads are in flight, gathers its neighbours once, and forms the
-// energy before and after the move from the same registers, each in the oracle's order.
-template <typename R, int ET, int AM>
-__global__ void __launch_bounds__(256)
-k_mc_colour(SysDev<R> s, Spin3<R> S, const int* __restrict__ list, const int4* __restrict__ cblk, int ncblk,
-            const double* __restrict__ sigma, uint64_t sweep, double* __restrict__ rejects) {
+// Atoms per thread of the Metropolis colour kernel: a workgroup takes 256 * McAtoms list entries.
+template <typename R> struct McAtoms { static constexpr int value = 2; };
+
+// The moves of a thread's U atoms idx[] of replica rep (ok[u]: a real entry; head[u]: its stencil
+// id; local[u]: its caller index within the replica, the Philox counter).  The thread forms the
+// moves from Philox while its own loads are in flight, gathers the neighbours of all its atoms
+// before it stores anything (atoms of one colour do not interact, so no gather can see another
+// lane's store), and forms each energy before and after the move from the same registers, in the
+// oracle's order.  Every thread of the workgroup reaches the LDS staging and the final barrier.
+template <typename R, int ET, int AM, int U>
+__device__ __forceinline__ void mc_moves(const SysDev<R>& s, const Spin3<R>& S, int rep, int lb, const int (&idx)[U],
+                                         const bool (&ok)[U], const int (&head)[U], const int (&local)[U],
+                                         const double* __restrict__ sigma, uint64_t sweep,
+                                         double* __restrict__ rejects) {
     constexpr bool kStaged = Access<AM>::TP != TP_CSR;
-    const int lb = logical_block(s.xcd_remap);
-    if (lb >= ncblk) return;                       // whole workgroup
-    const int4 b = cblk[lb];
-    const int rep = b.y;
-    const int t = b.x + (int)threadIdx.x;
-    const bool ok = t < b.z;
-    const int i = list[ok ? t : b.x];
-    const int m = s.nmat == 1 ? 0 : (int)s.type[i];
-    const R ox = S.x[i], oy = S.y[i], oz = S.z[i];
-    int head = 0;
-    if (kStaged) head = row_head<R, AM>(s, i);
-    const int local = caller_local(s, rep, i - b.w);
+    int m[U];
+    R ox[U], oy[U], oz[U];
+    const SpinRsrc<R> rs(s, S.x, S.y, S.z);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        m[u] = s.nmat == 1 ? 0 : (int)s.type[idx[u]];
+        rs.load(idx[u], ox[u], oy[u], oz[u]);
+    }
     const uint2 key = s.rep_key[rep];
-    const U4 um = philox4x32_10((uint32_t)local, (uint32_t)sweep, (uint32_t)(sweep >> 32), STREAM_MC_MOVE, key.x,
-                                key.y);
-    R g0, g1, g2;
-    box_muller3(um, g0, g1, g2);
+    R gx[U], gy[U], gz[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const U4 um = philox4x32_10((uint32_t)local[u], (uint32_t)sweep, (uint32_t)(sweep >> 32), STREAM_MC_MOVE,
+                                    key.x, key.y);
+        box_muller3(um, gx[u], gy[u], gz[u]);
+    }
     const R angle = (R)sigma[rep];
     if (threadIdx.x == 0) mc_wg_rejects() = 0u;
     stage_fused<R, AM>(s, rep);
-    const MatConst<R>& mc = FusedLds<R>::mat()[m];
-    const Cond<R>& c = FusedLds<R>::cond(s)[m];
-    R nx = ox + g0 * angle, ny = oy + g1 * angle, nz = oz + g2 * angle;
-    const R r = inv_norm(nx, ny, nz);
-    nx *= r; ny *= r; nz *= r;
-    R x_old, x_new;
+    const UniformJ<R, ET> ju(s, Access<AM>::JM);
+    R nx[U], ny[U], nz[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        nx[u] = ox[u] + gx[u] * angle; ny[u] = oy[u] + gy[u] * angle; nz[u] = oz[u] + gz[u] * angle;
+        const R r = inv_norm(nx[u], ny[u], nz[u]);
+        nx[u] *= r; ny[u] *= r; nz[u] *= r;
+    }
+    R x_old[U], x_new[U];
     if constexpr (kStaged) {
-        const SpinRsrc<R> rs(s, S.x, S.y, S.z);
-        const UniformJ<R, ET> ju(s, Access<AM>::JM);
-        RowIdx<AM> ri;
-        RowSpins<R, AM> rv;
-        fused_row_index(s, i, head, ri);
-        row_load(rs, ri, rv);
-        x_old = row_exchange_energy<R, ET, AM>(s, ri, rv, ju.v, ox, oy, oz);
-        x_new = row_exchange_energy<R, ET, AM>(s, ri, rv, ju.v, nx, ny, nz);
+        RowIdx<AM> ri[U];
+        RowSpins<R, AM> rv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            fused_row_index(s, idx[u], head[u], ri[u]);
+            row_load(rs, ri[u], rv[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            x_old[u] = row_exchange_energy<R, ET, AM>(s, ri[u], rv[u], ju.v, ox[u], oy[u], oz[u]);
+            x_new[u] = row_exchange_energy<R, ET, AM>(s, ri[u], rv[u], ju.v, nx[u], ny[u], nz[u]);
+        }
     } else {
-        x_old = exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, ox, oy, oz);
-        x_new = exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, nx, ny, nz);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            x_old[u] = exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, idx[u], ox[u], oy[u], oz[u]);
+            x_new[u] = exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, idx[u], nx[u], ny[u], nz[u]);
+        }
     }
-    // sim::calculate_spin_energy order: exchange, anisotropy, applied, magnetostatic (energy.cpp:147-174)
-    R e_old = (R(0) + x_old + anisotropy_energy(s.ani, mc, ox, oy, oz)) + applied_energy(c, ox, oy, oz);
-    R e_new = (R(0) + x_new + anisotropy_energy(s.ani, mc, nx, ny, nz)) + applied_energy(c, nx, ny, nz);
-    if (s.hd) {
-        e_old += magnetostatic_energy(s, i, ox, oy, oz);
-        e_new += magnetostatic_energy(s, i, nx, ny, nz);
+    bool accept[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const MatConst<R>& mc = FusedLds<R>::mat()[m[u]];
+        const Cond<R>& c = FusedLds<R>::cond(s)[m[u]];
+        // sim::calculate_spin_energy order: exchange, anisotropy, applied, magnetostatic
+        // (energy.cpp:147-174)
+        R e_old = (R(0) + x_old[u] + anisotropy_energy(s.ani, mc, ox[u], oy[u], oz[u])) +
+                  applied_energy(c, ox[u], oy[u], oz[u]);
+        R e_new = (R(0) + x_new[u] + anisotropy_energy(s.ani, mc, nx[u], ny[u], nz[u])) +
+                  applied_energy(c, nx[u], ny[u], nz[u]);
+        if (s.hd) {
+            e_old += magnetostatic_energy(s, idx[u], ox[u], oy[u], oz[u]);
+            e_new += magnetostatic_energy(s, idx[u], nx[u], ny[u], nz[u]);
+        }
+        const R de = (e_new - e_old) * mc.mu_s_SI * R(1.07828231e23);
+        accept[u] = de < R(0);
+        if (!accept[u]) {
+            const U4 ua = philox4x32_10((uint32_t)local[u], (uint32_t)sweep, (uint32_t)(sweep >> 32),
+                                        STREAM_MC_ACCEPT, key.x, key.y);
+            accept[u] = mc_accept(de * c.kBTBohr, ua.v[0]);
+        }
     }
-    const R de = (e_new - e_old) * mc.mu_s_SI * R(1.07828231e23);
-    bool accept = de < R(0);
-    if (!accept) {
-        const U4 ua = philox4x32_10((uint32_t)local, (uint32_t)sweep, (uint32_t)(sweep >> 32), STREAM_MC_ACCEPT,
-                                    key.x, key.y);
-        accept = mc_accept(de * c.kBTBohr, ua.v[0]);
-    }
-    if (ok && accept) {
-        S.x[i] = nx; S.y[i] = ny; S.z[i] = nz;
+    unsigned int wave_rejects = 0u;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (ok[u] && accept[u]) {
+            S.x[idx[u]] = nx[u]; S.y[idx[u]] = ny[u]; S.z[idx[u]] = nz[u];
+        }
+        wave_rejects += (unsigned int)__popcll(__ballot(ok[u] && !accept[u]));
     }
     // rejections of the replica: wavefront counts summed in LDS, then one atomic per workgroup
     // (counts are integers, exact in a double in any order)
-    const unsigned long long rejected = __ballot(ok && !accept);
-    if (rejected != 0ull && (int)__lane_id() == __ffsll(rejected) - 1)
-        atomicAdd(&mc_wg_rejects(), (unsigned int)__popcll(rejected));
+    if (wave_rejects != 0u && __lane_id() == 0u) atomicAdd(&mc_wg_rejects(), wave_rejects);
     __syncthreads();
     if (threadIdx.x == 0 && mc_wg_rejects() != 0u)
         atomicAdd(&rejects[((size_t)rep * kRejectSlots + (lb & (kRejectSlots - 1))) * kRejectStride],
                   (double)mc_wg_rejects());
+}
+
+// Storage-order launch: workgroup w of a sweep takes U * 256 entries (U = McAtoms) of the flat
+// launch arrays: atoms of one colour of replica wg[w].x (whose first atom is wg[w].y) and, with
+// stencil rows, their stencil ids.  The replica is uniform, its conditions and cone width are read
+// once, and the tables are staged in LDS as in the fused LLG kernels.  A thread's entries are t,
+// t + 256, ...: their atom and stencil id arrive together, the atom's spin and its stencil offsets
+// next, its neighbours last.  Entries past a run's end hold ~(the run's first atom): they load
+// harmlessly and store nothing.
+template <typename R, int ET, int AM>
+__global__ void __launch_bounds__(256)
+k_mc_colour(SysDev<R> s, Spin3<R> S, const int* __restrict__ atom, const uint16_t* __restrict__ pat,
+            const int2* __restrict__ wg, int wg0, int nwg, const double* __restrict__ sigma, uint64_t sweep,
+            double* __restrict__ rejects) {
+    constexpr int U = McAtoms<R>::value;
+    const int lb = logical_block(s.xcd_remap);
+    if (lb >= nwg) return;                         // whole workgroup
+    const size_t base = (size_t)(wg0 + lb) * (256 * U) + threadIdx.x;
+    int idx[U], head[U], local[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int a = atom[base + u * 256];
+        ok[u] = a >= 0;
+        idx[u] = ok[u] ? a : ~a;
+        head[u] = Access<AM>::TP == TP_PAT ? (int)pat[base + u * 256] : 0;
+    }
+    const int2 b = wg[wg0 + lb];
+#pragma unroll
+    for (int u = 0; u < U; ++u) local[u] = caller_local(s, b.x, idx[u] - b.y);
+    mc_moves<R, ET, AM, U>(s, S, b.x, lb, idx, ok, head, local, sigma, sweep, rejects);
+}
+
+// Colour-order launch: the spins, stencil ids and material ids of s and S are in colour order
+// (each replica's atoms colour by colour, storage order within a colour), so workgroup w takes
+// the contiguous atoms [wg[w].x, wg[w].z) of replica wg[w].y, all of one colour, and their
+// neighbours' loads are contiguous spans too.  lidx: each atom's caller index within its replica.
+// Same moves, counters and arithmetic as the storage-order launch.
+template <typename R, int ET, int AM>
+__global__ void __launch_bounds__(256)
+k_mc_colour_ordered(SysDev<R> s, Spin3<R> S, const int* __restrict__ lidx, const int4* __restrict__ wg, int wg0,
+                    int nwg, const double* __restrict__ sigma, uint64_t sweep, double* __restrict__ rejects) {
+    constexpr int U = McAtoms<R>::value;
+    const int lb = logical_block(s.xcd_remap);
+    if (lb >= nwg) return;                         // whole workgroup
+    const int4 b = wg[wg0 + lb];
+    int idx[U], head[U], local[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int t = b.x + (int)threadIdx.x + u * 256;
+        ok[u] = t < b.z;
+        idx[u] = ok[u] ? t : b.x;
+        head[u] = row_head<R, AM>(s, idx[u]);
+        local[u] = lidx[idx[u]];
+    }
+    mc_moves<R, ET, AM, U>(s, S, b.y, lb, idx, ok, head, local, sigma, sweep, rejects);
+}
+
+// Spins between storage order and colour order: dst[p] = src[perm[p]] (to colour order) or
+// dst[perm[p]] = src[p] (back).
+template <typename R>
+__global__ void k_mc_permute(int n, const int* __restrict__ perm, Spin3<R> src, Spin3<R> dst, int to_colour) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const int i = perm[p];
+        if (to_colour) {
+            dst.x[p] = src.x[i]; dst.y[p] = src.y[i]; dst.z[p] = src.z[i];
+        } else {
+            dst.x[i] = src.x[p]; dst.y[i] = src.y[p]; dst.z[i] = src.z[p];
+        }
+    }
 }
 
 // ======================= constrained Monte Carlo ===============================================

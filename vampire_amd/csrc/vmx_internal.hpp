@@ -47,10 +47,19 @@ struct Ops {
     static hipError_t energy_partial(int grid, hipStream_t st, int et, int jm, const SysDev<R>& s, Spin3<R> S,
                                      const int* mask, const double* mm, const int* atom_rep, int nbins,
                                      double* partial);
-    // Monte Carlo: one colour; cblk[ncblk] are the colour's workgroups {first list entry, replica,
-    // end list entry, replica's first atom}, 256 entries at most each
-    static hipError_t mc_colour(hipStream_t st, int et, int jm, const SysDev<R>& s, Spin3<R> S, const int* list,
-                                const int4* cblk, int ncblk, const double* sigma, uint64_t sweep, double* rejects);
+    // Monte Carlo: one launch of a sweep, workgroups wg0 .. wg0 + nwg - 1; workgroup w takes
+    // entries [w * 256 * McAtoms, ...) of atom (and pat, the stencil ids) and wg[w] = {replica,
+    // replica's first atom}
+    // Monte Carlo, colour order: workgroup w takes the atoms [wg[w].x, wg[w].z) of replica wg[w].y
+    static hipError_t mc_colour_ordered(hipStream_t st, int et, int jm, const SysDev<R>& s, Spin3<R> S,
+                                        const int* lidx, const int4* wg, int wg0, int nwg, const double* sigma,
+                                        uint64_t sweep, double* rejects);
+    // spins to colour order (dst[p] = src[perm[p]]) or back (dst[perm[p]] = src[p])
+    static hipError_t mc_permute(hipStream_t st, int n, const int* perm, Spin3<R> src, Spin3<R> dst,
+                                 bool to_colour);
+    static hipError_t mc_colour(hipStream_t st, int et, int jm, const SysDev<R>& s, Spin3<R> S, const int* atom,
+                                const uint16_t* pat, const int2* wg, int wg0, int nwg, const double* sigma,
+                                uint64_t sweep, double* rejects);
     // constrained Monte Carlo: one round of one colour (tab[ntab] workgroups of up to 256 pairs)
     static hipError_t cmc_pairs(hipStream_t st, int et, int jm, const SysDev<R>& s, Spin3<R> S, const double* mm,
                                 const int* list, const int4* tab, int ntab, const CmcFrame<R>* frame, const double* M,
