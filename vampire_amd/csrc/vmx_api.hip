@@ -152,9 +152,10 @@ __global__ void k_advance_step(unsigned long long* base, unsigned int n) { *base
 
 // adds dL/dS_z (the same for every atom) to the adjoint of a recorded state
 template <typename R>
-__global__ void k_lam_inject(int n, R* __restrict__ lz, R inj) {
+__global__ void k_lam_inject(int n, R* __restrict__ lz, const R* __restrict__ inj) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) lz[i] += inj;
+    const R v = *inj;
+    if (i < n && v != R(0)) lz[i] += v;
 }
 
 template <typename R>
@@ -566,6 +567,7 @@ struct vmx_ctx {
         for (auto* e : energies) delete e;
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
+        fit.drop_graph();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -801,6 +803,19 @@ struct vmx_ctx {
         double J = 0, k = 0, alpha = 0, gamma_rel = 1, H[3] = {0, 0, 0};
         bool has_h = false;
         DevBuf ckpt, seg, P, lam, lH1, lPd, lT, H0, sumS, lH0, lSp, dh, nhat, h, acc, mz;
+        // backward: dL/dm_z injections per step (inj[q], zero between records) and the slice of
+        // the segment being replayed (inj_seg[K]); the graph of one full segment, captured by
+        // each backward call (its kernels take that call's buffers and parameters) and replayed
+        // for every full segment
+        DevBuf inj, inj_seg;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t graph_exec = nullptr;
+        void drop_graph() {
+            if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            graph_exec = nullptr;
+            graph = nullptr;
+        }
     } fit;
     bool symmetric_checked = false, symmetric = false;
     template <typename R>
@@ -1438,7 +1453,8 @@ int vmx_ctx::fit_backward(const double* dmz, double* grad, double* dh_out) {
     for (DevBuf* b : {&fit.lam, &fit.lH1, &fit.lPd, &fit.lT, &fit.H0, &fit.sumS, &fit.lH0, &fit.lSp, &fit.dh})
         VMX_TRY(b->alloc(N3 * sizeof(R)));
     VMX_TRY(fit.nhat.alloc((size_t)n * sizeof(R)));
-    VMX_TRY(fit.acc.alloc(4 * sizeof(double)));
+    const int nwg = (n + 255) / 256;   // workgroups of the backward kernels: one slot of 4 sums each
+    VMX_TRY(fit.acc.alloc((size_t)4 * nwg * sizeof(double)));
     const SysDev<R> s = sys<R>();
     const FitPrm<R> p = fit_prm<R>();
     const Spin3<R> hh = fit.has_h ? arr3<R>(fit.h) : Spin3<R>{nullptr, nullptr, nullptr};
@@ -1446,41 +1462,85 @@ int vmx_ctx::fit_backward(const double* dmz, double* grad, double* dh_out) {
     const int fill_grid = (int)std::min<size_t>((N3 + 255) / 256, 4096);
     hipLaunchKernelGGL((k_fill<R>), dim3(fill_grid), dim3(256), 0, stream, N3, fit.lam.as<R>(), (R)0);
     hipLaunchKernelGGL((k_fill<R>), dim3(fill_grid), dim3(256), 0, stream, N3, fit.dh.as<R>(), (R)0);
-    hipLaunchKernelGGL((k_fill<double>), dim3(1), dim3(64), 0, stream, (size_t)4, fit.acc.as<double>(), 0.0);
+    hipLaunchKernelGGL((k_fill<double>), dim3(std::min(nwg, 1024)), dim3(64), 0, stream, (size_t)4 * nwg,
+                       fit.acc.as<double>(), 0.0);
     VMX_HIP(hipGetLastError());
     R* ck = fit.ckpt.as<R>();
     R* seg = fit.seg.as<R>();
     const Spin3<R> lam = arr3<R>(fit.lam);
     const int nseg = (fit.nsteps + fit.K - 1) / fit.K;
+    // dL/dm_z enters lambda_z after step q when q + 1 is a recorded step (zero elsewhere)
+    {
+        std::vector<R> inj((size_t)fit.nsteps, R(0));
+        for (int q = 0; q < fit.nsteps; ++q)
+            if ((q + 1) % fit.record_every == 0) inj[q] = (R)(dmz[(q + 1) / fit.record_every - 1] / n);
+        VMX_TRY(upload(fit.inj, inj));
+        VMX_TRY(fit.inj_seg.alloc((size_t)fit.K * sizeof(R)));
+    }
+    // One segment: recompute its states from the checkpoint in seg[0], then step back over them.
+    // inj: the segment's injections (inj[t] after local step t).  The same sequence for every
+    // full segment, so it is captured once and replayed.
+    auto segment = [&](int len, const R* inj) -> hipError_t {
+        for (int t = 0; t < len; ++t) {
+            R* a = seg + (size_t)t * N3;
+            hipError_t e = Ops<R>::fit_predictor(stream, am(), s, p, hh, state(a), arr3<R>(fit.P));
+            if (e == hipSuccess) e = Ops<R>::fit_corrector(stream, am(), s, p, hh, state(a), arr3<R>(fit.P),
+                                                          state(a + N3));
+            if (e != hipSuccess) return e;
+        }
+        for (int t = len - 1; t >= 0; --t) {
+            hipLaunchKernelGGL((k_lam_inject<R>), dim3((n + 255) / 256), dim3(256), 0, stream, n, lam.z, inj + t);
+            hipError_t e = hipGetLastError();
+            R* a = seg + (size_t)t * N3;
+            if (e == hipSuccess) e = Ops<R>::fit_predictor(stream, am(), s, p, hh, state(a), arr3<R>(fit.P));
+            if (e == hipSuccess)
+                e = Ops<R>::adj_b1(stream, am(), s, p, hh, state(a), arr3<R>(fit.P), lam, arr3<R>(fit.lH1),
+                                   arr3<R>(fit.lPd), arr3<R>(fit.lT), arr3<R>(fit.H0), arr3<R>(fit.sumS),
+                                   fit.nhat.as<R>(), fit.acc.as<double>());
+            if (e == hipSuccess)
+                e = Ops<R>::adj_b2(stream, am(), s, p, state(a), arr3<R>(fit.P), arr3<R>(fit.lH1), arr3<R>(fit.lPd),
+                                   arr3<R>(fit.lT), arr3<R>(fit.H0), arr3<R>(fit.sumS), fit.nhat.as<R>(),
+                                   arr3<R>(fit.lH0), arr3<R>(fit.lSp), arr3<R>(fit.dh), fit.acc.as<double>());
+            if (e == hipSuccess) e = Ops<R>::adj_b3(stream, am(), s, p, arr3<R>(fit.lH0), arr3<R>(fit.lSp), lam, (R)0);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    // full segments replay one graph when there are at least two of them
+    const int nfull = fit.nsteps / fit.K;
+    const bool use_graph = nfull >= 2;
+    if (use_graph) {
+        fit.drop_graph();
+        VMX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t e = segment(fit.K, fit.inj_seg.as<R>());
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(stream, &g);
+        if (e != hipSuccess || ec != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(VMX_ERR_HIP, "adjoint segment capture failed: %s",
+                        hipGetErrorString(e != hipSuccess ? e : ec));
+        }
+        fit.graph = g;
+        VMX_HIP(hipGraphInstantiate(&fit.graph_exec, fit.graph, nullptr, nullptr, 0));
+    }
     for (int c = nseg - 1; c >= 0; --c) {
         const int start = c * fit.K, end = std::min(start + fit.K, fit.nsteps);
-        // recompute the states of this segment from its checkpoint
         VMX_HIP(hipMemcpyAsync(seg, ck + (size_t)c * N3, N3 * sizeof(R), hipMemcpyDeviceToDevice, stream));
-        for (int t = 0; t < end - start; ++t) {
-            R* a = seg + (size_t)t * N3;
-            VMX_HIP(Ops<R>::fit_predictor(stream, am(), s, p, hh, state(a), arr3<R>(fit.P)));
-            VMX_HIP(Ops<R>::fit_corrector(stream, am(), s, p, hh, state(a), arr3<R>(fit.P), state(a + N3)));
-        }
-        for (int q = end - 1; q >= start; --q) {
-            if ((q + 1) % fit.record_every == 0) {
-                const double inj = dmz[(q + 1) / fit.record_every - 1] / n;
-                hipLaunchKernelGGL((k_lam_inject<R>), dim3((n + 255) / 256), dim3(256), 0, stream, n, lam.z, (R)inj);
-                VMX_HIP(hipGetLastError());
-            }
-            R* a = seg + (size_t)(q - start) * N3;
-            VMX_HIP(Ops<R>::fit_predictor(stream, am(), s, p, hh, state(a), arr3<R>(fit.P)));
-            VMX_HIP(Ops<R>::adj_b1(stream, am(), s, p, hh, state(a), arr3<R>(fit.P), lam, arr3<R>(fit.lH1),
-                                   arr3<R>(fit.lPd), arr3<R>(fit.lT), arr3<R>(fit.H0), arr3<R>(fit.sumS),
-                                   fit.nhat.as<R>(), fit.acc.as<double>()));
-            VMX_HIP(Ops<R>::adj_b2(stream, am(), s, p, state(a), arr3<R>(fit.P), arr3<R>(fit.lH1), arr3<R>(fit.lPd),
-                                   arr3<R>(fit.lT), arr3<R>(fit.H0), arr3<R>(fit.sumS), fit.nhat.as<R>(),
-                                   arr3<R>(fit.lH0), arr3<R>(fit.lSp), arr3<R>(fit.dh), fit.acc.as<double>()));
-            VMX_HIP(Ops<R>::adj_b3(stream, am(), s, p, arr3<R>(fit.lH0), arr3<R>(fit.lSp), lam, (R)0));
+        if (use_graph && end - start == fit.K) {
+            VMX_HIP(hipMemcpyAsync(fit.inj_seg.p, fit.inj.as<R>() + start, (size_t)fit.K * sizeof(R),
+                                   hipMemcpyDeviceToDevice, stream));
+            VMX_HIP(hipGraphLaunch(fit.graph_exec, stream));
+        } else {
+            VMX_HIP(segment(end - start, fit.inj.as<R>() + start));
         }
     }
-    double acc[4];
     VMX_HIP(hipStreamSynchronize(stream));
-    VMX_HIP(hipMemcpy(acc, fit.acc.p, sizeof acc, hipMemcpyDeviceToHost));
+    fit.drop_graph();   // its replays have finished
+    std::vector<double> part((size_t)4 * nwg);
+    VMX_HIP(hipMemcpy(part.data(), fit.acc.p, part.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int w = 0; w < nwg; ++w)
+        for (int k = 0; k < 4; ++k) acc[k] += part[4 * (size_t)w + k];
     // a = -gamma/(1+alpha^2), b = alpha a
     const double al = fit.alpha, g = fit.gamma_rel;
     const double a = -g / (1.0 + al * al);

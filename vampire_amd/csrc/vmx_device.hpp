@@ -114,6 +114,20 @@ struct Spin3 {
     R* __restrict__ z;
 };
 
+// The spins a corrector reads (S) and writes (O) when the step overwrites S in place: no
+// __restrict__, since the two may be the same arrays.  A thread reads S only at its own atoms
+// before it stores them (the in-place steps gather their neighbours from the predicted spins).
+template <typename R>
+struct Spin3Alias {
+    R* x;
+    R* y;
+    R* z;
+};
+template <typename R>
+__host__ __device__ inline Spin3Alias<R> alias(const Spin3<R>& a) {
+    return Spin3Alias<R>{a.x, a.y, a.z};
+}
+
 // Parameters of the deterministic fitting step (vmx_fit.inc): exchange J (T) of every bond,
 // uniaxial k (T) along e, LLG prefactors a = -gamma/(1+alpha^2), b = alpha a, uniform field H.
 template <typename R>
@@ -709,7 +723,7 @@ k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, Spin3<R> E, c
 // Own data of all the thread's atoms first, as above.
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(512)
-k_fused_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> D, Spin3<R> E,
+k_fused_corrector(SysDev<R> s, Spin3Alias<R> S, Spin3<R> P, Spin3Alias<R> O, Spin3<R> D, Spin3<R> E,
                   const double* __restrict__ noise, uint32_t step_off) {
     constexpr int U = 1;                          // launched on the one-atom block table
     constexpr bool kStaged = Access<AM>::TP != TP_CSR;
@@ -894,7 +908,7 @@ k_mid_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> E, const double* _
 // thread, like the Heun corrector, which loads as much own data.
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(512)
-k_mid_corrector(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> O, Spin3<R> E, const double* __restrict__ noise,
+k_mid_corrector(SysDev<R> s, Spin3Alias<R> S, Spin3<R> P, Spin3Alias<R> O, Spin3<R> E, const double* __restrict__ noise,
                 uint32_t step_off) {
     constexpr int U = 1;                          // launched on the one-atom block table
     constexpr bool kStaged = Access<AM>::TP != TP_CSR;
