@@ -4,10 +4,13 @@ HBM roofline of the Heun step and the reference's own CPU Heun step timed on the
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--L 100] [--cpu-seconds 10] [--no-cpu]
                     [--batching-steps 200] [--legs fp32_fused,fp64_fused,fp64_plain,fp32_midpoint]
+                    [--mc-sweeps 200] [--configs c0,c3,c4]
 
 One "step" is one Heun step (predictor + corrector) over all 4,000,000 spins, inputs resident
-in HBM.  Also reported: the FP32-vs-FP64 drift of the magnetisation, the replica-batching leg
-(256 SC replicas in one context against one context per replica), and on the host the
+in HBM.  Also reported: the FP32-vs-FP64 drift of the magnetisation, checkerboard and
+constrained Metropolis on the 256-replica batch, the replica-batching leg (256 SC replicas in one
+context against one context per replica), the other workloads (C0, the C3 adjoint, the C4 film),
+and on the host the
 reference's sim::LLG_Heun (the drop-in executable with gpu:acceleration=off, one thread) with the
 oracle beside it.  Prints one JSON line.
 """
@@ -140,7 +143,59 @@ def batching_leg(steps: int, graph_chunk: int = 50):
     return out
 
 
-def configs_leg(c0_steps: int = 10000, c3_steps: int = 2000, c4_steps: int = 1000):
+MC_BYTES_PER_MOVE = 64     # SC checkerboard Metropolis, FP32: 9p + 4z + 4 with z = 6
+
+
+def metropolis_leg(sweeps: int, warmup: int = 5):
+    """Config C2 under checkerboard Metropolis: the 256-replica batch (8.67M atoms, each replica at
+    its own temperature) swept as one context, FP32 and FP64, spins from +z; and the constrained
+    Monte Carlo sweep of the same batch (FP32, every replica constrained at phi = 40, theta = 30
+    degrees).  Spin-moves (pair trials for constrained MC: N per sweep) per second over all
+    replicas over the wall-clock window of one call of `sweeps` sweeps (the call moves the spins to
+    colour order and back once), with the FP32 sweep against 64 B per spin-move at 8 TB/s."""
+    from vampire_amd import system as S
+    from vampire_amd.context import FP32, FP64, Context
+    reps, temps = S.config_c2()
+    batched = S.concat_systems(reps)
+    total = batched.n
+    out = {"workload": f"{len(reps)} SC replicas, L in [8,48], {total} atoms, T uniform in [0, 1.5 Tc], adaptive-cone "
+                       "checkerboard Metropolis from +z",
+           "atoms": total, "sweeps": sweeps, "bytes_per_move": MC_BYTES_PER_MOVE}
+    for prec in (FP32, FP64):
+        c = Context(batched, precision=prec, seed=7, device=0)
+        for r, T in enumerate(temps):
+            c.set_conditions(T, replica=r)
+        sp = np.zeros((3, total))
+        sp[2] = 1.0
+        c.set_spins(sp)
+        c.mc_sweep(warmup)
+        c.synchronize()
+        t0 = time.perf_counter()
+        c.mc_sweep(sweeps)
+        c.synchronize()
+        wall = time.perf_counter() - t0
+        key = "fp32" if prec == FP32 else "fp64"
+        out[key] = {"value": round(total * sweeps / wall, 1), "ms_per_sweep": round(1e3 * wall / sweeps, 5),
+                    "colours": c.mc_num_colours()}
+        if prec == FP32:
+            achieved = MC_BYTES_PER_MOVE * total * sweeps / wall / 1e9
+            out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(achieved / HBM_PEAK_GBS, 4)}
+            c.cmc_set_constraint(40.0, 30.0)
+            c.set_spins(sp)
+            c.cmc_sweep(warmup)
+            c.synchronize()
+            t0 = time.perf_counter()
+            c.cmc_sweep(sweeps)
+            c.synchronize()
+            wall = time.perf_counter() - t0
+            out["cmc_fp32"] = {"value": round(total * sweeps / wall, 1), "ms_per_sweep": round(1e3 * wall / sweeps, 5),
+                               "unit": "pair trials/s"}
+        c.close()
+    return out
+
+
+def configs_leg(c0_steps: int = 10000, c3_steps: int = 2000, c4_steps: int = 1000, only=("c0", "c3", "c4")):
     """The other workloads of the path, timed on the device:
     C0  SC 20^3 (8000 atoms), 300 K, FP64 fused, HIP-graph replay; the oracle on one host thread
         for 1000 steps beside it (the reference's own CPU-runnable case);
@@ -153,7 +208,16 @@ def configs_leg(c0_steps: int = 10000, c3_steps: int = 2000, c4_steps: int = 100
     from vampire_amd.context import FP32, FP64, FUSED, Context
     from vampire_amd.fit import Trajectory
     out = {}
+    if "c0" in only:
+        out["c0"] = _c0_leg(S, FP64, FUSED, Context, c0_steps)
+    if "c3" in only:
+        out["c3"] = _c3_leg(S, FP32, FP64, Trajectory, c3_steps)
+    if "c4" in only:
+        out["c4"] = _c4_leg(S, FP32, FUSED, Context, c4_steps)
+    return out
 
+
+def _c0_leg(S, FP64, FUSED, Context, c0_steps):
     s0 = S.config_c0()
     sp0 = S.random_spins(s0.n, 1)
     c = Context(s0, precision=FP64, kernel=FUSED, seed=1, device=0)
@@ -172,10 +236,12 @@ def configs_leg(c0_steps: int = 10000, c3_steps: int = 2000, c4_steps: int = 100
     t1 = time.perf_counter()
     o.llg_heun(sp0, 1000, seed=1, threads=1)
     cpu = s0.n * 1000 / (time.perf_counter() - t1)
-    out["c0"] = {"workload": "SC 20^3, 8000 atoms, 300 K, FP64 fused, graph replay", "steps": c0_steps,
-                 "seconds": round(dt, 4), "spin_steps_per_s": round(s0.n * c0_steps / dt, 1),
-                 "cpu_1thread_spin_steps_per_s": round(cpu, 1)}
+    return {"workload": "SC 20^3, 8000 atoms, 300 K, FP64 fused, graph replay", "steps": c0_steps,
+            "seconds": round(dt, 4), "spin_steps_per_s": round(s0.n * c0_steps / dt, 1),
+            "cpu_1thread_spin_steps_per_s": round(cpu, 1)}
 
+
+def _c3_leg(S, FP32, FP64, Trajectory, c3_steps):
     s3 = S.config_c3()
     th = np.deg2rad(30.0)
     sp3 = np.zeros((3, s3.n))
@@ -197,8 +263,10 @@ def configs_leg(c0_steps: int = 10000, c3_steps: int = 2000, c4_steps: int = 100
         t.close()
         c3[prec] = {"forward_s": round(tb - ta, 4), "backward_s": round(tc - tb, 4),
                     "forward_spin_steps_per_s": round(s3.n * c3_steps / (tb - ta), 1)}
-    out["c3"] = c3
+    return c3
 
+
+def _c4_leg(S, FP32, FUSED, Context, c4_steps):
     s4 = S.config_c4()
     c = Context(s4, precision=FP32, kernel=FUSED, seed=3, device=0)
     c.set_spins(S.random_spins(s4.n, 3))
@@ -216,11 +284,10 @@ def configs_leg(c0_steps: int = 10000, c3_steps: int = 2000, c4_steps: int = 100
     dt = time.perf_counter() - t0
     k4 = c.time_kernels(20)
     c.close()
-    out["c4"] = {"workload": "SC film 256x256x8, DMI tensor exchange, 10% dilution, 472014 atoms, FP32 fused, "
-                             "per-step Gaussian T pulse 300->1200 K", "atoms": s4.n, "steps": c4_steps,
-                 "topology": name, "stencils": npat, "seconds": round(dt, 4),
-                 "spin_steps_per_s": round(s4.n * c4_steps / dt, 1), "kernel_ms": k4.tolist()}
-    return out
+    return {"workload": "SC film 256x256x8, DMI tensor exchange, 10% dilution, 472014 atoms, FP32 fused, "
+                        "per-step Gaussian T pulse 300->1200 K", "atoms": s4.n, "steps": c4_steps,
+            "topology": name, "stencils": npat, "seconds": round(dt, 4),
+            "spin_steps_per_s": round(s4.n * c4_steps / dt, 1), "kernel_ms": k4.tolist()}
 
 
 def drift_leg(system, steps: int, c0_steps: int = 1000):
@@ -420,8 +487,10 @@ def main():
     ap.add_argument("--block", type=int, default=-1, help="threads per workgroup; 0: library default, -1: autotune")
     ap.add_argument("--batching-steps", type=int, default=200, help="0 skips the replica-batching leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the C0 / C3 / C4 leg")
+    ap.add_argument("--configs", default="c0,c3,c4", help="comma list of the workloads the configs leg runs")
     ap.add_argument("--legs", default="fp32_fused,fp64_fused,fp64_plain,fp32_midpoint",
-                    help="comma list of fp32_fused, fp64_fused, fp64_plain (profiling runs use one)")
+                    help="comma list of fp32_fused, fp64_fused, fp64_plain, fp32_midpoint (profiling runs use one)")
+    ap.add_argument("--mc-sweeps", type=int, default=200, help="0 skips the Metropolis leg (C2 batch)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with three 256 MiB device-to-device copies (counter scale for traffic)")
     args = ap.parse_args()
@@ -474,6 +543,11 @@ def main():
                         kernel="fused predictor + corrector (one Heun step)",
                         bytes_per_spin_step=p32, step_ms=round(step_ms, 5), kernel_ms=head["kernel_ms"],
                         launch=head["launch"])
+    metropolis = None
+    if args.mc_sweeps:
+        metropolis = metropolis_leg(args.mc_sweeps)
+        print(f"# metropolis: {metropolis}", file=sys.stderr, flush=True)
+
     batching = None
     if args.batching_steps:
         batching = batching_leg(args.batching_steps)
@@ -481,7 +555,7 @@ def main():
 
     configs = None
     if not args.no_configs:
-        configs = configs_leg()
+        configs = configs_leg(only=[x for x in args.configs.split(",") if x])
         print(f"# configs: {configs}", file=sys.stderr, flush=True)
 
     cpu = None
@@ -550,6 +624,7 @@ def main():
                            "integrator": "LLG midpoint (sim::LLG_Midpoint), same workload"}
                           if "fp32_midpoint" in res else None),
         "fp32_vs_fp64_drift": drift,
+        "metropolis": metropolis,
         "batching": batching,
         "configs": configs,
     }

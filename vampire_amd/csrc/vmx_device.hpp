@@ -1295,6 +1295,7 @@ __device__ __forceinline__ bool mc_accept(float x, uint32_t word) {
 
 // Atoms per thread of the Metropolis colour kernel: a workgroup takes 256 * McAtoms list entries.
 template <typename R> struct McAtoms { static constexpr int value = 2; };
+template <> struct McAtoms<double> { static constexpr int value = 1; };   // FP64: 96 VGPRs at two
 
 // The moves of a thread's U atoms idx[] of replica rep (ok[u]: a real entry; head[u]: its stencil
 // id; local[u]: its caller index within the replica, the Philox counter).  The thread forms the
