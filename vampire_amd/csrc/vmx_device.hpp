@@ -458,6 +458,15 @@ __device__ __forceinline__ void spin_field(const SysDev<R>& s, const R* __restri
 }
 
 // The same field from a staged row, with the atom's material record mc.
+// The spin field of a staged row: its neighbours' spins gathered, then summed in row order.  FP64
+// rows of eight or more slots go in two halves (the first half's loads and sum, then the second
+// half's): the same terms in the same order, with half the gathered values live at once, so the
+// FP64 kernels need fewer registers and keep more wavefronts per SIMD in flight.
+template <typename R, int ET, int AM, bool Split>
+__device__ __forceinline__ void row_gather_field(const SysDev<R>& s, const SpinRsrc<R>& rs, const RowIdx<AM>& r,
+                                                 const R* ju, const MatConst<R>& mc, R sx, R sy, R sz, R& fx,
+                                                 R& fy, R& fz);
+
 template <typename R, int ET, int AM>
 __device__ __forceinline__ void row_field(const SysDev<R>& s, const RowIdx<AM>& r, const RowSpins<R, AM>& v,
                                           const R* ju, const MatConst<R>& mc, R sx, R sy, R sz, R& fx, R& fy,
@@ -467,6 +476,37 @@ __device__ __forceinline__ void row_field(const SysDev<R>& s, const RowIdx<AM>& 
         exchange_term<R, ET>(J, Sx, Sy, Sz, hx, hy, hz);
     });
     add_anisotropy(s.ani, mc, sx, sy, sz, hx, hy, hz, fx, fy, fz);
+}
+
+template <typename R, int ET, int AM, bool Split>
+__device__ __forceinline__ void row_gather_field(const SysDev<R>& s, const SpinRsrc<R>& rs, const RowIdx<AM>& r,
+                                                 const R* ju, const MatConst<R>& mc, R sx, R sy, R sz, R& fx,
+                                                 R& fy, R& fz) {
+    constexpr int W = Access<AM>::W;
+    if constexpr (Split && sizeof(R) == 8 && W >= 8) {
+        constexpr int K = (ET == ET_ISO) ? 1 : (ET == ET_VEC ? 3 : 9);
+        constexpr int JM = Access<AM>::JM, H = W / 2;
+        R hx = R(0), hy = R(0), hz = R(0);
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            R vx[H], vy[H], vz[H];
+#pragma unroll
+            for (int k = 0; k < H; ++k) rs.load(r.j[half * H + k], vx[k], vy[k], vz[k]);
+#pragma unroll
+            for (int k = 0; k < H; ++k) {
+                const int slot = half * H + k;
+                if (s.full_rows || slot < r.deg)
+                    exchange_term<R, ET>(JM == JM_UNIFORM ? ju : s.jtab + (size_t)K * r.jd[slot], vx[k], vy[k],
+                                         vz[k], hx, hy, hz);
+            }
+            if (half == 0) __builtin_amdgcn_sched_barrier(0);   // the second half's loads stay here
+        }
+        add_anisotropy(s.ani, mc, sx, sy, sz, hx, hy, hz, fx, fy, fz);
+    } else {
+        RowSpins<R, AM> rv;
+        row_load(rs, r, rv);
+        row_field<R, ET, AM>(s, r, rv, ju, mc, sx, sy, sz, fx, fy, fz);
+    }
 }
 
 // ---- external field at atom i (thermal + applied + dipole), src/simulate/fields.cpp:120-310 ---
@@ -690,10 +730,8 @@ k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, Spin3<R> E, c
         R fx, fy, fz;
         if constexpr (kStaged) {
             RowIdx<AM> ri;
-            RowSpins<R, AM> rv;
             fused_row_index(s, i, head[u], ri);
-            row_load(rs, ri, rv);
-            row_field<R, ET, AM>(s, ri, rv, ju.v, mc, sx[u], sy[u], sz[u], fx, fy, fz);
+            row_gather_field<R, ET, AM, true>(s, rs, ri, ju.v, mc, sx[u], sy[u], sz[u], fx, fy, fz);
         } else {
             spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m[u], sx[u], sy[u], sz[u], fx, fy, fz);
         }
@@ -721,8 +759,12 @@ k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, Spin3<R> E, c
 // (LLGHeun.cpp:257-269) with dS read back when the predictor kept it, else recomputed (FP64);
 // FP32 rebuilds A = S + dt/2 dS from S and S' and forms |A + dt/2 dS'|.  S_next may overwrite S.
 // Own data of all the thread's atoms first, as above.
+// Wavefronts per SIMD the corrector is compiled for: FP64 at four (128 registers, its rows gathered
+// in two halves; three wavefronts without the cap), FP32 unconstrained.
+template <typename R> struct CorrectorWaves { static constexpr int value = 1; };
+template <> struct CorrectorWaves<double> { static constexpr int value = 4; };
 template <typename R, int ET, int AM>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(512, CorrectorWaves<R>::value)
 k_fused_corrector(SysDev<R> s, Spin3Alias<R> S, Spin3<R> P, Spin3Alias<R> O, Spin3<R> D, Spin3<R> E,
                   const double* __restrict__ noise, uint32_t step_off) {
     constexpr int U = 1;                          // launched on the one-atom block table
@@ -765,10 +807,8 @@ k_fused_corrector(SysDev<R> s, Spin3Alias<R> S, Spin3<R> P, Spin3Alias<R> O, Spi
         RowIdx<AM> ri;
         R fx, fy, fz;
         if constexpr (kStaged) {
-            RowSpins<R, AM> rv;
             fused_row_index(s, i, head[u], ri);
-            row_load(rp, ri, rv);
-            row_field<R, ET, AM>(s, ri, rv, ju.v, mc, px[u], py[u], pz[u], fx, fy, fz);
+            row_gather_field<R, ET, AM, true>(s, rp, ri, ju.v, mc, px[u], py[u], pz[u], fx, fy, fz);
         } else {
             spin_field<R, ET, AM>(s, P.x, P.y, P.z, i, m[u], px[u], py[u], pz[u], fx, fy, fz);
         }
@@ -881,10 +921,8 @@ k_mid_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> E, const double* _
         R fx, fy, fz;
         if constexpr (kStaged) {
             RowIdx<AM> ri;
-            RowSpins<R, AM> rv;
             fused_row_index(s, i, head[u], ri);
-            row_load(rs, ri, rv);
-            row_field<R, ET, AM>(s, ri, rv, ju.v, mc, sx[u], sy[u], sz[u], fx, fy, fz);
+            row_gather_field<R, ET, AM, true>(s, rs, ri, ju.v, mc, sx[u], sy[u], sz[u], fx, fy, fz);
         } else {
             spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m[u], sx[u], sy[u], sz[u], fx, fy, fz);
         }
@@ -944,10 +982,8 @@ k_mid_corrector(SysDev<R> s, Spin3Alias<R> S, Spin3<R> P, Spin3Alias<R> O, Spin3
         R fx, fy, fz;
         if constexpr (kStaged) {
             RowIdx<AM> ri;
-            RowSpins<R, AM> rv;
             fused_row_index(s, i, head[u], ri);
-            row_load(rp, ri, rv);
-            row_field<R, ET, AM>(s, ri, rv, ju.v, mc, px[u], py[u], pz[u], fx, fy, fz);
+            row_gather_field<R, ET, AM, false>(s, rp, ri, ju.v, mc, px[u], py[u], pz[u], fx, fy, fz);
         } else {
             spin_field<R, ET, AM>(s, P.x, P.y, P.z, i, m[u], px[u], py[u], pz[u], fx, fy, fz);
         }
