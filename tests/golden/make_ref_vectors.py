@@ -341,6 +341,21 @@ def main(only=()):
     if want("film_vectorial"):
         case("film_vectorial", "sc", 6, 2.5, [va, vb], steps=100, snaps=[1, 100], T=200.0, H=1.0,
              Hdir=(0, 1, 1), pbc=(True, True, False), Lz=4)
+    # material-specific applied fields (material[n]:applied-field-strength / -unit-vector set
+    # sim::local_applied_field, src/vio/match.cpp:2160-2230): each atom takes H + its material's
+    # field (fields.cpp:193-205); two layered materials, global field along z
+    la = {"material-name": "A", "damping-constant": 0.1, "exchange-matrix[1]": 5e-21, "exchange-matrix[2]": 2e-21,
+          "atomic-spin-moment": "1.6 !muB", "second-order-uniaxial-anisotropy-constant": 1.0e-24,
+          "uniaxial-anisotropy-direction": "0,0,1", "applied-field-strength": "0.8 !T",
+          "applied-field-unit-vector": "1,0,0", "minimum-height": 0.0, "maximum-height": 0.5,
+          "initial-spin-direction": "random"}
+    lb = {"material-name": "B", "damping-constant": 0.05, "exchange-matrix[1]": 2e-21, "exchange-matrix[2]": 4e-21,
+          "atomic-spin-moment": "2.0 !muB", "second-order-uniaxial-anisotropy-constant": 0.0,
+          "applied-field-strength": "1.5 !T", "applied-field-unit-vector": "0,1,1", "minimum-height": 0.5,
+          "maximum-height": 1.0, "initial-spin-direction": "random"}
+    if want("film_local"):
+        case("film_local", "sc", 6, 2.5, [la, lb], steps=100, snaps=[1, 100], T=300.0, H=0.5, Hdir=(0, 0, 1),
+             pbc=(True, True, False), Lz=4)
     # thin-film demagnetising field (sim:demagnetisation-factor, fields.cpp:219-239): an SC film
     # open in z, statistics before every step so the field follows the system magnetisation
     if want("film_demag"):

@@ -32,6 +32,12 @@ CASES = {
                                  S.Material(mu_s_muB=2.4, alpha=0.05)], ani=dict(uniaxial=True),
                            J=[[[5e-21, 4.1e-21, 6.2e-21], [-1.2e-21, -0.9e-21, -1.6e-21]],
                               [[-1.2e-21, -0.9e-21, -1.6e-21], [5.5e-21, 6.1e-21, 4.4e-21]]]),
+    # material-specific applied fields (sim::local_applied_field): 0.8 T along x, 1.5 T along (0,1,1)
+    "film_local": dict(mats=[S.Material(mu_s_muB=1.6, alpha=0.1, Ku=1.0e-24, applied_field_strength=0.8,
+                                        applied_field_unit_vector=(1.0, 0.0, 0.0)),
+                             S.Material(mu_s_muB=2.0, alpha=0.05, applied_field_strength=1.5,
+                                        applied_field_unit_vector=(0.0, 1.0, 1.0))],
+                       ani=dict(uniaxial=True), local_applied=True),
     "sc6_uniaxial_mid": dict(mats=[S.Material(**CO)], ani=dict(uniaxial=True)),
     "film_dmi_mid": dict(mats=[S.Material(mu_s_muB=1.6, alpha=0.1), S.Material(mu_s_muB=7.6, alpha=0.1)], ani={}),
     "dipole_sc10": dict(mats=[S.Material(**CO), S.Material(**dict(CO, mu_s_muB=2.5))], ani=dict(uniaxial=True)),
@@ -63,6 +69,11 @@ def conditions(name: str):
     if "H" not in f.files:
         return float(f["T"]), np.zeros(3)
     return float(f["T"]), np.array(S.applied_field(float(f["H"]), f["Hdir"]))
+
+
+def flags(name: str) -> dict:
+    """Condition keywords a deck needs beyond (T, H): local_applied for material-specific fields."""
+    return {"local_applied": True} if CASES[name].get("local_applied") else {}
 
 
 def dipole_args(name: str, spins_key: str = "spins_in"):

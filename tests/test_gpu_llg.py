@@ -11,7 +11,8 @@ from vampire_amd.context import FP32, FP64, FUSED, PLAIN, Context
 
 pytestmark = pytest.mark.gpu
 
-GOLDEN_CASES = ["sc6_uniaxial", "fcc3_cubic", "bcc4_rotated", "film_dmi", "sc6_vectorial", "film_vectorial"]
+GOLDEN_CASES = ["sc6_uniaxial", "fcc3_cubic", "bcc4_rotated", "film_dmi", "sc6_vectorial", "film_vectorial",
+                "film_local"]
 
 
 TOPOLOGIES = ["csr", "ell", "stencil"]
@@ -62,7 +63,7 @@ def test_fields_match_reference(gpu, name):
     s = RC.system_of(name)
     T, H = RC.conditions(name)
     with _ctx(s) as c:
-        c.set_conditions(T, H)
+        c.set_conditions(T, H, **RC.flags(name))
         c.set_spins(f["spins_in"])
         hs, he = c.fields(noise=O.thermal_noise(int(f["noise_seed"]), s.n, 0))
     ref = f["fields"]
@@ -84,7 +85,7 @@ def test_trajectory_matches_reference(gpu, name, kernel, topology):
     seed = int(f["noise_seed"])
     steps = [int(v) for v in f["snap_steps"]]
     with _ctx(s, kernel=kernel, topology=topology) as c:
-        c.set_conditions(T, H)
+        c.set_conditions(T, H, **RC.flags(name))
         c.set_spins(f["spins_in"])
         last = 0
         for k, st in enumerate(steps):
@@ -136,7 +137,7 @@ def test_fp32_matches_reference_same_normals(gpu, name, kernel):
     T, H = RC.conditions(name)
     seed = int(f["noise_seed"])
     with _ctx(s, precision=FP32, kernel=kernel) as c:
-        c.set_conditions(T, H)
+        c.set_conditions(T, H, **RC.flags(name))
         c.set_spins(f["spins_in"])
         last = 0
         for k, st in enumerate(int(v) for v in f["snap_steps"]):

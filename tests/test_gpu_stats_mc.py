@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("topology", ["csr", "ell", "stencil"])
 @pytest.mark.parametrize("name", ["sc6_uniaxial", "fcc3_cubic", "bcc4_rotated", "film_dmi", "sc6_vectorial",
-                                  "film_vectorial"])
+                                  "film_vectorial", "film_local"])
 def test_stats_match_reference(gpu, name, topology):
     """System and material magnetisation and system energies after the reference's steps
     (stats_after = system M [8], material M [4(nmat+1)], system E [5x2])."""
@@ -22,7 +22,7 @@ def test_stats_match_reference(gpu, name, topology):
     T, H = RC.conditions(name)
     nmat = len(s.materials)
     with Context(s, precision=FP64, topology=topology) as c:
-        c.set_conditions(T, H)
+        c.set_conditions(T, H, **RC.flags(name))
         c.set_spins(f["snaps"][-1])
         hs = c.add_magnetization(np.zeros(s.n, np.int32), 2)
         hm = c.add_magnetization(s.type, nmat + 1)

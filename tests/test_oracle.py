@@ -11,7 +11,8 @@ import refcases as RC
 from oracle import oracle as O
 from vampire_amd import system as S
 
-GOLDEN_CASES = ["sc6_uniaxial", "fcc3_cubic", "bcc4_rotated", "film_dmi", "sc6_vectorial", "film_vectorial"]
+GOLDEN_CASES = ["sc6_uniaxial", "fcc3_cubic", "bcc4_rotated", "film_dmi", "sc6_vectorial", "film_vectorial",
+                "film_local"]
 
 
 # ---------------------------------------------------------------- Philox known answers
@@ -46,7 +47,7 @@ def test_midpoint_trajectory_equals_reference(name):
     s = RC.system_of(name)
     T, H = RC.conditions(name)
     o = O.Oracle(s)
-    o.set_conditions(T, H)
+    o.set_conditions(T, H, **RC.flags(name))
     seed = int(f["noise_seed"])
     cur, last = f["spins_in"], 0
     for k, st in enumerate(int(v) for v in f["snap_steps"]):
@@ -157,7 +158,7 @@ def test_fields_equal_reference(name):
     s = RC.system_of(name)
     T, H = RC.conditions(name)
     o = O.Oracle(s)
-    o.set_conditions(T, H)
+    o.set_conditions(T, H, **RC.flags(name))
     hs = o.spin_fields(f["spins_in"])
     he = o.external_fields(O.thermal_noise(int(f["noise_seed"]), s.n, 0))
     assert np.array_equal(hs, f["fields"][:3])
@@ -171,7 +172,7 @@ def test_trajectory_equals_reference(name):
     s = RC.system_of(name)
     T, H = RC.conditions(name)
     o = O.Oracle(s)
-    o.set_conditions(T, H)
+    o.set_conditions(T, H, **RC.flags(name))
     seed = int(f["noise_seed"])
     cur, last = f["spins_in"], 0
     for k, st in enumerate(int(v) for v in f["snap_steps"]):
@@ -188,7 +189,7 @@ def test_statistics_equal_reference(name):
     s = RC.system_of(name)
     T, H = RC.conditions(name)
     o = O.Oracle(s)
-    o.set_conditions(T, H)
+    o.set_conditions(T, H, **RC.flags(name))
     sp = f["snaps"][-1]
     nmat = len(s.materials)
     ref = f["stats_after"]
@@ -604,3 +605,21 @@ def test_cmc_pair_move_conserves_transverse_moment():
         assert np.abs((P @ out.sum(axis=1))[:2] - before).max() < 1e-11
         assert counts.sum() == 30 * s.n and counts[2] > 0 and counts[0] > 0
         assert np.abs(np.linalg.norm(out, axis=0) - 1.0).max() < 1e-14
+
+
+def test_local_applied_field_adds_each_material_field():
+    """The film_local deck's external field is H + the atom's material field: with the
+    material-specific fields switched off the oracle's external field differs from the
+    reference's by exactly those fields (to rounding), material by material."""
+    f = RC.load("film_local")
+    s = RC.system_of("film_local")
+    T, H = RC.conditions("film_local")
+    o = O.Oracle(s)
+    o.set_conditions(T, H)
+    he = o.external_fields(O.thermal_noise(int(f["noise_seed"]), s.n, 0))
+    diff = f["fields"][3:] - he
+    for m, mat in enumerate(s.materials):
+        want = mat["applied_field_strength"] * np.array(mat["applied_field_unit_vector"])
+        sel = s.type == m
+        assert sel.any()
+        np.testing.assert_allclose(diff[:, sel], np.repeat(want[:, None], sel.sum(), axis=1), atol=1e-12)
