@@ -11,6 +11,8 @@
 //   ref_harness <rundir> cmc <nsweeps> <T> <every> <phi> <theta>
 //                                                   montecarlo::cmc_step (constrained), reference RNG
 //   ref_harness <rundir> dipole                     macrocell dipole set-up and fields (dipole.bin)
+//   ref_harness_vmx <rundir> gpullg <nsteps> <every> <heun|mid> <seed>
+//   ref_harness_vmx <rundir> gpumc <nsweeps> <T> <every>
 //   ref_harness_vmx <rundir> gpucmc <nsweeps> <T> <every> <phi> <theta>
 //                                                   the cmc command through the gpu:: drop-in
 //                                                   (built with -DVMX_SHIM, linked with the shim)
@@ -45,7 +47,10 @@
 #ifdef VMX_SHIM
 #include "gpu.hpp"
 namespace gpu {
-void cmc_step();   // defined by the drop-in shim; the reference's gpu.hpp declares no such call
+// defined by the drop-in shim; the reference's gpu.hpp declares no such calls
+void mc_step();
+void llg_midpoint();
+void cmc_step();
 }
 #endif
 
@@ -356,6 +361,51 @@ int main(int argc, char** argv) {
         return 0;
     }
 #ifdef VMX_SHIM
+    if (cmd == "gpullg") {
+        // <nsteps> <every> <heun|mid> <seed>: the llg command through the gpu:: drop-in.
+        // gpu::initialize takes mtrandom::integration_seed as the Philox key and sim::time as the
+        // step counter, so seed = the noise seed and sim::time = 1 make the device draw the
+        // stream that the llg command's noise.bin holds for steps 1..nsteps
+        const int nsteps = std::atoi(argv[3]);
+        const int every = std::atoi(argv[4]);
+        const bool midpoint = std::string(argv[5]) == "mid";
+        mtrandom::integration_seed = std::atoi(argv[6]);
+        sim::time = 1;
+        gpu::initialize();
+        std::ofstream tf("traj.bin", std::ios::binary);
+        for (int s = 1; s <= nsteps; ++s) {
+            if (midpoint) gpu::llg_midpoint();
+            else gpu::llg_heun();
+            if (s % every == 0) {
+                gpu::transfer_spin_positions_from_gpu_to_cpu();
+                put(tf, atoms::x_spin_array);
+                put(tf, atoms::y_spin_array);
+                put(tf, atoms::z_spin_array);
+            }
+        }
+        gpu::finalize();
+        return 0;
+    }
+    if (cmd == "gpumc") {
+        // the mc command through the drop-in: gpu::mc_step per sweep, the host spin arrays
+        // refreshed before each sample
+        const int nsweeps = std::atoi(argv[3]);
+        sim::temperature = std::atof(argv[4]);
+        const int every = std::atoi(argv[5]);
+        gpu::initialize();
+        std::ofstream mf("mc.bin", std::ios::binary);
+        for (int s = 1; s <= nsweeps; ++s) {
+            gpu::mc_step();
+            if (s % every == 0) {
+                gpu::transfer_spin_positions_from_gpu_to_cpu();
+                stats::system_magnetization.calculate_magnetization(atoms::x_spin_array, atoms::y_spin_array,
+                                                                    atoms::z_spin_array, atoms::m_spin_array);
+                put(mf, stats::system_magnetization.get_magnetization());
+            }
+        }
+        gpu::finalize();
+        return 0;
+    }
     if (cmd == "gpucmc") {
         // the cmc command through the drop-in boundary: gpu::initialize, then gpu::cmc_step per
         // sweep (the device runs the sweep; its first call runs CMCinit on the host spins), the

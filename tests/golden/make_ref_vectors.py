@@ -289,28 +289,22 @@ CO = {"material-name": "Co", "damping-constant": 0.1, "exchange-matrix[1]": 6.06
       "uniaxial-anisotropy-direction": "0,0,1", "initial-spin-direction": "random"}
 
 
-def main(only=()):
-    """Writes every fixture, or those named in `only` (e.g. `dipole_sc10`)."""
-    if not os.path.exists(HARNESS):
-        sys.exit("build the reference harness first: make -C oracle ref")
-
-    def want(name):
-        return not only or name in only
-
+def llg_cases():
+    """The trajectory decks: name -> the keywords of case() (deck, steps, snapshots, conditions)."""
+    c = {}
     # SC, scalar exchange, uniaxial anisotropy, applied field, thermal noise
-    if want("sc6_uniaxial"):
-        case("sc6_uniaxial", "sc", 6, 3.54, [CO], steps=200, snaps=[1, 10, 200], T=300.0, H=0.5)
+    c["sc6_uniaxial"] = dict(crystal="sc", L=6, a=3.54, materials=[CO], steps=200, snaps=[1, 10, 200], T=300.0,
+                             H=0.5)
     # FCC, cubic anisotropy, alpha 0.05, T = 600 K
     fcc = dict(CO, **{"damping-constant": 0.05, "second-order-uniaxial-anisotropy-constant": 0.0,
                       "fourth-order-cubic-anisotropy-constant": 3.0e-24})
-    if want("fcc3_cubic"):
-        case("fcc3_cubic", "fcc", 3, 3.54, [fcc], steps=100, snaps=[1, 100], T=600.0, H=0.0)
+    c["fcc3_cubic"] = dict(crystal="fcc", L=3, a=3.54, materials=[fcc], steps=100, snaps=[1, 100], T=600.0, H=0.0)
     # BCC, rotated cubic basis, field along (1,0,1)/sqrt2
     bcc = dict(CO, **{"second-order-uniaxial-anisotropy-constant": 0.0,
                       "fourth-order-cubic-anisotropy-constant": -2.0e-24,
                       "cubic-anisotropy-direction-1": "1,1,0", "cubic-anisotropy-direction-2": "-1,1,0"})
-    if want("bcc4_rotated"):
-        case("bcc4_rotated", "bcc", 4, 2.87, [bcc], steps=100, snaps=[1, 100], T=100.0, H=2.0, Hdir=(1, 0, 1))
+    c["bcc4_rotated"] = dict(crystal="bcc", L=4, a=2.87, materials=[bcc], steps=100, snaps=[1, 100], T=100.0,
+                             H=2.0, Hdir=(1, 0, 1))
     # two-material SC film, open in z, ferrimagnetic inter-material exchange and interfacial DMI:
     # tensorial exchange with the reference's DMI tensors
     m1 = {"material-name": "A", "damping-constant": 0.1, "exchange-matrix[1]": 5e-21,
@@ -319,16 +313,14 @@ def main(only=()):
     m2 = {"material-name": "B", "damping-constant": 0.1, "exchange-matrix[1]": -1.2e-21,
           "exchange-matrix[2]": 5e-21, "atomic-spin-moment": "7.6 !muB", "dmi-constant[1]": 0.3e-21,
           "minimum-height": 0.5, "maximum-height": 1.0, "initial-spin-direction": "random"}
-    if want("film_dmi"):
-        case("film_dmi", "sc", 5, 2.5, [m1, m2], steps=100, snaps=[1, 100], T=300.0, H=0.0,
-             pbc=(True, True, False), dmi_cutoff=3.6, Lz=4)
+    c["film_dmi"] = dict(crystal="sc", L=5, a=2.5, materials=[m1, m2], steps=100, snaps=[1, 100], T=300.0, H=0.0,
+                         pbc=(True, True, False), dmi_cutoff=3.6, Lz=4)
     # vectorial exchange: a three-value exchange-matrix selects exchange::vectorial
     # (src/exchange/interface.cpp:86-95, initialize.cpp:63-64), diagonal Jxx, Jyy, Jzz per edge
     # (exchange_fields.cpp:85-116).  One material (every edge the same J) ...
     cov = dict(CO, **{"exchange-matrix[1]": "6.064e-21,5.2e-21,7.3e-21"})
-    if want("sc6_vectorial"):
-        case("sc6_vectorial", "sc", 6, 3.54, [cov], steps=200, snaps=[1, 10, 200], T=300.0, H=0.5,
-             Hdir=(1, 1, 1))
+    c["sc6_vectorial"] = dict(crystal="sc", L=6, a=3.54, materials=[cov], steps=200, snaps=[1, 10, 200], T=300.0,
+                              H=0.5, Hdir=(1, 1, 1))
     # ... and two materials in layers with antiparallel inter-layer coupling (per-edge J ids)
     va = {"material-name": "A", "damping-constant": 0.1, "exchange-matrix[1]": "5e-21,4.1e-21,6.2e-21",
           "exchange-matrix[2]": "-1.2e-21,-0.9e-21,-1.6e-21", "atomic-spin-moment": "1.6 !muB",
@@ -338,9 +330,8 @@ def main(only=()):
           "exchange-matrix[2]": "5.5e-21,6.1e-21,4.4e-21", "atomic-spin-moment": "2.4 !muB",
           "second-order-uniaxial-anisotropy-constant": 0.0,
           "minimum-height": 0.5, "maximum-height": 1.0, "initial-spin-direction": "random"}
-    if want("film_vectorial"):
-        case("film_vectorial", "sc", 6, 2.5, [va, vb], steps=100, snaps=[1, 100], T=200.0, H=1.0,
-             Hdir=(0, 1, 1), pbc=(True, True, False), Lz=4)
+    c["film_vectorial"] = dict(crystal="sc", L=6, a=2.5, materials=[va, vb], steps=100, snaps=[1, 100], T=200.0,
+                               H=1.0, Hdir=(0, 1, 1), pbc=(True, True, False), Lz=4)
     # material-specific applied fields (material[n]:applied-field-strength / -unit-vector set
     # sim::local_applied_field, src/vio/match.cpp:2160-2230): each atom takes H + its material's
     # field (fields.cpp:193-205); two layered materials, global field along z
@@ -353,25 +344,35 @@ def main(only=()):
           "atomic-spin-moment": "2.0 !muB", "second-order-uniaxial-anisotropy-constant": 0.0,
           "applied-field-strength": "1.5 !T", "applied-field-unit-vector": "0,1,1", "minimum-height": 0.5,
           "maximum-height": 1.0, "initial-spin-direction": "random"}
-    if want("film_local"):
-        case("film_local", "sc", 6, 2.5, [la, lb], steps=100, snaps=[1, 100], T=300.0, H=0.5, Hdir=(0, 0, 1),
-             pbc=(True, True, False), Lz=4)
+    c["film_local"] = dict(crystal="sc", L=6, a=2.5, materials=[la, lb], steps=100, snaps=[1, 100], T=300.0, H=0.5,
+                           Hdir=(0, 0, 1), pbc=(True, True, False), Lz=4)
     # thin-film demagnetising field (sim:demagnetisation-factor, fields.cpp:219-239): an SC film
     # open in z, statistics before every step so the field follows the system magnetisation
-    if want("film_demag"):
-        case("film_demag", "sc", 6, 3.54, [CO], steps=50, snaps=[1, 10, 50], T=300.0, H=0.5, Hdir=(1, 0, 0),
-             pbc=(True, True, False), Lz=3, integrator="demag", extra=["sim:demagnetisation-factor = 0,0.2,0.8"])
+    c["film_demag"] = dict(crystal="sc", L=6, a=3.54, materials=[CO], steps=50, snaps=[1, 10, 50], T=300.0, H=0.5,
+                           Hdir=(1, 0, 0), pbc=(True, True, False), Lz=3, integrator="demag",
+                           extra=["sim:demagnetisation-factor = 0,0.2,0.8"])
     # the LLG midpoint integrator (sim::LLG_Midpoint) on the SC uniaxial and two-material DMI decks
-    if want("sc6_uniaxial_mid"):
-        case("sc6_uniaxial_mid", "sc", 6, 3.54, [CO], steps=100, snaps=[1, 10, 100], T=300.0, H=0.5,
-             integrator="midpoint")
-    if want("film_dmi_mid"):
-        case("film_dmi_mid", "sc", 5, 2.5, [m1, m2], steps=100, snaps=[1, 100], T=300.0, H=0.0,
-             pbc=(True, True, False), dmi_cutoff=3.6, Lz=4, integrator="midpoint")
+    c["sc6_uniaxial_mid"] = dict(crystal="sc", L=6, a=3.54, materials=[CO], steps=100, snaps=[1, 10, 100],
+                                 T=300.0, H=0.5, integrator="midpoint")
+    c["film_dmi_mid"] = dict(crystal="sc", L=5, a=2.5, materials=[m1, m2], steps=100, snaps=[1, 100], T=300.0,
+                             H=0.0, pbc=(True, True, False), dmi_cutoff=3.6, Lz=4, integrator="midpoint")
     # the CPU-runnable workload: SC 20^3, 1000 steps at 300 K
-    if want("c0_sc20"):
-        case("c0_sc20", "sc", 20, 3.54, [CO], steps=1000, snaps=[1000], T=300.0, H=0.0, seed=1, spin_seed=1,
-             slim=True)
+    c["c0_sc20"] = dict(crystal="sc", L=20, a=3.54, materials=[CO], steps=1000, snaps=[1000], T=300.0, H=0.0,
+                        seed=1, spin_seed=1, slim=True)
+    return c
+
+
+def main(only=()):
+    """Writes every fixture, or those named in `only` (e.g. `dipole_sc10`)."""
+    if not os.path.exists(HARNESS):
+        sys.exit("build the reference harness first: make -C oracle ref")
+
+    def want(name):
+        return not only or name in only
+
+    for name, kw in llg_cases().items():
+        if want(name):
+            case(name, **kw)
     # macrocell dipole field, tensor solver: SC 10x10x6 block, open boundaries, two materials in
     # layers (1.72 and 2.5 mu_B), cells of 2x2x2 unit cells, field refreshed every 10 steps
     da = dict(CO, **{"maximum-height": 0.5, "exchange-matrix[2]": 6.064e-21})

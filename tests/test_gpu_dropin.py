@@ -444,3 +444,78 @@ def test_dropin_constrained_monte_carlo_against_reference(gpu, tmp_path):
         assert cnt[3] == sweeps * L ** 3 and cnt[:3].sum() == cnt[3]
         np.testing.assert_allclose(cnt[:3] / cnt[3], ref_counts[:3] / ref_counts[3], atol=0.01)
     RC.assert_cmc_curve(np.array(got))
+
+
+def _fixture_script():
+    """tests/golden/make_ref_vectors.py as a module: the decks the golden vectors were made from."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_ref_vectors",
+                                                  os.path.join(ROOT, "tests", "golden", "make_ref_vectors.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.mark.parametrize("name", ["sc6_uniaxial", "film_local", "sc6_uniaxial_mid", "film_dmi_mid"])
+def test_dropin_heun_and_midpoint_through_shim_equal_reference(gpu, tmp_path, name):
+    """gpu::initialize / gpu::llg_heun / gpu::llg_midpoint / gpu::transfer_spin_positions_from_gpu_to_cpu
+    on the oracle harness linked to the shim, on the deck a golden trajectory was made from: with
+    mtrandom::integration_seed = the noise seed and sim::time = 1 the device draws the Philox stream
+    the reference was fed, so its snapshots equal the reference's sim::LLG_Heun / sim::LLG_Midpoint
+    trajectory to 1e-10 (FP64)."""
+    if not os.path.exists(HARNESS_VMX):
+        pytest.skip("harness over the shim not built (make -C vampire_amd/shim)")
+    import refcases as RC
+    mrv = _fixture_script()
+    kw = mrv.llg_cases()[name]
+    f = RC.load(name)
+    inp, mat = mrv.deck(kw["crystal"], kw["L"], kw["a"], kw["materials"], kw.get("pbc", (True, True, True)),
+                        kw["T"], kw.get("H", 0.0), kw.get("Hdir", (0, 0, 1)), kw.get("dmi_cutoff"), kw.get("Lz"),
+                        kw.get("extra", ()))
+    d = tmp_path / name
+    d.mkdir()
+    (d / "input").write_text(inp)
+    (d / "test.mat").write_text(mat)
+    np.ascontiguousarray(f["spins_in"], dtype=np.float64).tofile(d / "spins_in.bin")
+    steps = [int(v) for v in f["snap_steps"]]
+    integ = "mid" if kw.get("integrator") == "midpoint" else "heun"
+    print("gpullg", name, integ, flush=True)
+    r = subprocess.run([HARNESS_VMX, str(d), "gpullg", str(steps[-1]), "1", integ, str(kw.get("seed", 11))],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    n = f["spins_in"].shape[1]
+    traj = np.fromfile(d / "traj.bin", dtype=np.float64).reshape(steps[-1], 3, n)
+    for k, st in enumerate(steps):
+        assert np.abs(traj[st - 1] - f["snaps"][k]).max() <= 1e-10, (name, st)
+
+
+def test_dropin_metropolis_through_shim_matches_reference(gpu, tmp_path):
+    """gpu::mc_step (one storage-order checkerboard sweep per call) on the harness linked to the
+    shim, on the deck of the reference's mc_step curve (SC 8^3, no anisotropy, from +z, 6000
+    sweeps, samples every 5 after 1000): <|m|>(T) at 100-900 K and the 1/2 crossing within the
+    combined error bars of the reference run."""
+    if not os.path.exists(HARNESS_VMX):
+        pytest.skip("harness over the shim not built (make -C vampire_amd/shim)")
+    import refcases as RC
+    mrv = _fixture_script()
+    f = RC.load("mc_sc8")
+    L, a = int(f["L"]), float(f["a"])
+    sweeps, equil, every = int(f["sweeps"]), int(f["equil"]), 5
+    mat = dict(mrv.CO, **{"second-order-uniaxial-anisotropy-constant": 0.0, "initial-spin-direction": "0,0,1"})
+    temps = f["M_T"][:, 0]
+    means, errs = [], []
+    for T in temps:
+        d = tmp_path / f"T{int(T)}"
+        d.mkdir()
+        inp, mtxt = mrv.deck("sc", L, a, [mat], (True, True, True), float(T))
+        (d / "input").write_text(inp)
+        (d / "test.mat").write_text(mtxt)
+        print("gpumc", T, flush=True)
+        r = subprocess.run([HARNESS_VMX, str(d), "gpumc", str(sweeps), str(T), str(every)],
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        m = np.fromfile(d / "mc.bin", dtype=np.float64).reshape(-1, 8)[:, 3]
+        mean, err = RC.block_mean(m[equil // every:])
+        means.append(mean)
+        errs.append(err)
+    RC.assert_mc_curve(temps, np.array(means), np.array(errs))
