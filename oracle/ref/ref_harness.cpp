@@ -13,7 +13,7 @@
 //   ref_harness <rundir> dipole                     macrocell dipole set-up and fields (dipole.bin)
 //   ref_harness_vmx <rundir> gpullg <nsteps> <every> <heun|mid> <seed>
 //   ref_harness_vmx <rundir> gpumc <nsweeps> <T> <every>
-//   ref_harness_vmx <rundir> gpucmc <nsweeps> <T> <every> <phi> <theta>
+//   ref_harness_vmx <rundir> gpucmc <nsweeps> <T> <every> <phi> <theta> [<phi2> <theta2> <sweep>]
 //                                                   the cmc command through the gpu:: drop-in
 //                                                   (built with -DVMX_SHIM, linked with the shim)
 //
@@ -416,9 +416,20 @@ int main(int argc, char** argv) {
         const int every = std::atoi(argv[5]);
         sim::constraint_phi = std::atof(argv[6]);
         sim::constraint_theta = std::atof(argv[7]);
+        // optional <phi2> <theta2> <sweep>: at that sweep the constraint moves and CMCinit runs on
+        // the host, as src/program/cmc_anisotropy.cpp:114-118,163-165 does between its loops
+        const bool move = argc > 10;
+        const int move_at = move ? std::atoi(argv[10]) : 0;
         gpu::initialize();
         std::ofstream cf("cmc.bin", std::ios::binary);
         for (int s = 1; s <= nsweeps; ++s) {
+            if (move && s == move_at) {
+                sim::constraint_phi = std::atof(argv[8]);
+                sim::constraint_theta = std::atof(argv[9]);
+                sim::constraint_phi_changed = true;
+                sim::constraint_theta_changed = true;
+                montecarlo::CMCinit();
+            }
             gpu::cmc_step();
             if (s % every == 0) {
                 gpu::transfer_spin_positions_from_gpu_to_cpu();

@@ -519,3 +519,42 @@ def test_dropin_metropolis_through_shim_matches_reference(gpu, tmp_path):
         means.append(mean)
         errs.append(err)
     RC.assert_mc_curve(temps, np.array(means), np.array(errs))
+
+
+def test_dropin_constrained_mc_follows_a_moved_constraint(gpu, tmp_path):
+    """cmc_anisotropy moves the constraint between its loops and re-runs CMCinit on the host
+    (src/program/cmc_anisotropy.cpp:114-118, 163-165).  Through gpu::cmc_step the device must then
+    start from CMCinit's spins and hold the new direction: SC 8^3 at 300 K, phi/theta 40/30 for
+    3000 sweeps, then 70/120.  Pair moves conserve the transverse moment exactly, so a device that
+    kept its old spins would carry the old moment's large transverse part for ever; here it is
+    below 1e-12 about each constraint in turn, and <m.v> under the new constraint equals the
+    reference's under the old one within 0.02 (the weak anisotropy barely changes the magnitude)."""
+    if not os.path.exists(HARNESS_VMX):
+        pytest.skip("harness over the shim not built (make -C vampire_amd/shim)")
+    import refcases as RC
+    mrv = _fixture_script()
+    f = RC.load("cmc_sc8")
+    L, a = int(f["L"]), float(f["a"])
+    ref300 = f["rows"][np.where(f["rows"][:, 0] == 300.0)[0][0]]
+    d = tmp_path / "move"
+    d.mkdir()
+    inp, mtxt = mrv.deck("sc", L, a, [mrv.CO], (True, True, True), 300.0)
+    (d / "input").write_text(inp)
+    (d / "test.mat").write_text(mtxt)
+    print("gpucmc move", flush=True)
+    r = subprocess.run([HARNESS_VMX, str(d), "gpucmc", "6000", "300.0", "5", "40", "30", "70", "120", "3001"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    raw = np.fromfile(d / "cmc.bin", dtype=np.float64)
+    data = raw[:-4].reshape(-1, 5)
+    m = data[:, :3] * data[:, 3:4]
+
+    def axis(phi, theta):
+        p, t = np.radians(phi), np.radians(theta)
+        return np.array([np.sin(p) * np.cos(t), np.sin(p) * np.sin(t), np.cos(p)])
+    v1, v2 = axis(40, 30), axis(70, 120)
+    before, after = m[:600], m[600:]                  # samples at sweeps 5..3000 and 3005..6000
+    assert np.abs(before - np.outer(before @ v1, v1)).max() < 1e-12
+    assert np.abs(after - np.outer(after @ v2, v2)).max() < 1e-12
+    mv, dm = RC.block_mean(after[200:] @ v2)
+    assert abs(mv - ref300[1]) <= 0.02 + 4 * np.hypot(dm, ref300[2]), (mv, ref300[1])

@@ -371,8 +371,9 @@ void mc_step() {
 // trials (montecarlo::cmc_step, src/montecarlo/cmc.cpp:282-497), colour-paired on the device.
 //     if(gpu::acceleration) gpu::cmc_step(); else montecarlo::cmc_step();
 // The first call after cmc::is_initialised was cleared runs CMCinit on the host spins (it sets or
-// rotates them to the constraint direction, cmc.cpp:205-273) and uploads them; every call sends
-// the constraint angles, as cmc_anisotropy changes them between temperature loops.  The move is
+// rotates them to the constraint direction, cmc.cpp:205-273) and uploads them; when the constraint
+// angles have moved (cmc_anisotropy moves them between temperature loops and re-runs CMCinit on
+// the host) the host spins and the new angles go to the device.  The move is
 // the reference's montecarlo::internal::mc_move: the adaptive cone at the device's Metropolis
 // width (the same width gpu::mc_step adapts) or the temperature-scaled angle move.
 void cmc_step() {
@@ -383,14 +384,19 @@ void cmc_step() {
         err::vexit();
     }
     refuse(dipole::activated, "the dipole field in constrained Monte Carlo");
+    const bool moved = sim::constraint_phi != g_cmc_sent[0] || sim::constraint_theta != g_cmc_sent[1];
+    bool upload = moved;   // the programs that move the constraint re-run CMCinit on the host spins
+                           // first (src/program/cmc_anisotropy.cpp:118): those hold the new start
     if (!montecarlo::cmc::is_initialised) {
         transfer_spin_positions_from_gpu_to_cpu();
         montecarlo::CMCinit();
+        upload = true;
+    }
+    if (upload)
         check(vmx_set_spins(g_ctx, atoms::x_spin_array.data(), atoms::y_spin_array.data(),
                             atoms::z_spin_array.data()),
               "vmx_set_spins");
-    }
-    if (sim::constraint_phi != g_cmc_sent[0] || sim::constraint_theta != g_cmc_sent[1]) {
+    if (moved) {
         check(vmx_cmc_set_constraint(g_ctx, -1, sim::constraint_phi, sim::constraint_theta),
               "vmx_cmc_set_constraint");
         g_cmc_sent[0] = sim::constraint_phi;
@@ -400,6 +406,9 @@ void cmc_step() {
           "vmx_cmc_set_move");
     push_conditions();
     check(vmx_cmc_sweep(g_ctx, 1), "vmx_cmc_sweep");
+    // with sim:constraint-rotation the next CMCinit rotates the current spins (cmc.cpp:240-262),
+    // so the host arrays follow the device after every sweep
+    if (sim::constraint_rotation) transfer_spin_positions_from_gpu_to_cpu();
 }
 
 void llg_midpoint() {
