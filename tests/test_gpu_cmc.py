@@ -38,6 +38,31 @@ def test_cmc_matches_oracle(gpu, topology, angle):
     assert counts[0] > 0 and counts[1] > 0 and counts[2] > 0
 
 
+@pytest.mark.parametrize("topology", ["csr", "ell", "stencil"])
+def test_cmc_odd_colour_runs_match_oracle(gpu, topology):
+    """SC 7^3 needs four colours of 117, 117, 55 and 54 atoms: runs of odd length, whose pairs
+    the device finds through the inverse of the permutation's stride (one slot per entry, the
+    first entries of pairs active).  FP64 sweeps equal the oracle's to 1e-12 with equal counts."""
+    s = S.ferromagnet("sc", 7)
+    o = O.Oracle(s)
+    _, _, start = o.colouring()
+    assert sorted(np.diff(start) % 2) == [0, 1, 1, 1]
+    o.set_conditions(500.0, (0.1, 0.0, 0.3))
+    k = o.cmc(40.0, 30.0, sigma=50.0)
+    spins = S.random_spins(s.n, 12)
+    ref, counts = o.cmc_checkerboard(spins, 5, k, seed=17, sweep0=3)
+    with Context(s, precision=FP64, seed=17, topology=topology) as c:
+        c.set_conditions(500.0, (0.1, 0.0, 0.3))
+        c.set_spins(spins)
+        c.sweep = 3
+        c.mc_set_sigma(50.0)
+        c.cmc_set_constraint(40.0, 30.0)
+        c.cmc_sweep(5)
+        assert np.abs(c.get_spins() - ref).max() <= 1e-12
+        assert np.array_equal(c.cmc_counts()[0], counts)
+    assert counts[0] > 0 and counts[1] > 0 and counts[2] > 0
+
+
 def test_cmc_two_materials_tensor_exchange_matches_oracle(gpu):
     """The thin film with two materials (moments 1.6 and 7.6 mu_B: the compensating spin scales by
     mu1/mu2) and tensor exchange with DMI: device equals the oracle to 1e-12 with equal counts."""

@@ -528,8 +528,9 @@ struct vmx_ctx {
     std::vector<int> colour_start;
     // constrained Monte Carlo: per colour, per replica, the colour's atoms in caller order (the
     // pairing follows that order); workgroup table {first entry, replica, run length, first pair}
-    DevBuf cmc_list, cmc_tab, cmc_frame, cmc_M, cmc_part, cmc_counts;
+    DevBuf cmc_list, cmc_tab, cmc_frame, cmc_M, cmc_part, cmc_counts, cmc_dm, cmc_range, cmc_mlen, cmc_perm;
     std::vector<int> cmc_tab_start;        // colour k's workgroups: cmc_tab[cmc_tab_start[k] ...]
+                                           // cmc_range[k * (nrep + 1) + r]: replica r's first of them
     std::vector<double> cmc_frame_h;       // [nrep][21]: polar matrix, transpose, constraint direction
     std::vector<char> cmc_frame_set;       // per replica
     bool cmc_frame_dirty = true;
@@ -540,7 +541,7 @@ struct vmx_ctx {
     template <typename R>
     SysDev<R> sys_colour_order() const;
     static constexpr int kMcOrderedMinSweeps = 4;   // calls with fewer sweeps stay in storage order
-    static constexpr int kMcBlock = 256;   // pairs per constrained Monte Carlo workgroup
+    static constexpr int kMcBlock = 256;   // slots per constrained Monte Carlo workgroup
     std::vector<int> mc_blk_start;         // launch l's workgroups: mc_blk[mc_blk_start[l] ...]
     DevBuf mc_atom, mc_pat, mc_blk, mc_sigma, mc_rej, mc_tot_moves, mc_tot_rej;
     // Metropolis in colour order (stencil rows only): each replica's atoms colour by colour,
@@ -1175,26 +1176,34 @@ int vmx_ctx::colour() {
     VMX_TRY(upload(mc_blk, tab));
     if (tp == TP_PAT) VMX_TRY(colour_order(c, lau, nc));
     // constrained Monte Carlo: the same colours, each replica's run in caller order, workgroups of
-    // up to 256 pairs per run
-    std::vector<int> clist;
+    // 256 slots per run (m/2 slots for a run of even length m, m for odd m)
+    std::vector<int> clist, crange((size_t)nc * (nrep + 1)), cmlen((size_t)nc * nrep);
     std::vector<int4> ctab;
     clist.reserve(n);
     cmc_tab_start.assign(nc + 1, 0);
     for (int k = 0; k < nc; ++k) {
         cmc_tab_start[k] = (int)ctab.size();
         for (int r = 0; r < nrep; ++r) {
+            crange[(size_t)k * (nrep + 1) + r] = (int)ctab.size() - cmc_tab_start[k];
             const int first = (int)clist.size();
             for (int o = rep_off_h[r]; o < rep_off_h[r + 1]; ++o) {
                 const int i = q > 1 ? pi[o] : o;
                 if (c[i] == k) clist.push_back(i);
             }
             const int m = (int)clist.size() - first;
-            for (int p0 = 0; p0 < m / 2; p0 += kMcBlock) ctab.push_back(make_int4(first, r, m, p0));
+            cmlen[(size_t)k * nrep + r] = m;
+            const int slots = m < 2 ? 0 : (m % 2 == 0 ? m / 2 : m);
+            for (int t0 = 0; t0 < slots; t0 += kMcBlock) ctab.push_back(make_int4(first, r, m, t0));
         }
+        crange[(size_t)k * (nrep + 1) + nrep] = (int)ctab.size() - cmc_tab_start[k];
     }
     cmc_tab_start[nc] = (int)ctab.size();
     VMX_TRY(upload(cmc_list, clist));
     VMX_TRY(upload(cmc_tab, ctab));
+    VMX_TRY(upload(cmc_range, crange));
+    VMX_TRY(upload(cmc_mlen, cmlen));
+    VMX_TRY(cmc_perm.alloc(2 * (size_t)nc * nrep * sizeof(int4)));
+    VMX_TRY(cmc_dm.alloc(3 * std::max<size_t>(ctab.size(), 1) * sizeof(double)));
     ncolours = nc;
     coloured = true;
     return VMX_OK;
@@ -1338,7 +1347,9 @@ int vmx_ctx::mc_sweeps(int nsweeps) {
 }
 
 // Constrained Monte Carlo sweeps (montecarlo::cmc_step, src/montecarlo/cmc.cpp:282-497): per
-// colour two rounds, each after recomputing every replica's magnetisation.
+// colour two rounds.  Every replica's magnetisation is summed over all spins at the start of each
+// sweep (as cmc_step does, cmc.cpp:346-351) and brought forward after each round by the sum of
+// the round's accepted changes, so no round but a sweep's first reads every spin for it.
 template <typename R>
 int vmx_ctx::cmc_sweeps(int nsweeps) {
     if (!coloured) VMX_TRY(colour());
@@ -1357,17 +1368,27 @@ int vmx_ctx::cmc_sweeps(int nsweeps) {
     VMX_TRY(cmc_part.alloc(3 * (size_t)nblocks * sizeof(double)));
     const SysDev<R> s = sys<R>();
     const Spin3<R> S = spins<R>(s_idx);
+    int last = -1;                                 // the sweep's last colour with pairs
+    for (int k = 0; k < ncolours; ++k)
+        if (cmc_tab_start[k + 1] > cmc_tab_start[k]) last = k;
     for (int w = 0; w < nsweeps; ++w) {
-        for (int k = 0; k < ncolours; ++k) {
+        if (last < 0) { ++sweep; continue; }
+        VMX_HIP(Ops<R>::cmc_perm(stream, s, nrep, ncolours, cmc_mlen.as<int>(), sweep, cmc_perm.as<int4>()));
+        VMX_HIP(Ops<R>::replica_moment(stream, s, S, mm.as<double>(), rep_blk.as<int>(), nrep, cmc_part.as<double>(),
+                                       cmc_M.as<double>()));
+        for (int k = 0; k <= last; ++k) {
             const int ntab = cmc_tab_start[k + 1] - cmc_tab_start[k];
             if (ntab == 0) continue;
+            double* dm = cmc_dm.as<double>() + 3 * (size_t)cmc_tab_start[k];
             for (int round = 0; round < 2; ++round) {
-                VMX_HIP(Ops<R>::replica_moment(stream, s, S, mm.as<double>(), rep_blk.as<int>(), nrep,
-                                               cmc_part.as<double>(), cmc_M.as<double>()));
                 VMX_HIP(Ops<R>::cmc_pairs(stream, et, am(), s, S, mm.as<double>(), cmc_list.as<int>(),
-                                          cmc_tab.as<int4>() + cmc_tab_start[k], ntab, cmc_frame.as<CmcFrame<R>>(),
-                                          cmc_M.as<double>(), mc_sigma.as<double>(), cmc_move, k, round, sweep,
-                                          cmc_counts.as<double>()));
+                                          cmc_tab.as<int4>() + cmc_tab_start[k], ntab,
+                                          cmc_perm.as<int4>() + (size_t)(2 * k + round) * nrep,
+                                          cmc_frame.as<CmcFrame<R>>(), cmc_M.as<double>(), mc_sigma.as<double>(),
+                                          cmc_move, round, sweep, cmc_counts.as<double>(), dm));
+                if (k < last || round == 0)
+                    VMX_HIP(Ops<R>::moment_add(stream, cmc_range.as<int>() + (size_t)k * (nrep + 1), nrep, dm,
+                                               cmc_M.as<double>()));
             }
         }
         ++sweep;   // a constrained sweep advances sim::time like a Heun step (sim.cpp:623-628)

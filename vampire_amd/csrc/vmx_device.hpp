@@ -1330,8 +1330,9 @@ __device__ __forceinline__ bool mc_accept(float x, uint32_t word) {
 }
 
 // Atoms per thread of the Metropolis colour kernel: a workgroup takes 256 * McAtoms list entries.
-template <typename R> struct McAtoms { static constexpr int value = 2; };
-template <> struct McAtoms<double> { static constexpr int value = 1; };   // FP64: 96 VGPRs at two
+// One atom per thread measured faster than two in both precisions (FP32: 4 % in colour order, 2 %
+// in storage order, from the extra resident wavefronts; FP64: 96 VGPRs at two).
+template <typename R> struct McAtoms { static constexpr int value = 1; };
 
 // The moves of a thread's U atoms idx[] of replica rep (ok[u]: a real entry; head[u]: its stencil
 // id; local[u]: its caller index within the replica, the Philox counter).  The thread forms the
@@ -1535,6 +1536,46 @@ __device__ __forceinline__ void cmc_permutation(uint2 key, int colour, int round
     a = x;
 }
 
+// a^-1 mod m for 0 < a < m coprime with m < 2^31 (extended Euclid; remainders stay below m and
+// the coefficients below m in magnitude, so 32-bit arithmetic holds them)
+__device__ __forceinline__ uint32_t mod_inverse(uint32_t a, uint32_t m) {
+    int32_t t = 0, nt = 1;
+    uint32_t r = m, nr = a;
+    while (nr != 0u) {
+        const uint32_t q = r / nr;
+        const int32_t t2 = t - (int32_t)q * nt;
+        const uint32_t r2 = r - q * nr;
+        t = nt; nt = t2; r = nr; nr = r2;
+    }
+    return (uint32_t)(t < 0 ? t + (int32_t)m : t);
+}
+
+// Slots of a round: pair p of the permutation joins entries e = pi(2p) and e + a (mod m), and the
+// pairs of a round are independent (disjoint atoms of one colour), so a workgroup may take them
+// in any order.  Slot t takes the pair whose first entry is the t-th in list order:
+//   m even: the first entries are exactly the entries of b's parity, so slot t is entry 2t + (b & 1)
+//           (m/2 slots, all used);
+//   m odd:  slot t is entry t when j = (t - b) a^-1 mod m is even and below m - 1, else idle
+//           (m slots, (m - 1)/2 used).
+// Both spins of the lanes of a wavefront then lie in a short stretch of the list, not at random.
+// Returns false for an idle slot; e1 is then the first entry of pair 0 (b), a valid pair.
+__device__ __forceinline__ bool cmc_slot(int t, int m, int4 pr, int& e1, int& e2) {
+    const uint32_t a = (uint32_t)pr.x, b = (uint32_t)pr.y;
+    bool ok;
+    if ((m & 1) == 0) {
+        ok = t < (m >> 1);
+        e1 = ok ? 2 * t + (int)(b & 1u) : (int)b;
+    } else {
+        const uint32_t d = (uint32_t)t >= b ? (uint32_t)t - b : (uint32_t)t + (uint32_t)m - b;
+        const uint32_t j = (uint32_t)(((uint64_t)d * (uint32_t)pr.z) % (uint32_t)m);
+        ok = t < m && (j & 1u) == 0u && j < (uint32_t)(m - 1);
+        e1 = ok ? t : (int)b;
+    }
+    const uint32_t f = (uint32_t)e1 + a;
+    e2 = (int)(f >= (uint32_t)m ? f - (uint32_t)m : f);
+    return ok;
+}
+
 template <typename R>
 __device__ __forceinline__ void cmc_rotate(const R* M, R x0, R x1, R x2, R (&o)[3]) {   // cmc.cpp:363-365
     o[0] = M[0] * x0 + M[1] * x1 + M[2] * x2;
@@ -1571,29 +1612,36 @@ __device__ __forceinline__ unsigned int* cmc_wg_counts() {
     return counts;
 }
 
+// The workgroup's change of magnetisation, one partial per wavefront (LDS)
+__device__ __forceinline__ double* cmc_wg_dm() {
+    __shared__ double dm[4][3];
+    return &dm[0][0];
+}
+
 // One round of one colour.  tab[lb] = {first cmc-list entry of the (replica, colour) run,
-// replica, run length m, first pair}; a workgroup takes up to blockDim pairs of one run.
+// replica, run length m, first slot}; a workgroup (256 threads) takes 256 slots (cmc_slot) of one
+// run; perm[rep] = {a, b, a^-1 mod m} of the round's permutation (k_cmc_perm).
 // M: each replica's magnetisation sum(mu s) at the start of the round (double [nrep][3]);
 // mm: moments (mu_B); sigma: cone width per replica (move 0); counts: kRejectSlots slots of three
-// outcome counts per replica, each slot on its own cache line.
+// outcome counts per replica, each slot on its own cache line; dM[3 lb ..]: the workgroup's
+// sum of mu (s_new - s_old) over its accepted pairs, summed in a fixed order (lanes by a
+// shuffle tree, then the four wavefronts in order), so that the next round's M follows
+// without a pass over all spins.
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(256)
 k_cmc_pairs(SysDev<R> s, Spin3<R> S, const double* __restrict__ mm, const int* __restrict__ list,
-            const int4* __restrict__ tab, int ntab, const CmcFrame<R>* __restrict__ frame,
-            const double* __restrict__ M, const double* __restrict__ sigma, int move, int colour, int round,
-            uint64_t sweep, double* __restrict__ counts) {
+            const int4* __restrict__ tab, int ntab, const int4* __restrict__ perm,
+            const CmcFrame<R>* __restrict__ frame, const double* __restrict__ M, const double* __restrict__ sigma,
+            int move, int round, uint64_t sweep, double* __restrict__ counts, double* __restrict__ dM) {
     const int lb = logical_block(s.xcd_remap);
     if (lb >= ntab) return;                        // whole workgroup
     const int4 b = tab[lb];
-    const int rep = b.y, m = b.z, half = m >> 1;
-    const int p = b.w + (int)threadIdx.x;
-    const bool ok = p < half;
+    const int rep = b.y, m = b.z;
+    int e1, e2;
+    const bool ok = cmc_slot(b.w + (int)threadIdx.x, m, perm[rep], e1, e2);
     const uint2 key = s.rep_key[rep];
-    uint64_t pa, pb;
-    cmc_permutation(key, colour, round, sweep, m, pa, pb);
-    const uint64_t t1 = 2 * (uint64_t)(ok ? p : b.w);
-    const int i1 = list[b.x + (int)((pa * t1 + pb) % (uint64_t)m)];
-    const int i2 = list[b.x + (int)((pa * (t1 + 1) + pb) % (uint64_t)m)];
+    const int i1 = list[b.x + e1];
+    const int i2 = list[b.x + e2];
     const int m1 = s.nmat == 1 ? 0 : (int)s.type[i1];
     const int m2 = s.nmat == 1 ? 0 : (int)s.type[i2];
     const R a0 = S.x[i1], a1 = S.y[i1], a2 = S.z[i1];      // spin 1 initial
@@ -1663,10 +1711,25 @@ k_cmc_pairs(SysDev<R> s, Spin3<R> S, const double* __restrict__ mm, const int* _
         const unsigned long long hit = __ballot(ok && outcome == k);
         if (hit != 0ull && (int)__lane_id() == __ffsll(hit) - 1) atomicAdd(&cmc_wg_counts()[k], (unsigned)__popcll(hit));
     }
+    double dm[3] = {0.0, 0.0, 0.0};
+    if (ok && accept) {
+        dm[0] = mu1 * (double)f0 + mu2 * (double)d[0] - mu1 * (double)a0 - mu2 * (double)c0;
+        dm[1] = mu1 * (double)f1 + mu2 * (double)d[1] - mu1 * (double)a1 - mu2 * (double)c1;
+        dm[2] = mu1 * (double)f2 + mu2 * (double)d[2] - mu1 * (double)a2 - mu2 * (double)c2;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        for (int off = 32; off > 0; off >>= 1) dm[q] += __shfl_down(dm[q], off, 64);
+        if (__lane_id() == 0) cmc_wg_dm()[3 * (threadIdx.x >> 6) + q] = dm[q];
+    }
     __syncthreads();
-    if (threadIdx.x < 3 && cmc_wg_counts()[threadIdx.x] != 0u)
-        atomicAdd(&counts[((size_t)rep * kRejectSlots + (lb & (kRejectSlots - 1))) * kRejectStride + threadIdx.x],
-                  (double)cmc_wg_counts()[threadIdx.x]);
+    if (threadIdx.x < 3) {
+        const double* w = cmc_wg_dm();
+        dM[3 * (size_t)lb + threadIdx.x] = ((w[threadIdx.x] + w[3 + threadIdx.x]) + w[6 + threadIdx.x]) + w[9 + threadIdx.x];
+        if (cmc_wg_counts()[threadIdx.x] != 0u)
+            atomicAdd(&counts[((size_t)rep * kRejectSlots + (lb & (kRejectSlots - 1))) * kRejectStride + threadIdx.x],
+                      (double)cmc_wg_counts()[threadIdx.x]);
+    }
 }
 
 }  // namespace vmx

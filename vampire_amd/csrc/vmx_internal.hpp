@@ -62,12 +62,19 @@ struct Ops {
                                 uint64_t sweep, double* rejects);
     // constrained Monte Carlo: one round of one colour (tab[ntab] workgroups of up to 256 pairs)
     static hipError_t cmc_pairs(hipStream_t st, int et, int jm, const SysDev<R>& s, Spin3<R> S, const double* mm,
-                                const int* list, const int4* tab, int ntab, const CmcFrame<R>* frame, const double* M,
-                                const double* sigma, int move, int colour, int round, uint64_t sweep, double* counts);
+                                const int* list, const int4* tab, int ntab, const int4* perm, const CmcFrame<R>* frame,
+                                const double* M, const double* sigma, int move, int round, uint64_t sweep,
+                                double* counts, double* dM);
+    // the permutations of every (colour, round, replica) of a sweep: perm[(2 k + round) nrep + r] =
+    // {a, b, a^-1 mod m, 0} for the run length m = mlen[k nrep + r] (m < 2: {1, 0, 1, 0})
+    static hipError_t cmc_perm(hipStream_t st, const SysDev<R>& s, int nrep, int ncolours, const int* mlen,
+                               uint64_t sweep, int4* perm);
     // each replica's sum(mu s) (double [nrep][3]): per-workgroup partials over the one-atom block
     // table, then one workgroup per replica over its partials (rep_blk[r] .. rep_blk[r+1])
     static hipError_t replica_moment(hipStream_t st, const SysDev<R>& s, Spin3<R> S, const double* mm,
                                      const int* rep_blk, int nrep, double* partial, double* M);
+    // M[r] += the sum of the cmc_pairs partials dM[range[r] .. range[r+1]], in a fixed order
+    static hipError_t moment_add(hipStream_t st, const int* range, int nrep, const double* dM, double* M);
     // deterministic fitting step and its adjoint (isotropic uniform exchange); grids of 256
     static hipError_t fit_predictor(hipStream_t st, int am, const SysDev<R>& s, const FitPrm<R>& p, Spin3<R> h,
                                     Spin3<R> S, Spin3<R> P);
