@@ -489,7 +489,8 @@ def main():
     ap.add_argument("--no-configs", action="store_true", help="skip the C0 / C3 / C4 leg")
     ap.add_argument("--configs", default="c0,c3,c4", help="comma list of the workloads the configs leg runs")
     ap.add_argument("--legs", default="fp32_fused,fp64_fused,fp64_plain,fp32_midpoint",
-                    help="comma list of fp32_fused, fp64_fused, fp64_plain, fp32_midpoint (profiling runs use one)")
+                    help="comma list of fp32_fused, fp64_fused, fp64_plain, fp32_midpoint, fp64_midpoint "
+                         "(profiling runs use one)")
     ap.add_argument("--mc-sweeps", type=int, default=200, help="0 skips the Metropolis leg (C2 batch)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with three 256 MiB device-to-device copies (counter scale for traffic)")
@@ -511,7 +512,8 @@ def main():
 
     legs = [x for x in args.legs.split(",") if x]
     table = {"fp32_fused": (FP32, FUSED, "heun"), "fp64_fused": (FP64, FUSED, "heun"),
-             "fp64_plain": (FP64, PLAIN, "heun"), "fp32_midpoint": (FP32, FUSED, "midpoint")}
+             "fp64_plain": (FP64, PLAIN, "heun"), "fp32_midpoint": (FP32, FUSED, "midpoint"),
+             "fp64_midpoint": (FP64, FUSED, "midpoint")}
     res = {}
     for name in legs:
         prec, kern, integ = table[name]

@@ -459,10 +459,10 @@ __device__ __forceinline__ void spin_field(const SysDev<R>& s, const R* __restri
 
 // The same field from a staged row, with the atom's material record mc.
 // The spin field of a staged row: its neighbours' spins gathered, then summed in row order.  FP64
-// rows of eight or more slots go in two halves (the first half's loads and sum, then the second
-// half's): the same terms in the same order, with half the gathered values live at once, so the
+// rows of eight or more slots go in Parts parts (the first part's loads and sum, then the next
+// part's): the same terms in the same order, with a part's gathered values live at once, so the
 // FP64 kernels need fewer registers and keep more wavefronts per SIMD in flight.
-template <typename R, int ET, int AM, bool Split>
+template <typename R, int ET, int AM, int Parts>
 __device__ __forceinline__ void row_gather_field(const SysDev<R>& s, const SpinRsrc<R>& rs, const RowIdx<AM>& r,
                                                  const R* ju, const MatConst<R>& mc, R sx, R sy, R sz, R& fx,
                                                  R& fy, R& fz);
@@ -478,28 +478,29 @@ __device__ __forceinline__ void row_field(const SysDev<R>& s, const RowIdx<AM>& 
     add_anisotropy(s.ani, mc, sx, sy, sz, hx, hy, hz, fx, fy, fz);
 }
 
-template <typename R, int ET, int AM, bool Split>
+template <typename R, int ET, int AM, int Parts>
 __device__ __forceinline__ void row_gather_field(const SysDev<R>& s, const SpinRsrc<R>& rs, const RowIdx<AM>& r,
                                                  const R* ju, const MatConst<R>& mc, R sx, R sy, R sz, R& fx,
                                                  R& fy, R& fz) {
     constexpr int W = Access<AM>::W;
-    if constexpr (Split && sizeof(R) == 8 && W >= 8) {
+    if constexpr (Parts > 1 && sizeof(R) == 8 && W >= 8) {
         constexpr int K = (ET == ET_ISO) ? 1 : (ET == ET_VEC ? 3 : 9);
-        constexpr int JM = Access<AM>::JM, H = W / 2;
+        constexpr int JM = Access<AM>::JM, H = (W + Parts - 1) / Parts;
         R hx = R(0), hy = R(0), hz = R(0);
 #pragma unroll
-        for (int half = 0; half < 2; ++half) {
+        for (int part = 0; part < Parts; ++part) {
             R vx[H], vy[H], vz[H];
 #pragma unroll
-            for (int k = 0; k < H; ++k) rs.load(r.j[half * H + k], vx[k], vy[k], vz[k]);
+            for (int k = 0; k < H; ++k)
+                if (part * H + k < W) rs.load(r.j[part * H + k], vx[k], vy[k], vz[k]);
 #pragma unroll
             for (int k = 0; k < H; ++k) {
-                const int slot = half * H + k;
-                if (s.full_rows || slot < r.deg)
+                const int slot = part * H + k;
+                if (slot < W && (s.full_rows || slot < r.deg))
                     exchange_term<R, ET>(JM == JM_UNIFORM ? ju : s.jtab + (size_t)K * r.jd[slot], vx[k], vy[k],
                                          vz[k], hx, hy, hz);
             }
-            if (half == 0) __builtin_amdgcn_sched_barrier(0);   // the second half's loads stay here
+            if (part + 1 < Parts) __builtin_amdgcn_sched_barrier(0);   // the next part's loads stay here
         }
         add_anisotropy(s.ani, mc, sx, sy, sz, hx, hy, hz, fx, fy, fz);
     } else {
@@ -731,7 +732,7 @@ k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, Spin3<R> E, c
         if constexpr (kStaged) {
             RowIdx<AM> ri;
             fused_row_index(s, i, head[u], ri);
-            row_gather_field<R, ET, AM, true>(s, rs, ri, ju.v, mc, sx[u], sy[u], sz[u], fx, fy, fz);
+            row_gather_field<R, ET, AM, 2>(s, rs, ri, ju.v, mc, sx[u], sy[u], sz[u], fx, fy, fz);
         } else {
             spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m[u], sx[u], sy[u], sz[u], fx, fy, fz);
         }
@@ -759,12 +760,11 @@ k_fused_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> D, Spin3<R> E, c
 // (LLGHeun.cpp:257-269) with dS read back when the predictor kept it, else recomputed (FP64);
 // FP32 rebuilds A = S + dt/2 dS from S and S' and forms |A + dt/2 dS'|.  S_next may overwrite S.
 // Own data of all the thread's atoms first, as above.
-// Wavefronts per SIMD the corrector is compiled for: FP64 at four (128 registers, its rows gathered
-// in two halves; three wavefronts without the cap), FP32 unconstrained.
-template <typename R> struct CorrectorWaves { static constexpr int value = 1; };
-template <> struct CorrectorWaves<double> { static constexpr int value = 4; };
+// Parts the correctors gather their FP64 rows in (both rows of the recomputing variant too, so
+// that the register budget of the default path is not set by a whole row of doubles).
+template <typename R> struct CorrectorParts { static constexpr int value = 2; };
 template <typename R, int ET, int AM>
-__global__ void __launch_bounds__(512, CorrectorWaves<R>::value)
+__global__ void __launch_bounds__(512)
 k_fused_corrector(SysDev<R> s, Spin3Alias<R> S, Spin3<R> P, Spin3Alias<R> O, Spin3<R> D, Spin3<R> E,
                   const double* __restrict__ noise, uint32_t step_off) {
     constexpr int U = 1;                          // launched on the one-atom block table
@@ -800,6 +800,7 @@ k_fused_corrector(SysDev<R> s, Spin3Alias<R> S, Spin3<R> P, Spin3Alias<R> O, Spi
     stage_fused<R, AM>(s, rep);      // own data above is in flight meanwhile
     if (!ok[0]) return;
     const UniformJ<R, ET> ju(s, Access<AM>::JM);
+    constexpr int parts = CorrectorParts<R>::value;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int i = idx[u];
@@ -808,7 +809,7 @@ k_fused_corrector(SysDev<R> s, Spin3Alias<R> S, Spin3<R> P, Spin3Alias<R> O, Spi
         R fx, fy, fz;
         if constexpr (kStaged) {
             fused_row_index(s, i, head[u], ri);
-            row_gather_field<R, ET, AM, true>(s, rp, ri, ju.v, mc, px[u], py[u], pz[u], fx, fy, fz);
+            row_gather_field<R, ET, AM, parts>(s, rp, ri, ju.v, mc, px[u], py[u], pz[u], fx, fy, fz);
         } else {
             spin_field<R, ET, AM>(s, P.x, P.y, P.z, i, m[u], px[u], py[u], pz[u], fx, fy, fz);
         }
@@ -838,9 +839,7 @@ k_fused_corrector(SysDev<R> s, Spin3Alias<R> S, Spin3<R> P, Spin3Alias<R> O, Spi
                 R gx, gy, gz;
                 if constexpr (kStaged) {
                     const SpinRsrc<R> rs(s, S.x, S.y, S.z);
-                    RowSpins<R, AM> sv;
-                    row_load(rs, ri, sv);
-                    row_field<R, ET, AM>(s, ri, sv, ju.v, mc, ax[u], ay[u], az[u], gx, gy, gz);
+                    row_gather_field<R, ET, AM, parts>(s, rs, ri, ju.v, mc, ax[u], ay[u], az[u], gx, gy, gz);
                 } else {
                     spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m[u], ax[u], ay[u], az[u], gx, gy, gz);
                 }
@@ -922,7 +921,7 @@ k_mid_predictor(SysDev<R> s, Spin3<R> S, Spin3<R> P, Spin3<R> E, const double* _
         if constexpr (kStaged) {
             RowIdx<AM> ri;
             fused_row_index(s, i, head[u], ri);
-            row_gather_field<R, ET, AM, true>(s, rs, ri, ju.v, mc, sx[u], sy[u], sz[u], fx, fy, fz);
+            row_gather_field<R, ET, AM, 2>(s, rs, ri, ju.v, mc, sx[u], sy[u], sz[u], fx, fy, fz);
         } else {
             spin_field<R, ET, AM>(s, S.x, S.y, S.z, i, m[u], sx[u], sy[u], sz[u], fx, fy, fz);
         }
@@ -983,7 +982,8 @@ k_mid_corrector(SysDev<R> s, Spin3Alias<R> S, Spin3<R> P, Spin3Alias<R> O, Spin3
         if constexpr (kStaged) {
             RowIdx<AM> ri;
             fused_row_index(s, i, head[u], ri);
-            row_gather_field<R, ET, AM, false>(s, rp, ri, ju.v, mc, px[u], py[u], pz[u], fx, fy, fz);
+            row_gather_field<R, ET, AM, CorrectorParts<R>::value>(s, rp, ri, ju.v, mc, px[u], py[u], pz[u], fx, fy,
+                                                                  fz);
         } else {
             spin_field<R, ET, AM>(s, P.x, P.y, P.z, i, m[u], px[u], py[u], pz[u], fx, fy, fz);
         }
