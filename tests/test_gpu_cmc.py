@@ -63,6 +63,60 @@ def test_cmc_odd_colour_runs_match_oracle(gpu, topology):
     assert counts[0] > 0 and counts[1] > 0 and counts[2] > 0
 
 
+def _isolated_spins(n):
+    """n spins with no neighbours and uniaxial anisotropy: one colour of n atoms."""
+    mat = S.Material(mu_s_muB=1.72, alpha=1.0, Ku=5e-24)
+    dt = 1e-16 * S.GAMMA_SI
+    return S.System(n=n, type=np.zeros(n, np.int32), moment=np.full(n, 1.72), nbr_start=np.zeros(n, np.int32),
+                    nbr_end=np.full(n, -1, np.int32), nbr=np.zeros(0, np.int32), nbr_iid=np.zeros(0, np.int32),
+                    exchange_type=0, exchange=np.zeros((1, 1)), materials=[S.derived_material(mat, dt)],
+                    anisotropy_flags=1, dt=dt, half_dt=0.5 * dt)
+
+
+def test_cmc_smallest_runs_match_oracle(gpu):
+    """Colour runs of 1, 2, 3, 5 and 7 atoms (isolated spins, one colour each): a run of one atom
+    has no pair and keeps its spin; the others equal the oracle's sweeps to 1e-12 with equal counts,
+    alone and as replicas of one batch (which equals the runs alone bit for bit)."""
+    sizes = [1, 2, 3, 5, 7]
+    systems = [_isolated_spins(n) for n in sizes]
+    spins = []
+    for r, n in enumerate(sizes):
+        sp = S.random_spins(n, 3 + r)
+        sp[2] = np.abs(sp[2]) + 0.5
+        spins.append(sp / np.linalg.norm(sp, axis=0))
+    alone = []
+    for r, x in enumerate(systems):
+        o = O.Oracle(x)
+        o.set_conditions(300.0, (0.0, 0.0, 0.5))
+        ref, counts = o.cmc_checkerboard(spins[r], 20, o.cmc(40.0, 30.0, sigma=40.0), seed=5 + r)
+        with Context(x, precision=FP64, seed=5 + r) as c:
+            c.set_conditions(300.0, (0.0, 0.0, 0.5))
+            c.set_spins(spins[r])
+            c.mc_set_sigma(40.0)
+            c.cmc_set_constraint(40.0, 30.0)
+            c.cmc_sweep(20)
+            dev, dcounts = c.get_spins(), c.cmc_counts()[0]
+        assert np.abs(dev - ref).max() <= 1e-12
+        assert np.array_equal(dcounts, counts)
+        if sizes[r] == 1:
+            assert np.array_equal(dev, spins[r]) and not counts.any()
+        else:
+            assert counts[0] > 0
+        alone.append((dev, dcounts))
+    batch = S.concat_systems(systems)
+    with Context(batch, precision=FP64, seed=5) as c:
+        c.set_conditions(300.0, (0.0, 0.0, 0.5))
+        c.set_spins(np.concatenate(spins, axis=1))
+        c.mc_set_sigma(40.0)
+        c.cmc_set_constraint(40.0, 30.0)
+        c.cmc_sweep(20)
+        got, gcounts = c.get_spins(), c.cmc_counts()
+    off = batch.replica_offset
+    for r in range(len(sizes)):
+        assert np.array_equal(got[:, off[r]:off[r + 1]], alone[r][0])
+        assert np.array_equal(gcounts[r], alone[r][1])
+
+
 def test_cmc_two_materials_tensor_exchange_matches_oracle(gpu):
     """The thin film with two materials (moments 1.6 and 7.6 mu_B: the compensating spin scales by
     mu1/mu2) and tensor exchange with DMI: device equals the oracle to 1e-12 with equal counts."""
