@@ -83,6 +83,16 @@ def dipole_args(name: str, spins_key: str = "spins_in"):
                 atom_cell=f["atom_cell"], volume=f["volume"], tensor=f["tensor"], update_rate=int(f["update_rate"]))
 
 
+def isolated_spins(n: int) -> S.System:
+    """n spins with no neighbours and uniaxial anisotropy (one colour of n atoms)."""
+    mat = S.Material(mu_s_muB=1.72, alpha=1.0, Ku=5e-24)
+    dt = 1e-16 * S.GAMMA_SI
+    return S.System(n=n, type=np.zeros(n, np.int32), moment=np.full(n, 1.72), nbr_start=np.zeros(n, np.int32),
+                    nbr_end=np.full(n, -1, np.int32), nbr=np.zeros(0, np.int32), nbr_iid=np.zeros(0, np.int32),
+                    exchange_type=0, exchange=np.zeros((1, 1)), materials=[S.derived_material(mat, dt)],
+                    anisotropy_flags=1, dt=dt, half_dt=0.5 * dt)
+
+
 def block_mean(samples, nblocks: int = 10):
     """Mean over axis 0 and its error from `nblocks` block means (autocorrelation-aware)."""
     bm = np.array([b.mean(axis=0) for b in np.array_split(np.asarray(samples), nblocks)])

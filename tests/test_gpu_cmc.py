@@ -63,22 +63,12 @@ def test_cmc_odd_colour_runs_match_oracle(gpu, topology):
     assert counts[0] > 0 and counts[1] > 0 and counts[2] > 0
 
 
-def _isolated_spins(n):
-    """n spins with no neighbours and uniaxial anisotropy: one colour of n atoms."""
-    mat = S.Material(mu_s_muB=1.72, alpha=1.0, Ku=5e-24)
-    dt = 1e-16 * S.GAMMA_SI
-    return S.System(n=n, type=np.zeros(n, np.int32), moment=np.full(n, 1.72), nbr_start=np.zeros(n, np.int32),
-                    nbr_end=np.full(n, -1, np.int32), nbr=np.zeros(0, np.int32), nbr_iid=np.zeros(0, np.int32),
-                    exchange_type=0, exchange=np.zeros((1, 1)), materials=[S.derived_material(mat, dt)],
-                    anisotropy_flags=1, dt=dt, half_dt=0.5 * dt)
-
-
 def test_cmc_smallest_runs_match_oracle(gpu):
     """Colour runs of 1, 2, 3, 5 and 7 atoms (isolated spins, one colour each): a run of one atom
     has no pair and keeps its spin; the others equal the oracle's sweeps to 1e-12 with equal counts,
     alone and as replicas of one batch (which equals the runs alone bit for bit)."""
     sizes = [1, 2, 3, 5, 7]
-    systems = [_isolated_spins(n) for n in sizes]
+    systems = [RC.isolated_spins(n) for n in sizes]
     spins = []
     for r, n in enumerate(sizes):
         sp = S.random_spins(n, 3 + r)

@@ -95,6 +95,43 @@ def test_mc_checkerboard_matches_oracle(gpu, topology):
     assert abs(dsig[0] - sigma) <= 1e-12 * sigma
 
 
+def test_mc_smallest_runs_match_oracle(gpu):
+    """Isolated spins (5 K, 5 T) in colour runs of 1, 2, 3, 5 and 7 atoms: FP64 sweeps equal the
+    oracle's to 1e-12 with equal move and rejection counts and cone width, in calls of one sweep
+    (storage order) and of eight (colour order); the batch of all five equals the runs alone bit for
+    bit."""
+    sizes = [1, 2, 3, 5, 7]
+    systems = [RC.isolated_spins(n) for n in sizes]
+    spins = [S.random_spins(n, 60 + r) for r, n in enumerate(sizes)]
+    alone = []
+    for r, x in enumerate(systems):
+        o = O.Oracle(x)
+        o.set_conditions(5.0, (0.0, 0.0, 5.0))
+        ref, sigma, moves, rej = o.mc_checkerboard(spins[r], 8, seed=9 + r, sweep0=0, sigma=60.0)
+        assert rej > 0
+        for per_call in (1, 8):
+            with Context(x, precision=FP64, seed=9 + r) as c:
+                c.set_conditions(5.0, (0.0, 0.0, 5.0))
+                c.set_spins(spins[r])
+                for _ in range(8 // per_call):
+                    c.mc_sweep(per_call)
+                dev = c.get_spins()
+                dsig, dmov, drej = c.mc_state()
+            assert np.abs(dev - ref).max() <= 1e-12
+            assert dmov[0] == moves and drej[0] == rej
+            assert abs(dsig[0] - sigma) <= 1e-12 * sigma
+        alone.append(dev)
+    batch = S.concat_systems(systems)
+    with Context(batch, precision=FP64, seed=9) as c:
+        c.set_conditions(5.0, (0.0, 0.0, 5.0))
+        c.set_spins(np.concatenate(spins, axis=1))
+        c.mc_sweep(8)
+        got = c.get_spins()
+    off = batch.replica_offset
+    for r in range(len(sizes)):
+        assert np.array_equal(got[:, off[r]:off[r + 1]], alone[r])
+
+
 @pytest.mark.parametrize("precision", [FP64, FP32])
 @pytest.mark.parametrize("case", ["sc_batch", "fcc", "film_dmi", "bcc_rotated"])
 def test_mc_colour_order_equals_storage_order(gpu, case, precision):
