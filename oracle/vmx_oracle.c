@@ -43,6 +43,17 @@
  *                         and statistic energies (src/simulate/energy.cpp:123-125, 171;
  *                         src/statistics/energy.cpp:198-208)
  *
+ * Pins (tests/test_oracle.py; every function above has one, none is left unpinned):
+ *   or_spin_fields, or_external_fields, or_llg_heun, or_llg_midpoint, or_dipole_update: bit-identical
+ *     to the reference's own runs of its input decks (golden vectors written by oracle/ref's harness,
+ *     built from the reference's sources), plus closed forms (damped precession, pair sums,
+ *     -dE/dS of each anisotropy) and second-order convergence;
+ *   or_magnetization, or_energy: the reference's statistic objects on the same decks (<= 1e-12);
+ *   or_philox4x32_10: the Random123 known-answer vectors;
+ *   or_mc_random_site, or_mc_checkerboard, or_cmc_*: the single-spin Boltzmann density and the
+ *     reference's own mc_step / cmc_step runs (M(T), E(T), outcome fractions, within error bars);
+ *     the constraint frame against its closed form.
+ *
  * Noise: the reference draws MT19937 + Ziggurat normals (src/random/random.cpp:219-244), x for all
  * atoms, then y, then z (fields.cpp:274-276).  The oracle takes either an explicit noise array in
  * that order or generates the device's documented stream: Philox-4x32-10 (Salmon et al., SC'11)
