@@ -1196,6 +1196,36 @@ __device__ __forceinline__ R row_exchange_energy(const SysDev<R>& s, const RowId
     return energy;
 }
 
+// The exchange energies of a staged row before (o) and after (n) a move, each summed in row order.
+// FP64 rows are gathered in Parts parts (a part's loads, then its terms of both sums): the same
+// terms in the same order, with a part's neighbour values live at once instead of the whole row's.
+template <typename R, int ET, int AM, int Parts>
+__device__ __forceinline__ void row_energy_pair(const SysDev<R>& s, const SpinRsrc<R>& rs, const RowIdx<AM>& r,
+                                                const R* ju, R ox, R oy, R oz, R nx, R ny, R nz, R& e_old,
+                                                R& e_new) {
+    constexpr int K = (ET == ET_ISO) ? 1 : (ET == ET_VEC ? 3 : 9);
+    constexpr int JM = Access<AM>::JM, W = Access<AM>::W, H = (W + Parts - 1) / Parts;
+    e_old = R(0);
+    e_new = R(0);
+#pragma unroll
+    for (int part = 0; part < Parts; ++part) {
+        R vx[H], vy[H], vz[H];
+#pragma unroll
+        for (int k = 0; k < H; ++k)
+            if (part * H + k < W) rs.load(r.j[part * H + k], vx[k], vy[k], vz[k]);
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+            const int slot = part * H + k;
+            if (slot < W && (s.full_rows || slot < r.deg)) {
+                const R* J = JM == JM_UNIFORM ? ju : s.jtab + (size_t)K * r.jd[slot];
+                exchange_energy_term<R, ET>(J, vx[k], vy[k], vz[k], ox, oy, oz, e_old);
+                exchange_energy_term<R, ET>(J, vx[k], vy[k], vz[k], nx, ny, nz, e_new);
+            }
+        }
+        if (part + 1 < Parts) __builtin_amdgcn_sched_barrier(0);   // the next part's loads stay here
+    }
+}
+
 // Anisotropy energy with the material record mc (from LDS or memory).
 template <typename R>
 __device__ __forceinline__ R anisotropy_energy(uint32_t ani, const MatConst<R>& mc, R sx, R sy, R sz) {
@@ -1374,7 +1404,15 @@ __device__ __forceinline__ void mc_moves(const SysDev<R>& s, const Spin3<R>& S, 
         nx[u] *= r; ny[u] *= r; nz[u] *= r;
     }
     R x_old[U], x_new[U];
-    if constexpr (kStaged) {
+    if constexpr (kStaged && sizeof(R) == 8) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            RowIdx<AM> ri;
+            fused_row_index(s, idx[u], head[u], ri);
+            row_energy_pair<R, ET, AM, 2>(s, rs, ri, ju.v, ox[u], oy[u], oz[u], nx[u], ny[u], nz[u], x_old[u],
+                                          x_new[u]);
+        }
+    } else if constexpr (kStaged) {
         RowIdx<AM> ri[U];
         RowSpins<R, AM> rv[U];
 #pragma unroll
@@ -1591,7 +1629,11 @@ __device__ __forceinline__ void move_energies(const SysDev<R>& s, const Spin3<R>
                                               const MatConst<R>& mc, const Cond<R>& c, R ox, R oy, R oz, R nx, R ny,
                                               R nz, R& e_old, R& e_new) {
     R x_old, x_new;
-    if constexpr (Access<AM>::TP != TP_CSR) {
+    if constexpr (Access<AM>::TP != TP_CSR && sizeof(R) == 8) {
+        RowIdx<AM> ri;
+        fused_row_index(s, i, head, ri);
+        row_energy_pair<R, ET, AM, 2>(s, rs, ri, ju, ox, oy, oz, nx, ny, nz, x_old, x_new);
+    } else if constexpr (Access<AM>::TP != TP_CSR) {
         RowIdx<AM> ri;
         RowSpins<R, AM> rv;
         fused_row_index(s, i, head, ri);
