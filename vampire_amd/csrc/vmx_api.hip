@@ -106,6 +106,16 @@ struct PinnedRing {
     }
 };
 
+// Small host->device updates (a few condition records) written by one workgroup whose kernel
+// arguments carry the bytes: they travel in the dispatch packet and the write is ordered on the
+// stream like the copy it replaces, without a DMA transfer between kernel launches.
+constexpr int kArgWords = 256;                    // 1 KiB of kernel arguments
+struct ArgWords { uint32_t w[kArgWords]; };
+__global__ void __launch_bounds__(kArgWords) k_put_words(uint32_t* __restrict__ dst, ArgWords v, int n) {
+    const int k = (int)threadIdx.x;
+    if (k < n) dst[k] = v.w[k];
+}
+
 // ---- scratch of the statistics update that allocates nothing -----------------------------------
 // vmx_stats_update_noalloc keeps its per-workgroup partial sums in this file-scope device array
 // instead of a buffer the library allocates.  Every GPU the library runs on holds its own copy
@@ -764,8 +774,17 @@ struct vmx_ctx {
             }
         }
         VMX_TRY(cond.alloc(cv.size() * sizeof(Cond<R>)));
-        if (ring.bytes < cv.size() * sizeof(Cond<R>)) return fail(VMX_ERR_STATE, "condition table exceeds staging");
-        return ring.copy(cond.p, cv.data(), cv.size() * sizeof(Cond<R>), stream);
+        const size_t b = cv.size() * sizeof(Cond<R>);
+        if (b % 4 == 0 && b <= sizeof(ArgWords)) {
+            ArgWords v;
+            std::memcpy(v.w, cv.data(), b);
+            hipLaunchKernelGGL(k_put_words, dim3(1), dim3(kArgWords), 0, stream, cond.as<uint32_t>(), v,
+                               (int)(b / 4));
+            VMX_HIP(hipGetLastError());
+            return VMX_OK;
+        }
+        if (ring.bytes < b) return fail(VMX_ERR_STATE, "condition table exceeds staging");
+        return ring.copy(cond.p, cv.data(), b, stream);
     }
     int upload_conditions_any() {
         return precision == VMX_FP64 ? upload_conditions<double>() : upload_conditions<float>();
