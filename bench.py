@@ -197,8 +197,9 @@ def metropolis_leg(sweeps: int, warmup: int = 5):
 
 def configs_leg(c0_steps: int = 10000, c3_steps: int = 2000, c4_steps: int = 1000, only=("c0", "c3", "c4")):
     """The other workloads of the path, timed on the device:
-    C0  SC 20^3 (8000 atoms), 300 K, FP64 fused, HIP-graph replay; the oracle on one host thread
-        for 1000 steps beside it (the reference's own CPU-runnable case);
+    C0  SC 20^3 (8000 atoms), 300 K, FP64 fused, HIP-graph replay at the autotuned launch shape
+        (8000 atoms fill few CUs, so the shape matters); the oracle on one host thread for 1000 steps
+        beside it (the reference's own CPU-runnable case);
     C3  BCC 32^3 (65,536 atoms), deterministic Heun under Hz = 1 T from a 30-degree tilt, 2000 steps,
         checkpoint every 100: trajectory forward and hand-written adjoint backward, FP32 and FP64;
     C4  ferrimagnetic SC film 256x256x8 with interfacial DMI (3x3 tensor exchange), 10% dilution
@@ -223,12 +224,15 @@ def _c0_leg(S, FP64, FUSED, Context, c0_steps):
     c = Context(s0, precision=FP64, kernel=FUSED, seed=1, device=0)
     c.set_conditions(300.0)
     c.set_spins(sp0)
+    c.autotune(20)                # launch shape for this system, outside the timed region
+    c.set_spins(sp0)
     c.llg_heun_graph(100, 1)
     c.synchronize()
     t0 = time.perf_counter()
     c.llg_heun_graph(100, c0_steps // 100)
     c.synchronize()
     dt = time.perf_counter() - t0
+    launch = c.launch_config
     c.close()
     from oracle import oracle as O
     o = O.Oracle(s0)
@@ -238,6 +242,7 @@ def _c0_leg(S, FP64, FUSED, Context, c0_steps):
     cpu = s0.n * 1000 / (time.perf_counter() - t1)
     return {"workload": "SC 20^3, 8000 atoms, 300 K, FP64 fused, graph replay", "steps": c0_steps,
             "seconds": round(dt, 4), "spin_steps_per_s": round(s0.n * c0_steps / dt, 1),
+            "launch": {"block": launch[0], "xcd_remap": launch[1]},
             "cpu_1thread_spin_steps_per_s": round(cpu, 1)}
 
 
@@ -283,11 +288,13 @@ def _c4_leg(S, FP32, FUSED, Context, c4_steps):
     c.synchronize()
     dt = time.perf_counter() - t0
     k4 = c.time_kernels(20)
+    launch = c.launch_config
     c.close()
     return {"workload": "SC film 256x256x8, DMI tensor exchange, 10% dilution, 472014 atoms, FP32 fused, "
                         "per-step Gaussian T pulse 300->1200 K", "atoms": s4.n, "steps": c4_steps,
             "topology": name, "stencils": npat, "seconds": round(dt, 4),
-            "spin_steps_per_s": round(s4.n * c4_steps / dt, 1), "kernel_ms": k4.tolist()}
+            "spin_steps_per_s": round(s4.n * c4_steps / dt, 1), "kernel_ms": k4.tolist(),
+            "launch": {"block": launch[0], "xcd_remap": launch[1]}}
 
 
 def drift_leg(system, steps: int, c0_steps: int = 1000):
