@@ -238,6 +238,38 @@ def test_graph_equals_eager(gpu):
     assert np.array_equal(res[0], res[1])
 
 
+def test_eager_and_graph_interleaved_equal_eager(gpu):
+    """Eager calls carry their step as an offset from the device step counter and leave it where
+    it is; a graph replay reads and advances the counter itself.  Eager calls, replays and a jump
+    of the step counter interleaved equal the same steps taken eagerly, bit for bit, with the
+    same thermal noise (the step is the Philox counter), in both precisions."""
+    s = S.ferromagnet("fcc", 6)
+    spins = S.random_spins(s.n, 9)
+    for precision in (FP64, FP32):
+        res = []
+        for use_graph in (False, True):
+            with _ctx(s, seed=5, precision=precision) as c:
+                def tens(n):     # n steps, as graph replays of 10 or eagerly
+                    if use_graph:
+                        c.llg_heun_graph(10, n // 10)
+                    else:
+                        c.llg_heun(n)
+                c.set_conditions(500.0, (0.0, 0.0, 0.5))
+                c.set_spins(spins)
+                c.llg_heun(7)
+                tens(20)
+                c.llg_heun(5)
+                tens(10)
+                assert c.step == 42
+                c.step = 1000
+                c.llg_heun(3)
+                tens(10)
+                c.llg_heun(1)
+                assert c.step == 1014
+                res.append(c.get_spins())
+        assert np.array_equal(res[0], res[1])
+
+
 def test_replicas_equal_standalone(gpu):
     """A batch of uneven replicas equals each replica run alone with seed + r."""
     sizes = [3, 5, 4]

@@ -159,6 +159,7 @@ struct EnergyStat {
 };
 
 __global__ void k_advance_step(unsigned long long* base, unsigned int n) { *base += n; }
+__global__ void k_set_step(unsigned long long* base, unsigned long long v) { *base = v; }
 
 // adds dL/dS_z (the same for every atom) to the adjoint of a recorded state
 template <typename R>
@@ -448,7 +449,26 @@ struct vmx_ctx {
     std::vector<int> type_h;         // material ids, caller order
     uint32_t ani = 0, fld = 0;
     double dt = 0.0, half_dt = 0.0;
-    uint64_t step = 0;       // host mirror of *step_base
+    uint64_t step = 0;       // Heun / midpoint steps taken (the Philox step counter)
+    // The value *step_base holds once the work queued so far has run.  Eager launches pass their
+    // step as an offset from it (step - dev_step + k), so they need no device-side advance; a
+    // graph replay reads and advances the device counter itself, so it is brought to `step`
+    // first (sync_step_base).
+    uint64_t dev_step = 0;
+    int sync_step_base() {
+        if (dev_step == step) return VMX_OK;
+        hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, stream, step_base.as<unsigned long long>(),
+                           (unsigned long long)step);
+        VMX_HIP(hipGetLastError());
+        dev_step = step;
+        return VMX_OK;
+    }
+    // the offset of an eager call's first step from *step_base, with room for nsteps more
+    int eager_step_off(int nsteps, uint32_t& off) {
+        if (step - dev_step + (uint64_t)nsteps >= (1ull << 31)) VMX_TRY(sync_step_base());
+        off = (uint32_t)(step - dev_step);
+        return VMX_OK;
+    }
     uint64_t sweep = 0;
 
     std::vector<vmx_material> mats;
@@ -851,6 +871,8 @@ template <typename R>
 int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
     const SysDev<R> s = sys<R>(), sf = sys_fused<R>();
     const Launch L = launch(), Lf = launch_fused();
+    uint32_t off0 = 0;
+    VMX_TRY(eager_step_off(nsteps, off0));
     const double* nd = nullptr;
     const bool normals64 = normals_from_fp64<R>(noise);
     if (noise || normals64) {
@@ -885,11 +907,11 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
         if (kernel == VMX_KERNEL_FUSED) {
             VMX_HIP(mark(0));
             VMX_HIP(Ops<R>::fused_predictor(Lf, et, am(), sf, spins<R>(s_idx), spins<R>(p_idx), euler<R>(),
-                                            field<R>(), nd, (uint32_t)k));
+                                            field<R>(), nd, off0 + (uint32_t)k));
             VMX_HIP(mark(1));
             const int out = corrector_in_place() ? s_idx : o_idx;
             VMX_HIP(Ops<R>::fused_corrector(L, et, am(), s, spins<R>(s_idx), spins<R>(p_idx), spins<R>(out),
-                                            euler<R>(), field<R>(), nd, (uint32_t)k));
+                                            euler<R>(), field<R>(), nd, off0 + (uint32_t)k));
             VMX_HIP(mark(2));
             nk = 2;
             if (out == o_idx) std::swap(s_idx, o_idx);
@@ -898,7 +920,7 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
             VMX_HIP(mark(0));
             VMX_HIP(Ops<R>::spin_fields(L, et, am(),s, spins<R>(s_idx), F));
             VMX_HIP(mark(1));
-            VMX_HIP(Ops<R>::ext_fields(L, s, E, nd, (uint32_t)k));
+            VMX_HIP(Ops<R>::ext_fields(L, s, E, nd, off0 + (uint32_t)k));
             VMX_HIP(mark(2));
             VMX_HIP(Ops<R>::plain_predictor(L, s, spins<R>(s_idx), F, E, D, spins<R>(p_idx)));
             VMX_HIP(mark(3));
@@ -924,10 +946,7 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
         for (int e = 0; e < 6; ++e) (void)hipEventDestroy(kev[e]);
     VMX_HIP(hipEventRecord(ev1, stream));
     timed_steps = nsteps;
-    hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(1), 0, stream, step_base.as<unsigned long long>(),
-                       (unsigned)nsteps);
-    VMX_HIP(hipGetLastError());
-    step += (uint64_t)nsteps;
+    step += (uint64_t)nsteps;   // *step_base stays: the next eager call's offset grows instead
     if (dip.on) dip.time += nsteps;
     return VMX_OK;
 }
@@ -938,6 +957,8 @@ template <typename R>
 int vmx_ctx::midpoint_steps(int nsteps, const double* noise) {
     const SysDev<R> s = sys<R>(), sf = sys_fused<R>();
     const Launch L = launch(), Lf = launch_fused();
+    uint32_t off0 = 0;
+    VMX_TRY(eager_step_off(nsteps, off0));
     const double* nd = nullptr;
     const bool normals64 = normals_from_fp64<R>(noise);
     if (noise || normals64) {
@@ -955,18 +976,16 @@ int vmx_ctx::midpoint_steps(int nsteps, const double* noise) {
             VMX_TRY(draw_fp64_normals(step + (uint64_t)k));
         }
         const int p_idx = 3 - s_idx - o_idx;
-        VMX_HIP(Ops<R>::mid_predictor(Lf, et, am(), sf, spins<R>(s_idx), spins<R>(p_idx), field<R>(), nd, (uint32_t)k));
+        VMX_HIP(Ops<R>::mid_predictor(Lf, et, am(), sf, spins<R>(s_idx), spins<R>(p_idx), field<R>(), nd,
+                                      off0 + (uint32_t)k));
         // the midpoint corrector gathers only the midpoint: S_next overwrites S
         VMX_HIP(Ops<R>::mid_corrector(L, et, am(), s, spins<R>(s_idx), spins<R>(p_idx), spins<R>(s_idx), field<R>(),
-                                      nd, (uint32_t)k));
+                                      nd, off0 + (uint32_t)k));
         if (dip.on && (dip.time + k + 1) % dip.rate == 0) VMX_TRY(dipole_refresh<R>());
     }
     VMX_HIP(hipEventRecord(ev1, stream));
     timed_steps = nsteps;
-    hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(1), 0, stream, step_base.as<unsigned long long>(),
-                       (unsigned)nsteps);
-    VMX_HIP(hipGetLastError());
-    step += (uint64_t)nsteps;
+    step += (uint64_t)nsteps;   // *step_base stays: the next eager call's offset grows instead
     if (dip.on) dip.time += nsteps;
     return VMX_OK;
 }
@@ -2189,11 +2208,13 @@ int vmx_llg_heun_graph(vmx_ctx* c, int32_t nsteps, int32_t nrepeat) {
     if (!c->graph_exec || c->graph_key != c->graph_key_of(nsteps)) {
         VMX_TRY(c->precision == VMX_FP64 ? c->capture_graph<double>(nsteps) : c->capture_graph<float>(nsteps));
     }
+    VMX_TRY(c->sync_step_base());   // the graph's kernels read and advance *step_base
     VMX_HIP(hipEventRecord(c->ev0, c->stream));
     for (int r = 0; r < nrepeat; ++r) VMX_HIP(hipGraphLaunch(c->graph_exec, c->stream));
     VMX_HIP(hipEventRecord(c->ev1, c->stream));
     c->timed_steps = nsteps * nrepeat;
     c->step += (uint64_t)nsteps * nrepeat;
+    c->dev_step = c->step;
     return VMX_OK;
 }
 
@@ -2210,6 +2231,7 @@ int vmx_set_step(vmx_ctx* c, int64_t step) {
     const unsigned long long v = (unsigned long long)step;
     VMX_HIP(hipMemcpy(c->step_base.p, &v, sizeof v, hipMemcpyHostToDevice));
     c->step = (uint64_t)step;
+    c->dev_step = c->step;
     return VMX_OK;
 }
 
