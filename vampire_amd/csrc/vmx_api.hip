@@ -2150,6 +2150,14 @@ int vmx_set_conditions(vmx_ctx* c, int32_t replica, double temperature, const do
     if (!(temperature >= 0.0) || !std::isfinite(temperature)) return fail(VMX_ERR_ARG, "temperature must be >= 0");
     VMX_TRY(c->use_device());
     const int r0 = replica < 0 ? 0 : replica, r1 = replica < 0 ? c->nrep : replica + 1;
+    // the device table already holds these conditions (every change is uploaded): nothing to do,
+    // as for a program that sets the same temperature and field before every step
+    bool same = (field_flags & 0x7u) == c->fld;
+    for (int r = r0; r < r1 && same; ++r) {
+        same = c->T_h[r] == temperature;
+        for (int k = 0; k < 3; ++k) same = same && c->H_h[3 * r + k] == H[k];
+    }
+    if (same) return VMX_OK;
     for (int r = r0; r < r1; ++r) {
         c->T_h[r] = temperature;
         for (int k = 0; k < 3; ++k) c->H_h[3 * r + k] = H[k];
