@@ -334,7 +334,9 @@ int vmx_thermal_noise(vmx_ctx* ctx, int64_t step, double* noise);
 int vmx_synchronize(vmx_ctx* ctx);
 
 /* Average device time (ms) of the last `vmx_llg_heun` call's kernels per Heun step, measured
- * with HIP events on the context's stream (0 if not measured). */
+ * with HIP events on the context's stream (0 if not measured: a call of one step records no
+ * events, since they would cost it more than its own launches; calls of two or more steps and
+ * graph replays are timed). */
 int vmx_last_step_time_ms(vmx_ctx* ctx, double* ms);
 
 /* Runs nsteps Heun steps (Philox noise) with HIP events around every kernel launch and returns

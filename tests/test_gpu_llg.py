@@ -270,6 +270,26 @@ def test_eager_and_graph_interleaved_equal_eager(gpu):
         assert np.array_equal(res[0], res[1])
 
 
+def test_step_timing_of_calls(gpu):
+    """Calls of two or more steps and graph replays are timed with events on the stream; a call of
+    one step is not (its events would cost more than its launches) and reports 0, not the timing
+    of an earlier call."""
+    s = S.ferromagnet("fcc", 6)
+    with _ctx(s, seed=2, precision=FP32) as c:
+        c.set_conditions(300.0)
+        c.set_spins(S.random_spins(s.n, 2))
+        c.llg_heun(20)
+        assert c.last_step_time_ms() > 0.0
+        c.llg_heun(1)
+        assert c.last_step_time_ms() == 0.0
+        c.llg_heun_graph(4, 2)
+        assert c.last_step_time_ms() > 0.0
+        c.llg_midpoint(1)
+        assert c.last_step_time_ms() == 0.0
+        c.llg_midpoint(3)
+        assert c.last_step_time_ms() > 0.0
+
+
 def test_replicas_equal_standalone(gpu):
     """A batch of uneven replicas equals each replica run alone with seed + r."""
     sizes = [3, 5, 4]

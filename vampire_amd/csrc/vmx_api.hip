@@ -891,7 +891,10 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
         for (int e = 0; e < 5; ++e) kernel_ms[e] = 0.0;
     }
     auto mark = [&](int e) -> hipError_t { return kernel_ms ? hipEventRecord(kev[e], stream) : hipSuccess; };
-    VMX_HIP(hipEventRecord(ev0, stream));
+    // a call of one step (gpu::llg_heun's) is not timed: the two events would cost it more than
+    // its own launches
+    const bool timed = nsteps > 1 || kernel_ms;
+    if (timed) VMX_HIP(hipEventRecord(ev0, stream));
     for (int k = 0; k < nsteps; ++k) {
         if (noise) {
             // the previous step still reads noise_dev: wait, then copy this step's block in
@@ -944,8 +947,8 @@ int vmx_ctx::heun_steps(int nsteps, const double* noise, double* kernel_ms) {
     }
     if (kernel_ms)
         for (int e = 0; e < 6; ++e) (void)hipEventDestroy(kev[e]);
-    VMX_HIP(hipEventRecord(ev1, stream));
-    timed_steps = nsteps;
+    if (timed) VMX_HIP(hipEventRecord(ev1, stream));
+    timed_steps = timed ? nsteps : 0;
     step += (uint64_t)nsteps;   // *step_base stays: the next eager call's offset grows instead
     if (dip.on) dip.time += nsteps;
     return VMX_OK;
@@ -965,7 +968,8 @@ int vmx_ctx::midpoint_steps(int nsteps, const double* noise) {
         VMX_TRY(noise_dev.alloc(3 * (size_t)n * sizeof(double)));
         nd = noise_dev.as<double>();
     }
-    VMX_HIP(hipEventRecord(ev0, stream));
+    const bool timed = nsteps > 1;   // as the Heun step: a one-step call is not timed
+    if (timed) VMX_HIP(hipEventRecord(ev0, stream));
     for (int k = 0; k < nsteps; ++k) {
         if (noise) {
             noise_host.resize(3 * (size_t)n);
@@ -983,8 +987,8 @@ int vmx_ctx::midpoint_steps(int nsteps, const double* noise) {
                                       nd, off0 + (uint32_t)k));
         if (dip.on && (dip.time + k + 1) % dip.rate == 0) VMX_TRY(dipole_refresh<R>());
     }
-    VMX_HIP(hipEventRecord(ev1, stream));
-    timed_steps = nsteps;
+    if (timed) VMX_HIP(hipEventRecord(ev1, stream));
+    timed_steps = timed ? nsteps : 0;
     step += (uint64_t)nsteps;   // *step_base stays: the next eager call's offset grows instead
     if (dip.on) dip.time += nsteps;
     return VMX_OK;
