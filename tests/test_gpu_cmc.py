@@ -153,6 +153,32 @@ def test_cmc_replicas_equal_standalone(gpu):
             assert np.array_equal(c.cmc_counts()[0], gcounts[r])
 
 
+def test_cmc_host_and_device_permutations_agree(gpu):
+    """A context of few replicas forms each sweep's permutation table on the host (up to 1 KiB);
+    a larger batch forms it on the device (k_cmc_perm).  A batch of 24 SC 4^3 replicas (two
+    colours: 96 entries, 1.5 KiB) equals each replica run alone (4 entries), bit for bit."""
+    systems = [S.ferromagnet("sc", 4) for _ in range(24)]
+    batch = S.concat_systems(systems)
+    spins = [S.random_spins(x.n, 70 + r) for r, x in enumerate(systems)]
+    with Context(batch, precision=FP64, seed=40) as c:
+        for r in range(len(systems)):
+            c.set_conditions(150.0 + 20.0 * r, replica=r)
+        c.cmc_set_constraint(35.0, 80.0)
+        c.set_spins(np.concatenate(spins, axis=1))
+        c.cmc_sweep(6)
+        got, gcounts = c.get_spins(), c.cmc_counts()
+    off = batch.replica_offset
+    for r in (0, 7, 23):
+        with Context(systems[r], precision=FP64, seed=40 + r) as c:
+            c.set_conditions(150.0 + 20.0 * r)
+            c.cmc_set_constraint(35.0, 80.0)
+            c.set_spins(spins[r])
+            c.cmc_sweep(6)
+            assert np.array_equal(c.get_spins(), got[:, off[r]:off[r + 1]])
+            assert np.array_equal(c.cmc_counts()[0], gcounts[r])
+    assert gcounts[:, 0].sum() > 0
+
+
 @pytest.mark.parametrize("precision", [FP64, FP32])
 def test_cmc_curve_matches_reference(gpu, precision):
     """Statistical parity with the reference's montecarlo::cmc_step on SC 8^3 with uniaxial

@@ -483,6 +483,7 @@ struct vmx_ctx {
 
     // device state
     DevBuf row, col, jidx, jtab, type, mat, cond, blk, blk_fused, rep_off, rep_key, step_base;
+    std::vector<uint2> rep_key_h;    // the replicas' Philox keys (seed + r)
     DevBuf deg, pat, pat_off, pat_jid, pat_deg;
     DevBuf rep_m;                 // [nrep] replica size / q
     DevBuf rep_blk;               // [nrep+1] each replica's first entry of the one-atom block table
@@ -559,6 +560,7 @@ struct vmx_ctx {
     // constrained Monte Carlo: per colour, per replica, the colour's atoms in caller order (the
     // pairing follows that order); workgroup table {first entry, replica, run length, first pair}
     DevBuf cmc_list, cmc_tab, cmc_frame, cmc_M, cmc_part, cmc_counts, cmc_dm, cmc_range, cmc_mlen, cmc_perm;
+    std::vector<int> cmc_mlen_h;           // [colour][replica] run lengths
     std::vector<int> cmc_tab_start;        // colour k's workgroups: cmc_tab[cmc_tab_start[k] ...]
                                            // cmc_range[k * (nrep + 1) + r]: replica r's first of them
     std::vector<double> cmc_frame_h;       // [nrep][21]: polar matrix, transpose, constraint direction
@@ -567,6 +569,7 @@ struct vmx_ctx {
     int cmc_move = 0;                      // 0: cone of the replica's Metropolis width; 1: angle move
     template <typename R>
     int cmc_sweeps(int nsweeps);
+    int cmc_permutations(uint64_t sw);
     int colour_order(const std::vector<int>& c, const std::vector<int>& lau, int nc);
     template <typename R>
     SysDev<R> sys_colour_order() const;
@@ -1219,7 +1222,9 @@ int vmx_ctx::colour() {
     if (tp == TP_PAT) VMX_TRY(colour_order(c, lau, nc));
     // constrained Monte Carlo: the same colours, each replica's run in caller order, workgroups of
     // 256 slots per run (m/2 slots for a run of even length m, m for odd m)
-    std::vector<int> clist, crange((size_t)nc * (nrep + 1)), cmlen((size_t)nc * nrep);
+    std::vector<int> clist, crange((size_t)nc * (nrep + 1));
+    std::vector<int>& cmlen = cmc_mlen_h;
+    cmlen.assign((size_t)nc * nrep, 0);
     std::vector<int4> ctab;
     clist.reserve(n);
     cmc_tab_start.assign(nc + 1, 0);
@@ -1388,6 +1393,33 @@ int vmx_ctx::mc_sweeps(int nsweeps) {
     return VMX_OK;
 }
 
+// The permutations of every (colour, round, replica) of sweep sw into cmc_perm: for a table of up
+// to 1 KiB (a few replicas) formed here with the device's own integer arithmetic and written by a
+// one-workgroup launch whose arguments carry it; else by k_cmc_perm, one thread per entry.  The
+// device kernel is a chain of dependent divisions (≈ 10 µs), longer than a small system's round.
+int vmx_ctx::cmc_permutations(uint64_t sw) {
+    const size_t count = 2 * (size_t)ncolours * nrep;
+    if (count * sizeof(int4) > sizeof(ArgWords)) {
+        VMX_HIP(Ops<float>::cmc_perm(stream, rep_key.as<uint2>(), nrep, ncolours, cmc_mlen.as<int>(), sw,
+                                     cmc_perm.as<int4>()));
+        return VMX_OK;
+    }
+    ArgWords v;
+    int4* p = reinterpret_cast<int4*>(v.w);
+    for (size_t g = 0; g < count; ++g) {
+        const int r = (int)(g % nrep), kr = (int)(g / nrep), k = kr >> 1, round = kr & 1;
+        const int m = cmc_mlen_h[(size_t)k * nrep + r];
+        if (m < 2) { p[g] = make_int4(1, 0, 1, 0); continue; }
+        uint64_t a, b;
+        cmc_permutation(rep_key_h[r], k, round, sw, m, a, b);
+        p[g] = make_int4((int)a, (int)b, (int)mod_inverse((uint32_t)a, (uint32_t)m), 0);
+    }
+    hipLaunchKernelGGL(k_put_words, dim3(1), dim3(kArgWords), 0, stream, cmc_perm.as<uint32_t>(), v,
+                       (int)(count * sizeof(int4) / 4));
+    VMX_HIP(hipGetLastError());
+    return VMX_OK;
+}
+
 // Constrained Monte Carlo sweeps (montecarlo::cmc_step, src/montecarlo/cmc.cpp:282-497): per
 // colour two rounds.  Every replica's magnetisation is summed over all spins at the start of each
 // sweep (as cmc_step does, cmc.cpp:346-351) and brought forward after each round by the sum of
@@ -1415,7 +1447,7 @@ int vmx_ctx::cmc_sweeps(int nsweeps) {
         if (cmc_tab_start[k + 1] > cmc_tab_start[k]) last = k;
     for (int w = 0; w < nsweeps; ++w) {
         if (last < 0) { ++sweep; continue; }
-        VMX_HIP(Ops<R>::cmc_perm(stream, s, nrep, ncolours, cmc_mlen.as<int>(), sweep, cmc_perm.as<int4>()));
+        VMX_TRY(cmc_permutations(sweep));
         VMX_HIP(Ops<R>::replica_moment(stream, s, S, mm.as<double>(), rep_blk.as<int>(), nrep, cmc_part.as<double>(),
                                        cmc_M.as<double>()));
         for (int k = 0; k <= last; ++k) {
@@ -2028,7 +2060,8 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
         for (int r = 0; r < nrep; ++r) rep_m[r] = (roff[r + 1] - roff[r]) / q;
         VMX_TRY(upload(c->rep_m, rep_m));
     }
-    std::vector<uint2> keys(nrep);
+    std::vector<uint2>& keys = c->rep_key_h;
+    keys.resize(nrep);
     for (int r = 0; r < nrep; ++r) {
         const uint64_t k = opt->seed + (uint64_t)r;
         keys[r] = make_uint2((uint32_t)k, (uint32_t)(k >> 32));

@@ -141,8 +141,8 @@ struct FitPrm {
 // 1, 2, 3", SC'11): 10 rounds of the 4x32 S-box with Weyl key schedule ---------------------
 struct U4 { uint32_t v[4]; };
 
-__device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                            uint32_t k0, uint32_t k1) {
+__host__ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                                     uint32_t k0, uint32_t k1) {
     const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
     const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
 #pragma unroll
@@ -1559,8 +1559,8 @@ struct CmcFrame {            // cmc::polar_matrix, its transpose, cmc::polar_vec
 
 // The round's permutation: from Philox (colour | round << 16, sweep, STREAM_CMC_PAIR), b = word 1
 // mod m, a = 1 + word 0 mod (m - 1), moved up (1 after m - 1) to the first value coprime with m.
-__device__ __forceinline__ void cmc_permutation(uint2 key, int colour, int round, uint64_t sweep, int m,
-                                                uint64_t& a, uint64_t& b) {
+__host__ __device__ __forceinline__ void cmc_permutation(uint2 key, int colour, int round, uint64_t sweep, int m,
+                                                         uint64_t& a, uint64_t& b) {
     const U4 w = philox4x32_10((uint32_t)colour | ((uint32_t)round << 16), (uint32_t)sweep, (uint32_t)(sweep >> 32),
                                STREAM_CMC_PAIR, key.x, key.y);
     b = (uint64_t)(w.v[1] % (uint32_t)m);
@@ -1576,7 +1576,7 @@ __device__ __forceinline__ void cmc_permutation(uint2 key, int colour, int round
 
 // a^-1 mod m for 0 < a < m coprime with m < 2^31 (extended Euclid; remainders stay below m and
 // the coefficients below m in magnitude, so 32-bit arithmetic holds them)
-__device__ __forceinline__ uint32_t mod_inverse(uint32_t a, uint32_t m) {
+__host__ __device__ __forceinline__ uint32_t mod_inverse(uint32_t a, uint32_t m) {
     int32_t t = 0, nt = 1;
     uint32_t r = m, nr = a;
     while (nr != 0u) {

@@ -67,7 +67,7 @@ struct Ops {
                                 double* counts, double* dM);
     // the permutations of every (colour, round, replica) of a sweep: perm[(2 k + round) nrep + r] =
     // {a, b, a^-1 mod m, 0} for the run length m = mlen[k nrep + r] (m < 2: {1, 0, 1, 0})
-    static hipError_t cmc_perm(hipStream_t st, const SysDev<R>& s, int nrep, int ncolours, const int* mlen,
+    static hipError_t cmc_perm(hipStream_t st, const uint2* rep_key, int nrep, int ncolours, const int* mlen,
                                uint64_t sweep, int4* perm);
     // each replica's sum(mu s) (double [nrep][3]): per-workgroup partials over the one-atom block
     // table, then one workgroup per replica over its partials (rep_blk[r] .. rep_blk[r+1])
