@@ -270,6 +270,29 @@ def test_eager_and_graph_interleaved_equal_eager(gpu):
         assert np.array_equal(res[0], res[1])
 
 
+def test_repeated_and_changed_conditions_match_oracle(gpu):
+    """gpu::llg_heun sets the conditions before every one-step call.  Repeated values write
+    nothing (the device table already holds them), changed ones are written: a per-step sequence of
+    temperatures and fields with repeats, on/off thermal flags included, equals the oracle stepped
+    with the same sequence (FP64, Philox noise at the same step counters, <= 1e-12)."""
+    s = S.ferromagnet("fcc", 5)
+    spins = S.random_spins(s.n, 12)
+    seq = [(300.0, (0.0, 0.0, 1.0), True), (300.0, (0.0, 0.0, 1.0), True), (700.0, (0.0, 0.0, 1.0), True),
+           (700.0, (0.0, 0.0, 1.0), True), (700.0, (0.0, 0.5, 1.0), True), (700.0, (0.0, 0.5, 1.0), False),
+           (700.0, (0.0, 0.5, 1.0), False), (200.0, (0.0, 0.5, 1.0), True), (200.0, (0.0, 0.5, 1.0), True)]
+    o = O.Oracle(s)
+    ref = spins
+    for k, (T, H, th) in enumerate(seq):
+        o.set_conditions(T, H, thermal=th)
+        ref = o.llg_heun(ref, 1, seed=21, step0=k)
+    with _ctx(s, seed=21) as c:
+        c.set_spins(spins)
+        for T, H, th in seq:
+            c.set_conditions(T, H, thermal=th)
+            c.llg_heun(1)
+        assert np.abs(c.get_spins() - ref).max() <= 1e-12
+
+
 def test_step_timing_of_calls(gpu):
     """Calls of two or more steps and graph replays are timed with events on the stream; a call of
     one step is not (its events would cost more than its launches) and reports 0, not the timing
