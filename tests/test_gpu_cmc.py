@@ -180,6 +180,41 @@ def test_cmc_host_and_device_permutations_agree(gpu):
 
 
 @pytest.mark.parametrize("precision", [FP64, FP32])
+@pytest.mark.parametrize("case", ["sc_batch", "fcc", "film_dmi", "bcc_rotated"])
+def test_cmc_colour_order_equals_storage_order(gpu, case, precision):
+    """Calls of several constrained sweeps run on the colour-ordered copy of the spins (pair list,
+    moments and tables in colour order, Philox counters from the caller's indices, the
+    magnetisation summed in storage order); single sweeps run in storage order.  Eight single
+    sweeps and one call of eight give the same spins and outcome counts bit for bit: replicas of
+    odd and even size, FCC and BCC stored basis-major, two materials with DMI tensors."""
+    if case == "sc_batch":
+        s = S.concat_systems([S.ferromagnet("sc", L) for L in (5, 4, 7, 6)])
+    elif case == "fcc":
+        s = S.ferromagnet("fcc", 4)
+    elif case == "film_dmi":
+        s = RC.system_of("film_dmi")
+    else:
+        s = RC.system_of("bcc4_rotated")
+    spins = S.random_spins(s.n, 18)
+    spins[2] = np.abs(spins[2]) + 0.3
+    spins /= np.linalg.norm(spins, axis=0)
+    out = []
+    for calls in ([1] * 8, [8]):
+        with Context(s, precision=precision, seed=6, topology="stencil") as c:
+            for r in range(s.num_replicas):
+                c.set_conditions(200.0 + 100.0 * r, (0.0, 0.0, 0.3), replica=r)
+            c.mc_set_sigma(40.0)
+            c.cmc_set_constraint(30.0, 50.0)
+            c.set_spins(spins)
+            for k in calls:
+                c.cmc_sweep(k)
+            out.append((c.get_spins(), c.cmc_counts()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][1][:, 0].sum() > 0
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32])
 def test_cmc_curve_matches_reference(gpu, precision):
     """Statistical parity with the reference's montecarlo::cmc_step on SC 8^3 with uniaxial
     anisotropy, magnetisation held at phi = 40, theta = 30 degrees: <m.v>(T) and <E>(T) at

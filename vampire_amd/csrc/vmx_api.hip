@@ -561,6 +561,9 @@ struct vmx_ctx {
     // pairing follows that order); workgroup table {first entry, replica, run length, first pair}
     DevBuf cmc_list, cmc_tab, cmc_frame, cmc_M, cmc_part, cmc_counts, cmc_dm, cmc_range, cmc_mlen, cmc_perm;
     std::vector<int> cmc_mlen_h;           // [colour][replica] run lengths
+    DevBuf cmc_list_co, mco_invp, mco_mm;  // colour order: the pair list's atoms, storage -> colour
+                                           // position, moments
+    std::vector<int> mco_invp_h;
     std::vector<int> cmc_tab_start;        // colour k's workgroups: cmc_tab[cmc_tab_start[k] ...]
                                            // cmc_range[k * (nrep + 1) + r]: replica r's first of them
     std::vector<double> cmc_frame_h;       // [nrep][21]: polar matrix, transpose, constraint direction
@@ -1246,6 +1249,11 @@ int vmx_ctx::colour() {
     }
     cmc_tab_start[nc] = (int)ctab.size();
     VMX_TRY(upload(cmc_list, clist));
+    if (mco_ok) {
+        std::vector<int> clist_co(clist.size());
+        for (size_t e = 0; e < clist.size(); ++e) clist_co[e] = mco_invp_h[clist[e]];
+        VMX_TRY(upload(cmc_list_co, clist_co));
+    }
     VMX_TRY(upload(cmc_tab, ctab));
     VMX_TRY(upload(cmc_range, crange));
     VMX_TRY(upload(cmc_mlen, cmlen));
@@ -1321,6 +1329,14 @@ int vmx_ctx::colour_order(const std::vector<int>& c, const std::vector<int>& lau
     }
     mco_blk_start[nc] = (int)tab.size();
     VMX_TRY(upload(mco_perm, perm));
+    VMX_TRY(upload(mco_invp, invp));
+    mco_invp_h = invp;
+    {
+        std::vector<double> mm_s(n), mm_c(n);
+        VMX_HIP(hipMemcpy(mm_s.data(), mm.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+        for (int q2 = 0; q2 < n; ++q2) mm_c[q2] = mm_s[perm[q2]];
+        VMX_TRY(upload(mco_mm, mm_c));
+    }
     VMX_TRY(upload(mco_lidx, lidx));
     VMX_TRY(upload(mco_type, type_c));
     VMX_TRY(upload(mco_pat, patc));
@@ -1445,17 +1461,33 @@ int vmx_ctx::cmc_sweeps(int nsweeps) {
     int last = -1;                                 // the sweep's last colour with pairs
     for (int k = 0; k < ncolours; ++k)
         if (cmc_tab_start[k + 1] > cmc_tab_start[k]) last = k;
+    // Calls of several sweeps run on the colour-ordered copy of the spins, as Metropolis does: each
+    // pair's atoms come from the colour-ordered list, the moments and tables follow, the Philox
+    // counters use the caller's indices, and the magnetisation is summed in storage order through
+    // the inverse permutation, so the results are those of storage order bit for bit.
+    const bool ordered = mco_ok && !dip.on && nsweeps >= kMcOrderedMinSweeps && last >= 0;
+    const SysDev<R> so = ordered ? sys_colour_order<R>() : s;
+    Spin3<R> C = S;
+    if (ordered) {
+        R* base = mco_spin.as<R>();
+        C = Spin3<R>{base, base + n, base + 2 * (size_t)n};
+        VMX_HIP(Ops<R>::mc_permute(stream, n, mco_perm.as<int>(), S, C, true));
+    }
+    const int* plist = ordered ? cmc_list_co.as<int>() : cmc_list.as<int>();
+    const int* plidx = ordered ? mco_lidx.as<int>() : nullptr;
+    const double* pmm = ordered ? mco_mm.as<double>() : mm.as<double>();
+    const int* gather = ordered ? mco_invp.as<int>() : nullptr;
     for (int w = 0; w < nsweeps; ++w) {
         if (last < 0) { ++sweep; continue; }
         VMX_TRY(cmc_permutations(sweep));
-        VMX_HIP(Ops<R>::replica_moment(stream, s, S, mm.as<double>(), rep_blk.as<int>(), nrep, cmc_part.as<double>(),
-                                       cmc_M.as<double>()));
+        VMX_HIP(Ops<R>::replica_moment(stream, s, C, mm.as<double>(), gather, rep_blk.as<int>(), nrep,
+                                       cmc_part.as<double>(), cmc_M.as<double>()));
         for (int k = 0; k <= last; ++k) {
             const int ntab = cmc_tab_start[k + 1] - cmc_tab_start[k];
             if (ntab == 0) continue;
             double* dm = cmc_dm.as<double>() + 3 * (size_t)cmc_tab_start[k];
             for (int round = 0; round < 2; ++round) {
-                VMX_HIP(Ops<R>::cmc_pairs(stream, et, am(), s, S, mm.as<double>(), cmc_list.as<int>(),
+                VMX_HIP(Ops<R>::cmc_pairs(stream, et, am(), so, C, pmm, plist, plidx,
                                           cmc_tab.as<int4>() + cmc_tab_start[k], ntab,
                                           cmc_perm.as<int4>() + (size_t)(2 * k + round) * nrep,
                                           cmc_frame.as<CmcFrame<R>>(), cmc_M.as<double>(), mc_sigma.as<double>(),
@@ -1467,6 +1499,7 @@ int vmx_ctx::cmc_sweeps(int nsweeps) {
         }
         ++sweep;   // a constrained sweep advances sim::time like a Heun step (sim.cpp:623-628)
     }
+    if (ordered) VMX_HIP(Ops<R>::mc_permute(stream, n, mco_perm.as<int>(), C, S, false));
     return VMX_OK;
 }
 

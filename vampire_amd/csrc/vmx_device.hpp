@@ -1664,7 +1664,9 @@ __device__ __forceinline__ double* cmc_wg_dm() {
 // replica, run length m, first slot}; a workgroup (256 threads) takes 256 slots (cmc_slot) of one
 // run; perm[rep] = {a, b, a^-1 mod m} of the round's permutation (k_cmc_perm).
 // M: each replica's magnetisation sum(mu s) at the start of the round (double [nrep][3]);
-// mm: moments (mu_B); sigma: cone width per replica (move 0); counts: kRejectSlots slots of three
+// mm: moments (mu_B); lidx: with the spins in colour order (list, mm and the system's tables in
+// colour order too), each atom's caller index within its replica, the Philox counter (else null:
+// computed from the storage index); sigma: cone width per replica (move 0); counts: kRejectSlots slots of three
 // outcome counts per replica, each slot on its own cache line; dM[3 lb ..]: the workgroup's
 // sum of mu (s_new - s_old) over its accepted pairs, summed in a fixed order (lanes by a
 // shuffle tree, then the four wavefronts in order), so that the next round's M follows
@@ -1672,7 +1674,7 @@ __device__ __forceinline__ double* cmc_wg_dm() {
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(256)
 k_cmc_pairs(SysDev<R> s, Spin3<R> S, const double* __restrict__ mm, const int* __restrict__ list,
-            const int4* __restrict__ tab, int ntab, const int4* __restrict__ perm,
+            const int* __restrict__ lidx, const int4* __restrict__ tab, int ntab, const int4* __restrict__ perm,
             const CmcFrame<R>* __restrict__ frame, const double* __restrict__ M, const double* __restrict__ sigma,
             int move, int round, uint64_t sweep, double* __restrict__ counts, double* __restrict__ dM) {
     const int lb = logical_block(s.xcd_remap);
@@ -1691,7 +1693,7 @@ k_cmc_pairs(SysDev<R> s, Spin3<R> S, const double* __restrict__ mm, const int* _
     const double mu1 = mm[i1], mu2 = mm[i2];
     int h1 = 0, h2 = 0;
     if (Access<AM>::TP != TP_CSR) { h1 = row_head<R, AM>(s, i1); h2 = row_head<R, AM>(s, i2); }
-    const int local1 = caller_local(s, rep, i1 - s.rep_off[rep]);
+    const int local1 = lidx ? lidx[i1] : caller_local(s, rep, i1 - s.rep_off[rep]);
     const U4 um = philox4x32_10((uint32_t)local1, (uint32_t)sweep, (uint32_t)(sweep >> 32),
                                 STREAM_CMC_MOVE + (uint32_t)round, key.x, key.y);
     R g0, g1, g2;

@@ -61,8 +61,10 @@ struct Ops {
                                 const uint16_t* pat, const int2* wg, int wg0, int nwg, const double* sigma,
                                 uint64_t sweep, double* rejects);
     // constrained Monte Carlo: one round of one colour (tab[ntab] workgroups of up to 256 pairs)
+    // lidx: each atom's caller index within its replica when the spins are in colour order (else null)
     static hipError_t cmc_pairs(hipStream_t st, int et, int jm, const SysDev<R>& s, Spin3<R> S, const double* mm,
-                                const int* list, const int4* tab, int ntab, const int4* perm, const CmcFrame<R>* frame,
+                                const int* list, const int* lidx, const int4* tab, int ntab, const int4* perm,
+                                const CmcFrame<R>* frame,
                                 const double* M, const double* sigma, int move, int round, uint64_t sweep,
                                 double* counts, double* dM);
     // the permutations of every (colour, round, replica) of a sweep: perm[(2 k + round) nrep + r] =
@@ -71,8 +73,9 @@ struct Ops {
                                uint64_t sweep, int4* perm);
     // each replica's sum(mu s) (double [nrep][3]): per-workgroup partials over the one-atom block
     // table, then one workgroup per replica over its partials (rep_blk[r] .. rep_blk[r+1])
+    // (gather: spins in colour order, atom i's at gather[i]; else null)
     static hipError_t replica_moment(hipStream_t st, const SysDev<R>& s, Spin3<R> S, const double* mm,
-                                     const int* rep_blk, int nrep, double* partial, double* M);
+                                     const int* gather, const int* rep_blk, int nrep, double* partial, double* M);
     // M[r] += the sum of the cmc_pairs partials dM[range[r] .. range[r+1]], in a fixed order
     static hipError_t moment_add(hipStream_t st, const int* range, int nrep, const double* dM, double* M);
     // deterministic fitting step and its adjoint (isotropic uniform exchange); grids of 256
