@@ -499,6 +499,11 @@ struct vmx_ctx {
     int store_field = 0;          // fused step: predictor keeps H_ext for the corrector
     DevBuf field_buf;             // 3N R
     DevBuf mm, atom_rep;
+    DevBuf mm_tab;                   // [nmat] each material's moment, when every atom has its material's
+    // the moments the statistics read: the per-material table when it describes every atom
+    Moments moments() const {
+        return Moments{mm.as<double>(), type.as<uint8_t>(), mm_tab.p ? mm_tab.as<double>() : nullptr};
+    }
     DevBuf staging;               // 3N doubles for host<->device spin copies and noise
     DevBuf noise_dev;             // 3N doubles
     // FP32 contexts: draw the thermal normals with the FP64 kernels' Box-Muller into noise_dev
@@ -1058,7 +1063,7 @@ int vmx_ctx::stats_update(double* fixed) {
     }
     for (MagStat* st : mags) {
         const int grid = stats_grid(st->nbins);
-        VMX_HIP(Ops<R>::mag_partial(grid, stream, n, S, st->mask.as<int>(), mm.as<double>(), st->nbins, part));
+        VMX_HIP(Ops<R>::mag_partial(grid, stream, n, S, st->mask.as<int>(), moments(), st->nbins, part));
         hipLaunchKernelGGL(k_mag_finish, dim3(1), dim3(256), 0, stream, grid, part, st->nbins,
                            st->cur.as<double>(), st->mean.as<double>(), st->empty_dev.as<int>(),
                            (int)st->empty.size(), st->counter.as<long long>());
@@ -1480,7 +1485,7 @@ int vmx_ctx::cmc_sweeps(int nsweeps) {
     for (int w = 0; w < nsweeps; ++w) {
         if (last < 0) { ++sweep; continue; }
         VMX_TRY(cmc_permutations(sweep));
-        VMX_HIP(Ops<R>::replica_moment(stream, s, C, mm.as<double>(), gather, rep_blk.as<int>(), nrep,
+        VMX_HIP(Ops<R>::replica_moment(stream, s, C, moments(), gather, rep_blk.as<int>(), nrep,
                                        cmc_part.as<double>(), cmc_M.as<double>()));
         for (int k = 0; k <= last; ++k) {
             const int ntab = cmc_tab_start[k + 1] - cmc_tab_start[k];
@@ -2081,6 +2086,17 @@ int vmx_create(const vmx_system* sys, const vmx_options* opt, vmx_ctx** out) {
     VMX_HIP(hipMemcpy(c->type.p, t8.data(), n, hipMemcpyHostToDevice));
     VMX_TRY(c->mm.alloc((size_t)n * sizeof(double)));
     VMX_HIP(hipMemcpy(c->mm.p, mom_int.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    {   // every atom's moment its material's (bit for bit)?  then the statistics read the table
+        std::vector<double> tab(c->nmat, 0.0);
+        std::vector<char> seen(c->nmat, 0);
+        bool by_type = true;
+        for (int i = 0; i < n && by_type; ++i) {
+            const int t = type_int[i];
+            if (!seen[t]) { seen[t] = 1; tab[t] = mom_int[i]; }
+            else by_type = std::memcmp(&tab[t], &mom_int[i], sizeof(double)) == 0;
+        }
+        if (by_type) VMX_TRY(upload(c->mm_tab, tab));
+    }
     VMX_TRY(c->rep_off.alloc((nrep + 1) * sizeof(int)));
     VMX_HIP(hipMemcpy(c->rep_off.p, roff.data(), (nrep + 1) * sizeof(int), hipMemcpyHostToDevice));
     if (nrep > 1) {   // replica of every atom (the energy statistics' applied field); replicas keep

@@ -1094,6 +1094,16 @@ __device__ __forceinline__ void reduce_hist_regs(const double (&acc)[MAXB][4], i
     }
 }
 
+// Each atom's moment (mu_B) for the statistics: per atom, or, when every atom's moment is its
+// material's (as VAMPIRE builds them), the material's from a small table through the 1-byte
+// material id: the same doubles with 1 byte read per atom instead of 8.
+struct Moments {
+    const double* __restrict__ mm;
+    const uint8_t* __restrict__ type;
+    const double* __restrict__ tab;
+    __device__ __forceinline__ double at(int i) const { return tab ? tab[type[i]] : mm[i]; }
+};
+
 // Masks of at most MAXB bins: each thread keeps every bin's four sums in registers and adds each
 // atom to its bin with predicated adds, so the cost does not depend on how often the id changes
 // along the atoms (a random alloy's material mask changes at nearly every atom).  The workgroup
@@ -1101,14 +1111,14 @@ __device__ __forceinline__ void reduce_hist_regs(const double (&acc)[MAXB][4], i
 // order) into partial[block][nbins*4], the layout k_mag_partial writes.
 template <typename R, int MAXB>
 __global__ void __launch_bounds__(256)
-k_mag_partial_regs(int n, Spin3<R> S, const int* __restrict__ mask, const double* __restrict__ mm,
-                   int nbins, double* __restrict__ partial) {
+k_mag_partial_regs(int n, Spin3<R> S, const int* __restrict__ mask, Moments mo, int nbins,
+                   double* __restrict__ partial) {
     double acc[MAXB][4];
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) { acc[b][0] = 0.0; acc[b][1] = 0.0; acc[b][2] = 0.0; acc[b][3] = 0.0; }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int id = mask[i];
-        const double m = mm[i];
+        const int id = MAXB == 1 ? 0 : mask[i];   // a one-bin mask holds 0 everywhere: not read
+        const double m = mo.at(i);
         const double v0 = (double)S.x[i] * m, v1 = (double)S.y[i] * m, v2 = (double)S.z[i] * m;
 #pragma unroll
         for (int b = 0; b < MAXB; ++b) {
@@ -1125,8 +1135,8 @@ k_mag_partial_regs(int n, Spin3<R> S, const int* __restrict__ mask, const double
 // W = 4: sum(m s_x), sum(m s_y), sum(m s_z), sum(m)   (magnetization.cpp:141-147)
 template <typename R>
 __global__ void __launch_bounds__(256)
-k_mag_partial(int n, Spin3<R> S, const int* __restrict__ mask, const double* __restrict__ mm,
-              int nbins, double* __restrict__ partial) {
+k_mag_partial(int n, Spin3<R> S, const int* __restrict__ mask, Moments mo, int nbins,
+              double* __restrict__ partial) {
     extern __shared__ double hist[];
     for (int k = threadIdx.x; k < 4 * nbins; k += blockDim.x) hist[k] = 0.0;
     __syncthreads();
@@ -1141,7 +1151,7 @@ k_mag_partial(int n, Spin3<R> S, const int* __restrict__ mask, const double* __r
             }
             cur = id; a0 = a1 = a2 = a3 = 0.0;
         }
-        const double m = mm[i];
+        const double m = mo.at(i);
         a0 += (double)S.x[i] * m;
         a1 += (double)S.y[i] * m;
         a2 += (double)S.z[i] * m;

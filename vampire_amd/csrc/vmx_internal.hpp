@@ -43,7 +43,7 @@ struct Ops {
                                       Spin3<R> F, Spin3<R> E, Spin3<R> D);
     // statistics partial sums (grid-stride, `grid` workgroups of 256)
     static hipError_t mag_partial(int grid, hipStream_t st, int n, Spin3<R> S, const int* mask,
-                                  const double* mm, int nbins, double* partial);
+                                  Moments mo, int nbins, double* partial);
     static hipError_t energy_partial(int grid, hipStream_t st, int et, int jm, const SysDev<R>& s, Spin3<R> S,
                                      const int* mask, const double* mm, const int* atom_rep, int nbins,
                                      double* partial);
@@ -74,7 +74,7 @@ struct Ops {
     // each replica's sum(mu s) (double [nrep][3]): per-workgroup partials over the one-atom block
     // table, then one workgroup per replica over its partials (rep_blk[r] .. rep_blk[r+1])
     // (gather: spins in colour order, atom i's at gather[i]; else null)
-    static hipError_t replica_moment(hipStream_t st, const SysDev<R>& s, Spin3<R> S, const double* mm,
+    static hipError_t replica_moment(hipStream_t st, const SysDev<R>& s, Spin3<R> S, Moments mo,
                                      const int* gather, const int* rep_blk, int nrep, double* partial, double* M);
     // M[r] += the sum of the cmc_pairs partials dM[range[r] .. range[r+1]], in a fixed order
     static hipError_t moment_add(hipStream_t st, const int* range, int nrep, const double* dM, double* M);
