@@ -1073,7 +1073,7 @@ int vmx_ctx::stats_update(double* fixed) {
         const SysDev<R> s = sys<R>();
         for (EnergyStat* st : energies) {
             const int grid = stats_grid(st->nbins);
-            VMX_HIP(Ops<R>::energy_partial(grid, stream, et, am(), s, S, st->mask.as<int>(), mm.as<double>(),
+            VMX_HIP(Ops<R>::energy_partial(grid, stream, et, am(), s, S, st->mask.as<int>(), moments(),
                                            nrep > 1 ? atom_rep.as<int>() : nullptr, st->nbins, part));
             hipLaunchKernelGGL(k_energy_finish, dim3(1), dim3(256), 0, stream, grid, part, st->nbins,
                                st->cur.as<double>(), st->mean.as<double>(), st->empty_dev.as<int>(),

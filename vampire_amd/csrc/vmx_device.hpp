@@ -1281,7 +1281,7 @@ __device__ __forceinline__ R magnetostatic_energy(const SysDev<R>& s, int i, R s
 // last is zero without a device dipole field
 template <typename R, int ET, int AM>
 __global__ void __launch_bounds__(256)
-k_energy_partial(SysDev<R> s, Spin3<R> S, const int* __restrict__ mask, const double* __restrict__ mm,
+k_energy_partial(SysDev<R> s, Spin3<R> S, const int* __restrict__ mask, Moments mo,
                  const int* __restrict__ atom_rep, int nbins, double* __restrict__ partial) {
     if (nbins <= 4) {                // few bins: register histograms, as k_mag_partial_regs
         double acc[4][4];
@@ -1292,7 +1292,7 @@ k_energy_partial(SysDev<R> s, Spin3<R> S, const int* __restrict__ mask, const do
             const R sx = S.x[i], sy = S.y[i], sz = S.z[i];
             const int m = s.type[i];
             const int rep = atom_rep ? atom_rep[i] : 0;
-            const double w = mm[i];
+            const double w = mo.at(i);
             const double e0 = (double)exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, sx, sy, sz) * w;
             const double e1 = (double)anisotropy_energy(s, m, sx, sy, sz) * w;
             const double e2 = (double)applied_energy(s.cond[rep * s.nmat + m], sx, sy, sz) * w;
@@ -1326,7 +1326,7 @@ k_energy_partial(SysDev<R> s, Spin3<R> S, const int* __restrict__ mask, const do
         const R sx = S.x[i], sy = S.y[i], sz = S.z[i];
         const int m = s.type[i];
         const int rep = atom_rep ? atom_rep[i] : 0;
-        const double w = mm[i];
+        const double w = mo.at(i);
         a0 += (double)exchange_energy<R, ET, AM>(s, S.x, S.y, S.z, i, sx, sy, sz) * w;
         a1 += (double)anisotropy_energy(s, m, sx, sy, sz) * w;
         a2 += (double)applied_energy(s.cond[rep * s.nmat + m], sx, sy, sz) * w;

@@ -45,7 +45,7 @@ struct Ops {
     static hipError_t mag_partial(int grid, hipStream_t st, int n, Spin3<R> S, const int* mask,
                                   Moments mo, int nbins, double* partial);
     static hipError_t energy_partial(int grid, hipStream_t st, int et, int jm, const SysDev<R>& s, Spin3<R> S,
-                                     const int* mask, const double* mm, const int* atom_rep, int nbins,
+                                     const int* mask, Moments mo, const int* atom_rep, int nbins,
                                      double* partial);
     // Monte Carlo: one launch of a sweep, workgroups wg0 .. wg0 + nwg - 1; workgroup w takes
     // entries [w * 256 * McAtoms, ...) of atom (and pat, the stencil ids) and wg[w] = {replica,
