@@ -1116,10 +1116,12 @@ k_mag_partial_regs(int n, Spin3<R> S, const int* __restrict__ mask, Moments mo, 
     double acc[MAXB][4];
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) { acc[b][0] = 0.0; acc[b][1] = 0.0; acc[b][2] = 0.0; acc[b][3] = 0.0; }
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int id = MAXB == 1 ? 0 : mask[i];   // a one-bin mask holds 0 everywhere: not read
-        const double m = mo.at(i);
-        const double v0 = (double)S.x[i] * m, v1 = (double)S.y[i] * m, v2 = (double)S.z[i] * m;
+    // with up to four bins, four of the thread's atoms are loaded before any is added (then added
+    // in the same order as one at a time), so each wavefront has four atoms' loads in flight; with
+    // eight the histogram's 64 registers leave no room, and atoms go one at a time
+    constexpr int U = MAXB >= 8 ? 1 : 4;
+    auto add = [&](int id, double m, R x, R y, R z) {
+        const double v0 = (double)x * m, v1 = (double)y * m, v2 = (double)z * m;
 #pragma unroll
         for (int b = 0; b < MAXB; ++b) {
             const bool hit = id == b;
@@ -1128,6 +1130,28 @@ k_mag_partial_regs(int n, Spin3<R> S, const int* __restrict__ mask, Moments mo, 
             acc[b][2] += hit ? v2 : 0.0;
             acc[b][3] += hit ? m : 0.0;
         }
+    };
+    if constexpr (U == 1) {
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+            add(MAXB == 1 ? 0 : mask[i], mo.at(i), S.x[i], S.y[i], S.z[i]);
+    } else {
+        const int stride = (int)(gridDim.x * blockDim.x);
+        int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+        for (; (long long)i + (U - 1) * (long long)stride < n; i += U * stride) {
+            int id[U];
+            double m[U];
+            R x[U], y[U], z[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = i + u * stride;
+                id[u] = MAXB == 1 ? 0 : mask[k];   // a one-bin mask holds 0 everywhere: not read
+                m[u] = mo.at(k);
+                x[u] = S.x[k]; y[u] = S.y[k]; z[u] = S.z[k];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) add(id[u], m[u], x[u], y[u], z[u]);
+        }
+        for (; i < n; i += stride) add(MAXB == 1 ? 0 : mask[i], mo.at(i), S.x[i], S.y[i], S.z[i]);
     }
     reduce_hist_regs<MAXB>(acc, nbins, partial);
 }
